@@ -1,0 +1,221 @@
+/*
+ * wrk.h — C ABI of the MI355X wavefront path-tracing kernels.
+ *
+ * This is the seam that replaces libWR's OptiX/CUDA layer: the OptiX pipeline
+ * seam createPipeline/launchPipeline/destroyPipeline
+ * (libWR/src/internal/renderer.cpp:18-21, pipeline.cpp:169-190), the CUBIN
+ * kernels "shade" and "compact" fetched by name (renderer.cpp:50-59) and the
+ * OptiX GAS build (scene.cpp:493-550). The C++ wr::Scene / wr::Renderer layer
+ * is the only product caller; tests call it directly through ctypes.
+ *
+ * Conventions: plain C types only; device memory is passed as pointers the
+ * caller allocated (no function here allocates device memory unless its name
+ * says so); `stream` is a hipStream_t passed as void* (NULL = default stream);
+ * every function returns WRK_OK or a negative code, with a message available
+ * from wrk_last_error().
+ */
+#ifndef WRK_H
+#define WRK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    WRK_OK = 0,
+    WRK_ERR_INVALID = -1,   /* bad argument (null pointer, size out of range) */
+    WRK_ERR_HIP = -2,       /* HIP runtime call failed */
+    WRK_ERR_CAPACITY = -3,  /* input exceeds a documented limit */
+};
+
+/* Material kinds: libWR/src/internal/gpu_types.h:11-17 */
+enum {
+    WRK_MAT_LAMBERTIAN = 0,
+    WRK_MAT_EMISSIVE = 1,
+    WRK_MAT_GLASS = 2,
+    WRK_MAT_GGX_REFLECTION = 3,
+    WRK_MAT_GGX_TRANSMISSION = 4,
+};
+
+/* 64-byte material record, field for field the reference MaterialData
+ * (gpu_types.h:28-37): albedo.xyz, emission.xyz, ior, type, roughness,
+ * metallic, 1-based texture ids (0 = none). */
+typedef struct wrk_material {
+    float albedo[4];
+    float emission[4];
+    float ior;
+    uint32_t type;
+    float roughness;
+    float metallic;
+    uint32_t albedoTextureId;
+    uint32_t normalTextureId;
+    uint32_t _pad[2];
+} wrk_material;
+
+/* RGBA8 texture, row-major, row 0 = v in [0, 1/height). */
+typedef struct wrk_texture {
+    uint64_t texels; /* device pointer to width*height*4 bytes, 0 = destroyed */
+    uint32_t width;
+    uint32_t height;
+} wrk_texture;
+
+/* Device view of a finalized scene (the getters of libWR/include/wr/scene.h:72-95). */
+typedef struct wrk_scene_view {
+    uint64_t vertices;       /* float[3V] */
+    uint64_t normals;        /* float[3V], 0 if no mesh supplied normals */
+    uint64_t uvs;            /* float[2V], 0 if no mesh supplied uvs */
+    uint64_t indices;        /* uint32[3T] */
+    uint64_t triMaterialIds; /* uint32[T] */
+    uint64_t materials;      /* wrk_material[numMaterials] */
+    uint64_t textures;       /* wrk_texture[numTextures] or 0 */
+    uint64_t emissiveTris;   /* uint32[numEmissive] */
+    uint64_t emissiveCdf;    /* float[numEmissive] */
+    uint32_t numTris;
+    uint32_t numMaterials;
+    uint32_t numTextures;
+    uint32_t numEmissive;
+    float envRadiance[4];
+} wrk_scene_view;
+
+/* 4-wide BVH over leaf-ordered triangles. Node = 128 bytes: child boxes as
+ * lo.x[4] hi.x[4] lo.y[4] hi.y[4] lo.z[4] hi.z[4], then child[4], then pad[4].
+ * child: bit31 clear -> inner node index; bit31 set -> leaf with
+ * count-1 in bits 27..30 and first leaf-ordered triangle in bits 0..26;
+ * 0xFFFFFFFF -> empty slot (its box is inverted). Triangles: 3 float4 each,
+ * (v0.xyz, original primitive id as bits), (v1.xyz, 0), (v2.xyz, 0). */
+typedef struct wrk_bvh {
+    uint64_t nodes;
+    uint64_t tris;
+    uint32_t numNodes;
+    uint32_t numTris;
+    uint32_t maxDepth;
+    uint32_t _pad;
+    float sceneLo[4];
+    float sceneHi[4];
+} wrk_bvh;
+
+/* Camera as the reference CameraData (gpu_types.h:89-98), basis precomputed
+ * on the host (renderer.cpp:144-156). */
+typedef struct wrk_camera {
+    float position[4];
+    float forward[4];
+    float right[4];
+    float up[4];
+    float tanHalfFovY;
+    float aspect;
+    float focalDistance;
+    float lensRadius;
+} wrk_camera;
+
+/* Render parameters (libWR/include/wr/renderer.h:25-36 RenderParams plus the
+ * host-loop iteration cap of renderer.cpp:213; iterationCap 0 = no cap). */
+typedef struct wrk_render_params {
+    uint32_t width;
+    uint32_t height;
+    uint32_t useNEE;
+    uint32_t maxBounces;
+    float rrStartDepth;
+    float fireflyClamp;
+    uint32_t iterationCap;
+    uint32_t _pad;
+} wrk_render_params;
+
+/* Structure-of-arrays path pool for one wavefront batch (capacity paths).
+ * Replaces the 116-byte RayState + 56-byte HitInfo AoS of gpu_types.h:48-86. */
+typedef struct wrk_paths {
+    uint64_t org;      /* float4: origin.xyz, tMin */
+    uint64_t dir;      /* float4: direction.xyz, tMax */
+    uint64_t thr;      /* float4: throughput.xyz, prevPdf */
+    uint64_t rad;      /* float4: radiance.xyz, - */
+    uint64_t state;    /* uint4: seed, rngDimension, flags, pixelIndex */
+    uint64_t contrib;  /* float4: pending NEE contribution.xyz */
+    uint64_t savedOrg; /* float4: origin of the ray that produced the hit */
+    uint64_t savedDir; /* float4: savedDirection.xyz */
+    uint64_t hit;      /* uint2: t (float bits), primitive id (0xFFFFFFFF = miss) */
+    uint64_t occluded; /* uint8 */
+    uint64_t result;   /* float4: value added to the accumulation buffer */
+    uint64_t sampleIdx;/* uint32: sample index of the path */
+    uint32_t capacity;
+    uint32_t _pad;
+} wrk_paths;
+
+const char* wrk_last_error(void);
+int wrk_device_count(int* count);
+
+/* ---------------------------------------------------------------- BVH --- */
+/* Sizes of the caller-provided buffers for wrk_bvh_build. */
+int wrk_bvh_query_sizes(uint32_t numTris, uint64_t* nodeBytes, uint64_t* triBytes,
+                        uint64_t* scratchBytes);
+/* GPU LBVH: 30-bit Morton codes of triangle centroids, LDS radix sort,
+ * Karras 2012 hierarchy, bottom-up boxes, collapse to 4-wide nodes. Replaces
+ * optixAccelComputeMemoryUsage + optixAccelBuild (scene.cpp:493-550).
+ * Synchronizes `stream` (reads level counts back to the host). */
+int wrk_bvh_build(const float* vertices, const uint32_t* indices, uint32_t numTris,
+                  void* nodes, void* tris, void* scratch, uint64_t scratchBytes,
+                  wrk_bvh* out, void* stream);
+
+/* ---------------------------------------------------------- traversal --- */
+/* Closest hit for rays queue[i] (or i if queue is NULL), i < n (or
+ * i < *countDev when countDev is non-NULL, n then being an upper bound).
+ * Replaces optixTrace in the Trace stage (trace.cu:127-151) together with
+ * __closesthit__/__miss__trace's hit/miss decision. Hits are t in
+ * [tMin, tMax] of the watertight test; ties resolve to the smaller primitive
+ * id. hits[ray] = {t bits, prim}; prim = 0xFFFFFFFF on a miss. */
+int wrk_trace_closest(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue,
+                      uint32_t n, const uint32_t* countDev, uint32_t* hits, void* stream);
+/* Any hit in [tMin, tMax]: occluded[ray] = 1/0. Replaces the Shadow-stage
+ * optixTrace + __closesthit__trace payload (trace.cu:27-51,162-165). */
+int wrk_trace_any(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue,
+                  uint32_t n, const uint32_t* countDev, uint8_t* occluded, void* stream);
+
+/* -------------------------------------------------------- compaction --- */
+/* Scratch bytes for wrk_compact / wrk_partition over n items. */
+uint64_t wrk_partition_scratch_bytes(uint32_t n, uint32_t numCategories);
+/* Stable compaction: out = [in[i] (or i) for i in 0..n if keep[i]] in input
+ * order, *countOut (device) = its length. Replaces compact.cu:6-17, whose
+ * atomic-per-survivor order is nondeterministic; this one is ordered. */
+int wrk_compact(const uint8_t* keep, const uint32_t* in, uint32_t n, uint32_t* out,
+                uint32_t* countOut, void* scratch, uint64_t scratchBytes, void* stream);
+/* Stable K-way partition (K <= 8): items with category k, k < numCategories,
+ * land in out[offsets[k] ...] in input order; categories >= numCategories
+ * are dropped. offsetsOut (device) receives K+1 exclusive offsets. */
+int wrk_partition(const uint8_t* category, const uint32_t* in, uint32_t n, uint32_t numCategories,
+                  uint32_t* out, uint32_t* offsetsOut, void* scratch, uint64_t scratchBytes,
+                  void* stream);
+
+/* ---------------------------------------------------------- wavefront --- */
+/* Paths i < n of a batch start as camera rays for (pixel = (first + i) % npix,
+ * sample = (first + i) / npix) with npix = width*height (trace.cu:53-125). */
+int wrk_generate(const wrk_camera* cam, const wrk_render_params* params, const wrk_paths* paths,
+                 uint64_t first, uint32_t n, uint32_t* queue, void* stream);
+/* Resolves the trace results of the active queue [closest | shadow] and
+ * assigns each path a shade category (trace.cu:27-51 shadow resolve,
+ * trace.cu:143-148 miss flush, gpu_types.h:11-17 material kinds). */
+int wrk_classify(const wrk_scene_view* scene, const wrk_render_params* params, const wrk_paths* paths,
+                 const uint32_t* queue, uint32_t numClosest, uint32_t numShadow, uint8_t* category,
+                 void* stream);
+/* Runs the material shaders over the material-sorted queue (offsets as from
+ * wrk_partition with 5 categories): shade.cu:24-85 and the materials/ headers.
+ * nextStage[i] = 0 trace, 1 shadow, 2 finished, for queue position i.
+ * lastIteration != 0 drops paths that are still alive (renderer.cpp:213). */
+int wrk_shade(const wrk_scene_view* scene, const wrk_render_params* params, const wrk_paths* paths,
+              const uint32_t* queue, const uint32_t* offsets, uint32_t maxItems, uint32_t lastIteration,
+              uint8_t* nextStage, void* stream);
+/* accum[pixel] += result[path] for the batch paths in sample order
+ * (renderer.cpp:190 serial samples; accumulation order is per pixel). */
+int wrk_accumulate(const wrk_paths* paths, uint64_t first, uint32_t n, uint32_t npix, float* accum,
+                   void* stream);
+/* out = accum * (1/spp) (renderer.cpp:308-313). */
+int wrk_average(const float* accum, uint32_t npix, uint32_t spp, float* out, void* stream);
+/* Exposure 2.2, ACES (Narkowicz), gamma 1/2.2, truncation to uint8
+ * (libWR/test/helpers/image.cpp:14-49). */
+int wrk_tonemap_aces(const float* image, uint32_t npix, uint8_t* rgb8, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WRK_H */

@@ -1,0 +1,58 @@
+// wrk_common.h — error plumbing shared by the C-ABI entry points.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "../../include/wrk.h"
+
+namespace wrk {
+
+void set_error(const std::string& msg);
+
+#define WRK_HIP_TRY(expr)                                                                   \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess) {                                                             \
+            ::wrk::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));            \
+            return WRK_ERR_HIP;                                                             \
+        }                                                                                   \
+    } while (0)
+
+#define WRK_CHECK_LAUNCH()                                                                  \
+    do {                                                                                    \
+        hipError_t _e = hipGetLastError();                                                  \
+        if (_e != hipSuccess) {                                                             \
+            ::wrk::set_error(std::string("kernel launch: ") + hipGetErrorString(_e));       \
+            return WRK_ERR_HIP;                                                             \
+        }                                                                                   \
+    } while (0)
+
+#define WRK_REQUIRE(cond, msg)                                                              \
+    do {                                                                                    \
+        if (!(cond)) {                                                                      \
+            ::wrk::set_error(msg);                                                          \
+            return WRK_ERR_INVALID;                                                         \
+        }                                                                                   \
+    } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// BVH node as laid out in HBM (128 bytes, one cache line pair).
+struct alignas(16) Node4 {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    uint4 child;
+    uint4 pad;
+};
+static_assert(sizeof(Node4) == 128, "Node4 must be 128 bytes");
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
+constexpr uint32_t kLeafMax = 4;        // triangles per leaf
+constexpr uint32_t kStackLds = 16;      // traversal stack entries per lane in LDS
+constexpr uint32_t kStackSpill = 176;   // further entries per lane in scratch
+constexpr uint32_t kStackTotal = kStackLds + kStackSpill;
+
+}  // namespace wrk
