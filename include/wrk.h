@@ -137,7 +137,6 @@ typedef struct wrk_paths {
     uint64_t hit;      /* uint2: t (float bits), primitive id (0xFFFFFFFF = miss) */
     uint64_t occluded; /* uint8 */
     uint64_t result;   /* float4: value added to the accumulation buffer */
-    uint64_t sampleIdx;/* uint32: sample index of the path */
     uint32_t capacity;
     uint32_t _pad;
 } wrk_paths;
@@ -171,9 +170,21 @@ int wrk_trace_closest(const wrk_bvh* bvh, const void* org4, const void* dir4, co
 int wrk_trace_any(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue,
                   uint32_t n, const uint32_t* countDev, uint8_t* occluded, void* stream);
 
+/* The same traversals built with fetch counters: fetches[0] += inner nodes
+ * visited, fetches[1] += triangles tested (device uint64 pair). They give the
+ * mean node/triangle fetches per ray that the algorithmic-bytes model of the
+ * traversal kernels needs; results are identical to the uncounted kernels. */
+int wrk_trace_closest_counted(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue,
+                              uint32_t n, const uint32_t* countDev, uint32_t* hits, unsigned long long* fetches,
+                              void* stream);
+int wrk_trace_any_counted(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue,
+                          uint32_t n, const uint32_t* countDev, uint8_t* occluded, unsigned long long* fetches,
+                          void* stream);
+
 /* -------------------------------------------------------- compaction --- */
-/* Scratch bytes for wrk_compact / wrk_partition over n items. */
+/* Scratch bytes for wrk_partition / wrk_compact over n items. */
 uint64_t wrk_partition_scratch_bytes(uint32_t n, uint32_t numCategories);
+uint64_t wrk_compact_scratch_bytes(uint32_t n);
 /* Stable compaction: out = [in[i] (or i) for i in 0..n if keep[i]] in input
  * order, *countOut (device) = its length. Replaces compact.cu:6-17, whose
  * atomic-per-survivor order is nondeterministic; this one is ordered. */

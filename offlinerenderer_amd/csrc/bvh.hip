@@ -1,0 +1,639 @@
+// bvh.hip — GPU LBVH build (Morton codes, radix sort, Karras 2012 hierarchy,
+// bottom-up boxes, collapse to 4-wide nodes) and watertight FP32
+// closest-hit / any-hit traversal for gfx950.
+//
+// Replaces the OptiX GAS build (libWR/src/internal/scene.cpp:493-550) and
+// optixTrace (libWR/kernels/trace.cu:27-51,127-151).
+#include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "wrk_common.h"
+#include "wrk_device.h"
+
+namespace wrk {
+
+// ----------------------------------------------------------------- build
+struct alignas(16) BinNode {
+    float4 lo;   // xyz box, w = left child ref bits
+    float4 hi;   // xyz box, w = right child ref bits
+    uint32_t first, last, pad0, pad1;
+};
+
+enum Counter : uint32_t {
+    C_NODES = 0,      // 4-wide nodes allocated
+    C_TASKS_NEXT = 1, // tasks produced for the next level
+    C_MAXDEPTH = 2,
+    C_CMIN = 4,       // ordered-uint centroid min (3)
+    C_CMAX = 8,       // ordered-uint centroid max (3)
+    C_BMIN = 12,      // ordered-uint scene min (3)
+    C_BMAX = 16,      // ordered-uint scene max (3)
+    C_COUNT = 20
+};
+
+__device__ __forceinline__ uint32_t ordered(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __host__ __forceinline__ float unordered(uint32_t u) {
+    uint32_t b = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+__global__ void k_init_counters(uint32_t* c) {
+    uint32_t i = threadIdx.x;
+    if (i < C_COUNT) {
+        uint32_t v = 0;
+        if ((i >= C_CMIN && i < C_CMIN + 3) || (i >= C_BMIN && i < C_BMIN + 3)) v = 0xFFFFFFFFu;
+        c[i] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tri_bounds(const float* __restrict__ verts, const uint32_t* __restrict__ idx,
+                                                    uint32_t n, float4* triLo, float4* triHi, uint32_t* counters) {
+    uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (t < n) {
+        uint32_t i0 = idx[t * 3 + 0], i1 = idx[t * 3 + 1], i2 = idx[t * 3 + 2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float x0 = verts[i0 * 3 + a], x1 = verts[i1 * 3 + a], x2 = verts[i2 * 3 + a];
+            lo[a] = fminf(x0, fminf(x1, x2));
+            hi[a] = fmaxf(x0, fmaxf(x1, x2));
+            float c = (lo[a] + hi[a]) * 0.5f;
+            cmin[a] = c; cmax[a] = c;
+        }
+        triLo[t] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+        triHi[t] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+    }
+    // wave reduction then one atomic per wave per value
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        for (int off = 32; off > 0; off >>= 1) {
+            cmin[a] = fminf(cmin[a], __shfl_xor(cmin[a], off));
+            cmax[a] = fmaxf(cmax[a], __shfl_xor(cmax[a], off));
+            lo[a] = fminf(lo[a], __shfl_xor(lo[a], off));
+            hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (cmin[a] <= cmax[a]) {
+                atomicMin(&counters[C_CMIN + a], ordered(cmin[a]));
+                atomicMax(&counters[C_CMAX + a], ordered(cmax[a]));
+                atomicMin(&counters[C_BMIN + a], ordered(lo[a]));
+                atomicMax(&counters[C_BMAX + a], ordered(hi[a]));
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ triLo, const float4* __restrict__ triHi,
+                                                uint32_t n, const uint32_t* __restrict__ counters,
+                                                uint32_t* keys, uint32_t* vals) {
+    uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    float4 lo = triLo[t], hi = triHi[t];
+    float c[3] = {(lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f};
+    uint32_t q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float mn = unordered(counters[C_CMIN + a]);
+        float mx = unordered(counters[C_CMAX + a]);
+        float ext = mx - mn;
+        float u = ext > 0.0f ? (c[a] - mn) / ext : 0.0f;
+        float s = fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
+        q[a] = (uint32_t)s;
+    }
+    keys[t] = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+    vals[t] = t;
+}
+
+// Karras 2012: common prefix length of sorted keys i and j, with the index
+// appended to break ties between equal codes.
+__device__ __forceinline__ int delta(const uint32_t* keys, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t ki = keys[i], kj = keys[j];
+    if (ki == kj) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+    return __clz(ki ^ kj);
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ keys, int n, BinNode* bnodes,
+                                                uint32_t* parent) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) > 0 ? 1 : -1;
+    int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(keys, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + (d < 0 ? -1 : 0);
+    int first = i < j ? i : j, last = i < j ? j : i;
+    uint32_t left = (first == gamma) ? (kLeafBit | (uint32_t)gamma) : (uint32_t)gamma;
+    uint32_t right = (last == gamma + 1) ? (kLeafBit | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+    BinNode& b = bnodes[i];
+    b.lo.w = __uint_as_float(left);
+    b.hi.w = __uint_as_float(right);
+    b.first = (uint32_t)first;
+    b.last = (uint32_t)last;
+    // parent table: internal nodes at [0, n-1), leaves at n-1+k
+    parent[(left & kLeafBit) ? (uint32_t)(n - 1) + (left & ~kLeafBit) : left] = (uint32_t)i;
+    parent[(right & kLeafBit) ? (uint32_t)(n - 1) + (right & ~kLeafBit) : right] = (uint32_t)i;
+    if (i == 0) parent[0] = 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ void store_box(BinNode* b, const float lo[3], const float hi[3]) {
+    uint32_t* L = reinterpret_cast<uint32_t*>(&b->lo);
+    uint32_t* H = reinterpret_cast<uint32_t*>(&b->hi);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        __hip_atomic_store(L + a, __float_as_uint(lo[a]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(H + a, __float_as_uint(hi[a]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+__device__ __forceinline__ void load_child_box(const BinNode* bnodes, const float4* triLo, const float4* triHi,
+                                               const uint32_t* vals, uint32_t ref, float lo[3], float hi[3]) {
+    if (ref & kLeafBit) {
+        uint32_t t = vals[ref & ~kLeafBit];
+        float4 a = triLo[t], b = triHi[t];
+        lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
+        hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
+    } else {
+        const uint32_t* L = reinterpret_cast<const uint32_t*>(&bnodes[ref].lo);
+        const uint32_t* H = reinterpret_cast<const uint32_t*>(&bnodes[ref].hi);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = __uint_as_float(__hip_atomic_load(L + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            hi[a] = __uint_as_float(__hip_atomic_load(H + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        }
+    }
+}
+
+// Bottom-up boxes: the second child to arrive at a node computes its box.
+__global__ __launch_bounds__(256) void k_refit(int n, BinNode* bnodes, const uint32_t* __restrict__ parent,
+                                               uint32_t* flags, const float4* __restrict__ triLo,
+                                               const float4* __restrict__ triHi, const uint32_t* __restrict__ vals) {
+    int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    uint32_t p = parent[(uint32_t)(n - 1) + (uint32_t)k];
+    while (p != 0xFFFFFFFFu) {
+        __threadfence();
+        uint32_t old = atomicAdd(&flags[p], 1u);
+        if (old == 0) return;
+        __threadfence();
+        BinNode* b = &bnodes[p];
+        uint32_t l = __float_as_uint(b->lo.w), r = __float_as_uint(b->hi.w);
+        float llo[3], lhi[3], rlo[3], rhi[3], lo[3], hi[3];
+        load_child_box(bnodes, triLo, triHi, vals, l, llo, lhi);
+        load_child_box(bnodes, triLo, triHi, vals, r, rlo, rhi);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { lo[a] = fminf(llo[a], rlo[a]); hi[a] = fmaxf(lhi[a], rhi[a]); }
+        store_box(b, lo, hi);
+        p = parent[p];
+    }
+}
+
+__device__ __forceinline__ float half_area(const float lo[3], const float hi[3]) {
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// Collapse one binary subtree root into a 4-wide node: repeatedly open the
+// child with the largest surface area until 4 children, leaving subtrees of
+// at most kLeafMax triangles as leaves.
+__global__ __launch_bounds__(256) void k_collapse(const uint4* __restrict__ tasksIn, uint32_t numTasks,
+                                                  const BinNode* __restrict__ bnodes, const float4* __restrict__ triLo,
+                                                  const float4* __restrict__ triHi, const uint32_t* __restrict__ vals,
+                                                  Node4* nodes, uint32_t* counters, uint4* tasksOut) {
+    uint32_t ti = blockIdx.x * 256 + threadIdx.x;
+    if (ti >= numTasks) return;
+    uint4 task = tasksIn[ti];
+    float sceneExtent = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        sceneExtent = fmaxf(sceneExtent, unordered(counters[C_BMAX + a]) - unordered(counters[C_BMIN + a]));
+    uint32_t ref[4];
+    uint32_t m = 0;
+    if (task.x & kLeafBit) {  // single-triangle scene
+        ref[m++] = task.x;
+    } else {
+        ref[m++] = __float_as_uint(bnodes[task.x].lo.w);
+        ref[m++] = __float_as_uint(bnodes[task.x].hi.w);
+    }
+    auto count_of = [&](uint32_t r) -> uint32_t {
+        return (r & kLeafBit) ? 1u : (bnodes[r].last - bnodes[r].first + 1u);
+    };
+    while (m < 4) {
+        int best = -1;
+        float bestA = -1.0f;
+        for (uint32_t c = 0; c < m; ++c) {
+            if (!(ref[c] & kLeafBit) && count_of(ref[c]) > kLeafMax) {
+                float lo[3] = {bnodes[ref[c]].lo.x, bnodes[ref[c]].lo.y, bnodes[ref[c]].lo.z};
+                float hi[3] = {bnodes[ref[c]].hi.x, bnodes[ref[c]].hi.y, bnodes[ref[c]].hi.z};
+                float a = half_area(lo, hi);
+                if (a > bestA) { bestA = a; best = (int)c; }
+            }
+        }
+        if (best < 0) break;
+        uint32_t r = ref[best];
+        ref[best] = __float_as_uint(bnodes[r].lo.w);
+        ref[m++] = __float_as_uint(bnodes[r].hi.w);
+    }
+    Node4 out;
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    uint32_t child[4];
+    for (uint32_t c = 0; c < 4; ++c) {
+        if (c >= m) {
+            lox[c] = loy[c] = loz[c] = INFINITY;
+            hix[c] = hiy[c] = hiz[c] = -INFINITY;
+            child[c] = kEmptyChild;
+            continue;
+        }
+        uint32_t r = ref[c];
+        float lo[3], hi[3];
+        if (r & kLeafBit) {
+            uint32_t t = vals[r & ~kLeafBit];
+            float4 a = triLo[t], b = triHi[t];
+            lo[0] = a.x; lo[1] = a.y; lo[2] = a.z; hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
+            child[c] = kLeafBit | (r & ~kLeafBit);  // count-1 = 0
+        } else {
+            const BinNode& b = bnodes[r];
+            lo[0] = b.lo.x; lo[1] = b.lo.y; lo[2] = b.lo.z; hi[0] = b.hi.x; hi[1] = b.hi.y; hi[2] = b.hi.z;
+            uint32_t cnt = b.last - b.first + 1u;
+            if (cnt <= kLeafMax) {
+                child[c] = kLeafBit | ((cnt - 1u) << 27) | b.first;
+            } else {
+                uint32_t slot = atomicAdd(&counters[C_NODES], 1u);
+                uint32_t qi = atomicAdd(&counters[C_TASKS_NEXT], 1u);
+                tasksOut[qi] = make_uint4(r, slot, task.z + 1u, 0u);
+                child[c] = slot;
+            }
+        }
+        // Pad outward so the float slab test stays conservative with
+        // respect to the watertight triangle test, whose rounding can
+        // report hits on an edge or vertex an ulp outside the exact bounds.
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float pad = (fmaxf(fabsf(lo[a]), fabsf(hi[a])) + sceneExtent) * kBoxPadRel;
+            lo[a] -= pad;
+            hi[a] += pad;
+        }
+        lox[c] = lo[0]; loy[c] = lo[1]; loz[c] = lo[2];
+        hix[c] = hi[0]; hiy[c] = hi[1]; hiz[c] = hi[2];
+    }
+    out.lox = make_float4(lox[0], lox[1], lox[2], lox[3]);
+    out.hix = make_float4(hix[0], hix[1], hix[2], hix[3]);
+    out.loy = make_float4(loy[0], loy[1], loy[2], loy[3]);
+    out.hiy = make_float4(hiy[0], hiy[1], hiy[2], hiy[3]);
+    out.loz = make_float4(loz[0], loz[1], loz[2], loz[3]);
+    out.hiz = make_float4(hiz[0], hiz[1], hiz[2], hiz[3]);
+    out.child = make_uint4(child[0], child[1], child[2], child[3]);
+    out.pad = make_uint4(0, 0, 0, 0);
+    nodes[task.y] = out;
+    atomicMax(&counters[C_MAXDEPTH], task.z);
+}
+
+__global__ __launch_bounds__(256) void k_reorder_tris(const uint32_t* __restrict__ vals, const float* __restrict__ verts,
+                                                      const uint32_t* __restrict__ idx, uint32_t n, float4* tris) {
+    uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    uint32_t t = vals[k];
+    uint32_t i0 = idx[t * 3 + 0], i1 = idx[t * 3 + 1], i2 = idx[t * 3 + 2];
+    tris[k * 3 + 0] = make_float4(verts[i0 * 3 + 0], verts[i0 * 3 + 1], verts[i0 * 3 + 2], __uint_as_float(t));
+    tris[k * 3 + 1] = make_float4(verts[i1 * 3 + 0], verts[i1 * 3 + 1], verts[i1 * 3 + 2], 0.0f);
+    tris[k * 3 + 2] = make_float4(verts[i2 * 3 + 0], verts[i2 * 3 + 1], verts[i2 * 3 + 2], 0.0f);
+}
+
+// ------------------------------------------------------------- traversal
+// One lane per ray. The traversal stack keeps its top kStackLds entries in
+// LDS (a per-lane ring, column-major so a wave touches distinct banks) and
+// spills older entries to a per-lane scratch array. Children are visited
+// near to far; leaf children are intersected as soon as their node is
+// visited. Box test: slabs (lo - o) * (1/d) on boxes padded outward by
+// 2^-20 * (|coordinate| + scene extent), far distance (clamped to the best
+// hit) scaled by 1 + 2*gamma_3 (Ize 2013), so a box is never culled while it
+// may hold a triangle the watertight test reports at t <= best (ties
+// included). Ties in t resolve to the smaller primitive id, so the
+// result does not depend on the traversal order.
+template <bool ANY, bool COUNT>
+__global__ __launch_bounds__(256) void k_trace(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
+                                               const float4* __restrict__ org, const float4* __restrict__ dir,
+                                               const uint32_t* __restrict__ queue, uint32_t n,
+                                               const uint32_t* __restrict__ countDev, uint2* __restrict__ hits,
+                                               uint8_t* __restrict__ occl, unsigned long long* __restrict__ fetches) {
+    __shared__ uint32_t sNode[kStackLds][256];
+    __shared__ float sT[kStackLds][256];
+    uint32_t qi = blockIdx.x * 256 + threadIdx.x;
+    uint32_t cnt = n;
+    if (countDev) cnt = min(cnt, *countDev);
+    if (qi >= cnt) return;
+    const uint32_t lane = threadIdx.x;
+    uint32_t ray = queue ? queue[qi] : qi;
+    float4 o = org[ray], d = dir[ray];
+    const float tmin = o.w;
+    float tbest = d.w;
+    uint32_t bestPrim = 0xFFFFFFFFu;
+    RayPre rp = ray_pre(o, d);
+    const float ix = 1.0f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x));
+    const float iy = 1.0f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y));
+    const float iz = 1.0f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
+    const float kRobust = 1.0000005f;
+
+    uint32_t spillN[kStackSpill];
+    float spillT[kStackSpill];
+    uint32_t sp = 0;
+    uint32_t cur = 0;  // root
+    bool found = false;
+    uint32_t nodeFetches = 0, triFetches = 0;
+
+    for (;;) {
+        if (cur != kEmptyChild) {
+            const Node4* nd = nodes + cur;
+            if (COUNT) ++nodeFetches;
+            float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
+            uint4 ch = nd->child;
+            float blo[3][4] = {{lx.x, lx.y, lx.z, lx.w}, {ly.x, ly.y, ly.z, ly.w}, {lz.x, lz.y, lz.z, lz.w}};
+            float bhi[3][4] = {{hx.x, hx.y, hx.z, hx.w}, {hy.x, hy.y, hy.z, hy.w}, {hz.x, hz.y, hz.z, hz.w}};
+            uint32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
+            float key[4];
+            uint32_t code[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                float t0x = (blo[0][c] - o.x) * ix, t1x = (bhi[0][c] - o.x) * ix;
+                float t0y = (blo[1][c] - o.y) * iy, t1y = (bhi[1][c] - o.y) * iy;
+                float t0z = (blo[2][c] - o.z) * iz, t1z = (bhi[2][c] - o.z) * iz;
+                float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+                float tf = fminf(fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)), tbest) * kRobust;
+                bool hit = (cc[c] != kEmptyChild) && (tn <= tf);
+                key[c] = hit ? tn : INFINITY;
+                code[c] = cc[c];
+            }
+            // leaf children first
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if (key[c] != INFINITY && (code[c] & kLeafBit)) {
+                    key[c] = INFINITY;
+                    uint32_t first = code[c] & 0x07FFFFFFu;
+                    uint32_t count = ((code[c] >> 27) & 0xFu) + 1u;
+                    for (uint32_t k = 0; k < count; ++k) {
+                        const float4* tp = tris + (size_t)(first + k) * 3;
+                        if (COUNT) ++triFetches;
+                        float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+                        float th;
+                        if (intersect_tri(rp, v0, v1, v2, tmin, tbest, &th)) {
+                            uint32_t prim = __float_as_uint(v0.w);
+                            if (ANY) { found = true; break; }
+                            if (th < tbest || prim < bestPrim) { tbest = th; bestPrim = prim; }
+                        }
+                    }
+                    if (ANY && found) break;
+                }
+            }
+            if (ANY && found) break;
+            // inner children near to far (sorting network on 4 keys)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (key[c] > tbest * kRobust) key[c] = INFINITY;
+#define WRK_CSWAP(a, b)                                                             \
+    if (key[b] < key[a]) {                                                          \
+        float tk = key[a]; key[a] = key[b]; key[b] = tk;                            \
+        uint32_t tc = code[a]; code[a] = code[b]; code[b] = tc;                     \
+    }
+            WRK_CSWAP(0, 1) WRK_CSWAP(2, 3) WRK_CSWAP(0, 2) WRK_CSWAP(1, 3) WRK_CSWAP(1, 2)
+#undef WRK_CSWAP
+            // push far children (farthest first), descend into the nearest
+#pragma unroll
+            for (int c = 3; c >= 1; --c) {
+                if (key[c] != INFINITY) {
+                    if (sp >= kStackLds) {
+                        uint32_t slot = sp & (kStackLds - 1);
+                        spillN[sp - kStackLds] = sNode[slot][lane];
+                        spillT[sp - kStackLds] = sT[slot][lane];
+                    }
+                    sNode[sp & (kStackLds - 1)][lane] = code[c];
+                    sT[sp & (kStackLds - 1)][lane] = key[c];
+                    ++sp;
+                }
+            }
+            cur = (key[0] != INFINITY) ? code[0] : kEmptyChild;
+        }
+        if (cur == kEmptyChild) {
+            bool done = true;
+            while (sp > 0) {
+                --sp;
+                uint32_t slot = sp & (kStackLds - 1);
+                uint32_t nn = sNode[slot][lane];
+                float tt = sT[slot][lane];
+                if (sp >= kStackLds) {
+                    sNode[slot][lane] = spillN[sp - kStackLds];
+                    sT[slot][lane] = spillT[sp - kStackLds];
+                }
+                if (tt <= tbest * kRobust) { cur = nn; done = false; break; }
+            }
+            if (done) break;
+        }
+    }
+    if (ANY) {
+        occl[ray] = found ? 1 : 0;
+    } else {
+        hits[ray] = make_uint2(__float_as_uint(tbest), bestPrim);
+    }
+    if (COUNT) {
+        atomicAdd(&fetches[0], (unsigned long long)nodeFetches);
+        atomicAdd(&fetches[1], (unsigned long long)triFetches);
+    }
+}
+
+}  // namespace wrk
+
+using namespace wrk;
+
+namespace {
+struct BuildLayout {
+    size_t triLo, triHi, keysA, keysB, valsA, valsB, bnodes, parent, flags, tasksA, tasksB, counters, sortTemp;
+    size_t sortTempBytes, total;
+};
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+BuildLayout layout_for(uint32_t n) {
+    BuildLayout L{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off = align256(off + bytes); return o; };
+    size_t nn = std::max<size_t>(n, 2);
+    L.triLo = take(nn * 16);
+    L.triHi = take(nn * 16);
+    L.keysA = take(nn * 4);
+    L.keysB = take(nn * 4);
+    L.valsA = take(nn * 4);
+    L.valsB = take(nn * 4);
+    L.bnodes = take(nn * sizeof(BinNode));
+    L.parent = take(2 * nn * 4);
+    L.flags = take(nn * 4);
+    L.tasksA = take(nn * 16);
+    L.tasksB = take(nn * 16);
+    L.counters = take(C_COUNT * 4);
+    size_t tmp = 0;
+    (void)rocprim::radix_sort_pairs(nullptr, tmp, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                    (uint32_t*)nullptr, (size_t)nn, 0u, 30u);
+    L.sortTempBytes = tmp;
+    L.sortTemp = take(tmp);
+    L.total = off;
+    return L;
+}
+}  // namespace
+
+extern "C" int wrk_bvh_query_sizes(uint32_t numTris, uint64_t* nodeBytes, uint64_t* triBytes,
+                                   uint64_t* scratchBytes) {
+    WRK_REQUIRE(nodeBytes && triBytes && scratchBytes, "wrk_bvh_query_sizes: null output");
+    WRK_REQUIRE(numTris > 0 && numTris < (1u << 27), "wrk_bvh_query_sizes: numTris must be in [1, 2^27)");
+    *nodeBytes = (uint64_t)std::max<uint32_t>(numTris, 1u) * sizeof(Node4);
+    *triBytes = (uint64_t)numTris * 48u;
+    *scratchBytes = layout_for(numTris).total;
+    return WRK_OK;
+}
+
+extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uint32_t numTris, void* nodes,
+                             void* tris, void* scratch, uint64_t scratchBytes, wrk_bvh* out, void* stream) {
+    WRK_REQUIRE(vertices && indices && nodes && tris && scratch && out, "wrk_bvh_build: null argument");
+    WRK_REQUIRE(numTris > 0 && numTris < (1u << 27), "wrk_bvh_build: numTris must be in [1, 2^27)");
+    BuildLayout L = layout_for(numTris);
+    WRK_REQUIRE(scratchBytes >= L.total, "wrk_bvh_build: scratch too small");
+    hipStream_t s = as_stream(stream);
+    char* base = static_cast<char*>(scratch);
+    float4* triLo = (float4*)(base + L.triLo);
+    float4* triHi = (float4*)(base + L.triHi);
+    uint32_t* keysA = (uint32_t*)(base + L.keysA);
+    uint32_t* keysB = (uint32_t*)(base + L.keysB);
+    uint32_t* valsA = (uint32_t*)(base + L.valsA);
+    uint32_t* valsB = (uint32_t*)(base + L.valsB);
+    BinNode* bnodes = (BinNode*)(base + L.bnodes);
+    uint32_t* parent = (uint32_t*)(base + L.parent);
+    uint32_t* flags = (uint32_t*)(base + L.flags);
+    uint4* tasksA = (uint4*)(base + L.tasksA);
+    uint4* tasksB = (uint4*)(base + L.tasksB);
+    uint32_t* counters = (uint32_t*)(base + L.counters);
+    const uint32_t n = numTris;
+    const uint32_t g = ceil_div(n, 256);
+
+    hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(64), 0, s, counters);
+    hipLaunchKernelGGL(k_tri_bounds, dim3(g), dim3(256), 0, s, vertices, indices, n, triLo, triHi, counters);
+    hipLaunchKernelGGL(k_morton, dim3(g), dim3(256), 0, s, triLo, triHi, n, counters, keysA, valsA);
+    WRK_CHECK_LAUNCH();
+    size_t tmp = L.sortTempBytes;
+    hipError_t sortErr = rocprim::radix_sort_pairs(base + L.sortTemp, tmp, keysA, keysB, valsA, valsB, (size_t)n, 0u,
+                                                   30u, s);
+    WRK_HIP_TRY(sortErr);
+    if (n > 1) {
+        WRK_HIP_TRY(hipMemsetAsync(flags, 0, (size_t)n * 4, s));
+        hipLaunchKernelGGL(k_karras, dim3(ceil_div(n - 1, 256)), dim3(256), 0, s, keysB, (int)n, bnodes, parent);
+        hipLaunchKernelGGL(k_refit, dim3(g), dim3(256), 0, s, (int)n, bnodes, parent, flags, triLo, triHi, valsB);
+        WRK_CHECK_LAUNCH();
+    }
+    // level-by-level collapse; root 4-wide node is slot 0
+    uint4 rootTask = make_uint4(n > 1 ? 0u : kLeafBit, 0u, 1u, 0u);
+    WRK_HIP_TRY(hipMemcpyAsync(tasksA, &rootTask, sizeof(uint4), hipMemcpyHostToDevice, s));
+    uint32_t one = 1;
+    WRK_HIP_TRY(hipMemcpyAsync(counters + C_NODES, &one, 4, hipMemcpyHostToDevice, s));
+    uint32_t numTasks = 1;
+    uint4* tin = tasksA;
+    uint4* tout = tasksB;
+    while (numTasks > 0) {
+        WRK_HIP_TRY(hipMemsetAsync(counters + C_TASKS_NEXT, 0, 4, s));
+        hipLaunchKernelGGL(k_collapse, dim3(ceil_div(numTasks, 256)), dim3(256), 0, s, tin, numTasks, bnodes, triLo,
+                           triHi, valsB, (Node4*)nodes, counters, tout);
+        WRK_CHECK_LAUNCH();
+        WRK_HIP_TRY(hipMemcpyAsync(&numTasks, counters + C_TASKS_NEXT, 4, hipMemcpyDeviceToHost, s));
+        WRK_HIP_TRY(hipStreamSynchronize(s));
+        std::swap(tin, tout);
+    }
+    hipLaunchKernelGGL(k_reorder_tris, dim3(g), dim3(256), 0, s, valsB, vertices, indices, n, (float4*)tris);
+    WRK_CHECK_LAUNCH();
+    uint32_t hc[C_COUNT];
+    WRK_HIP_TRY(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+    WRK_HIP_TRY(hipStreamSynchronize(s));
+    std::memset(out, 0, sizeof(*out));
+    out->nodes = (uint64_t)(uintptr_t)nodes;
+    out->tris = (uint64_t)(uintptr_t)tris;
+    out->numNodes = hc[C_NODES];
+    out->numTris = n;
+    out->maxDepth = hc[C_MAXDEPTH];
+    for (int a = 0; a < 3; ++a) {
+        out->sceneLo[a] = unordered(hc[C_BMIN + a]);
+        out->sceneHi[a] = unordered(hc[C_BMAX + a]);
+    }
+    if (3u * out->maxDepth + 4u > kStackTotal) {
+        set_error("wrk_bvh_build: BVH depth " + std::to_string(out->maxDepth) + " exceeds the traversal stack");
+        return WRK_ERR_CAPACITY;
+    }
+    return WRK_OK;
+}
+
+template <bool ANY>
+static int launch_trace(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue, uint32_t n,
+                        const uint32_t* countDev, uint32_t* hits, uint8_t* occl, unsigned long long* fetches,
+                        void* stream) {
+    WRK_REQUIRE(bvh && bvh->nodes && bvh->tris && org4 && dir4, "wrk_trace: null argument");
+    WRK_REQUIRE(3u * bvh->maxDepth + 4u <= kStackTotal, "wrk_trace: BVH deeper than the traversal stack");
+    if (n == 0) return WRK_OK;
+    dim3 grid(ceil_div(n, 256)), block(256);
+    hipStream_t s = as_stream(stream);
+    if (fetches) {
+        hipLaunchKernelGGL((k_trace<ANY, true>), grid, block, 0, s, (const Node4*)bvh->nodes, (const float4*)bvh->tris,
+                           (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl, fetches);
+    } else {
+        hipLaunchKernelGGL((k_trace<ANY, false>), grid, block, 0, s, (const Node4*)bvh->nodes, (const float4*)bvh->tris,
+                           (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl, nullptr);
+    }
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_trace_closest(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue,
+                                 uint32_t n, const uint32_t* countDev, uint32_t* hits, void* stream) {
+    WRK_REQUIRE(hits, "wrk_trace_closest: null hits");
+    return launch_trace<false>(bvh, org4, dir4, queue, n, countDev, hits, nullptr, nullptr, stream);
+}
+
+extern "C" int wrk_trace_any(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue,
+                             uint32_t n, const uint32_t* countDev, uint8_t* occluded, void* stream) {
+    WRK_REQUIRE(occluded, "wrk_trace_any: null occluded");
+    return launch_trace<true>(bvh, org4, dir4, queue, n, countDev, nullptr, occluded, nullptr, stream);
+}
+
+extern "C" int wrk_trace_closest_counted(const wrk_bvh* bvh, const void* org4, const void* dir4,
+                                         const uint32_t* queue, uint32_t n, const uint32_t* countDev, uint32_t* hits,
+                                         unsigned long long* fetches, void* stream) {
+    WRK_REQUIRE(hits && fetches, "wrk_trace_closest_counted: null argument");
+    return launch_trace<false>(bvh, org4, dir4, queue, n, countDev, hits, nullptr, fetches, stream);
+}
+
+extern "C" int wrk_trace_any_counted(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue,
+                                     uint32_t n, const uint32_t* countDev, uint8_t* occluded,
+                                     unsigned long long* fetches, void* stream) {
+    WRK_REQUIRE(occluded && fetches, "wrk_trace_any_counted: null argument");
+    return launch_trace<true>(bvh, org4, dir4, queue, n, countDev, nullptr, occluded, fetches, stream);
+}

@@ -1,0 +1,659 @@
+// wavefront.hip — camera generation, trace-result classification, the five
+// material shaders, accumulation and tonemapping for gfx950.
+//
+// Restates libWR/kernels/trace.cu (camera, hit attributes, emissive MIS,
+// miss and shadow resolution), shade.cu and materials/*.cuh over a
+// structure-of-arrays path pool with material-sorted queues. Built with
+// -ffp-contract=off: the arithmetic order follows the reference source.
+#include <hip/hip_runtime.h>
+
+#include "wrk_common.h"
+#include "wrk_device.h"
+
+namespace wrk {
+
+enum : uint32_t { CAT_DONE = 5, NEXT_TRACE = 0, NEXT_SHADOW = 1, NEXT_DONE = 2 };
+
+struct Pool {
+    float4* org;
+    float4* dir;
+    float4* thr;
+    float4* rad;
+    uint4* st;
+    float4* contrib;
+    float4* savedOrg;
+    float4* savedDir;
+    uint2* hit;
+    uint8_t* occl;
+    float4* result;
+};
+static Pool pool_of(const wrk_paths* p) {
+    Pool q;
+    q.org = (float4*)p->org; q.dir = (float4*)p->dir; q.thr = (float4*)p->thr; q.rad = (float4*)p->rad;
+    q.st = (uint4*)p->state; q.contrib = (float4*)p->contrib; q.savedOrg = (float4*)p->savedOrg;
+    q.savedDir = (float4*)p->savedDir; q.hit = (uint2*)p->hit; q.occl = (uint8_t*)p->occluded;
+    q.result = (float4*)p->result;
+    return q;
+}
+
+struct SceneDev {
+    const float* vertices;
+    const float* normals;
+    const float* uvs;
+    const uint32_t* indices;
+    const uint32_t* triMat;
+    const wrk_material* materials;
+    const wrk_texture* textures;
+    const uint32_t* emissiveTris;
+    const float* emissiveCdf;
+    uint32_t numTris, numMaterials, numTextures, numEmissive;
+    float env[3];
+};
+static SceneDev scene_of(const wrk_scene_view* v) {
+    SceneDev s;
+    s.vertices = (const float*)v->vertices; s.normals = (const float*)v->normals; s.uvs = (const float*)v->uvs;
+    s.indices = (const uint32_t*)v->indices; s.triMat = (const uint32_t*)v->triMaterialIds;
+    s.materials = (const wrk_material*)v->materials; s.textures = (const wrk_texture*)v->textures;
+    s.emissiveTris = (const uint32_t*)v->emissiveTris; s.emissiveCdf = (const float*)v->emissiveCdf;
+    s.numTris = v->numTris; s.numMaterials = v->numMaterials; s.numTextures = v->numTextures;
+    s.numEmissive = v->numEmissive;
+    s.env[0] = v->envRadiance[0]; s.env[1] = v->envRadiance[1]; s.env[2] = v->envRadiance[2];
+    return s;
+}
+
+// ------------------------------------------------------------- generate
+// trace.cu:53-125: camera ray for (pixel, sample), path state reset.
+__global__ __launch_bounds__(256) void k_generate(wrk_camera cam, wrk_render_params prm, Pool P, uint64_t first,
+                                                  uint32_t n, uint32_t* queue) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t npix = (uint64_t)prm.width * prm.height;
+    const uint64_t g = first + i;
+    const uint32_t pixel = (uint32_t)(g % npix);
+    const uint32_t sample = (uint32_t)(g / npix);
+    const uint32_t px = pixel % prm.width;
+    const uint32_t py = pixel / prm.width;
+    uint32_t seed = (pixel * 1664525u + sample * 1013904223u) ^ 0x9e3779b9u;
+    uint32_t js = seed;
+    float r1 = randf(js);
+    float r2 = randf(js);
+    float ndcX = (2.0f * ((float)px + r1) / (float)prm.width - 1.0f) * cam.aspect;
+    float ndcY = 1.0f - 2.0f * ((float)py + r2) / (float)prm.height;
+    f3 Pp = mk(cam.position[0], cam.position[1], cam.position[2]);
+    f3 F = mk(cam.forward[0], cam.forward[1], cam.forward[2]);
+    f3 R = mk(cam.right[0], cam.right[1], cam.right[2]);
+    f3 U = mk(cam.up[0], cam.up[1], cam.up[2]);
+    float th = cam.tanHalfFovY;
+    f3 d = F + R * ndcX * th + U * ndcY * th;
+    d = normalize(d);
+    f3 o = Pp;
+    if (cam.lensRadius > 0.0f && cam.focalDistance > 0.0f) {
+        float r3 = randf(js);
+        float r4 = randf(js);
+        float theta = (2.0f * kPi) * r3;
+        float radius = sqrtf(r4) * cam.lensRadius;
+        float sn, cs;
+        wr_sincos(theta, &sn, &cs);
+        f3 lo = mk(R.x * radius * cs + U.x * radius * sn, R.y * radius * cs + U.y * radius * sn,
+                   R.z * radius * cs + U.z * radius * sn);
+        float fd = cam.focalDistance;
+        f3 fp = mk(Pp.x + d.x * fd, Pp.y + d.y * fd, Pp.z + d.z * fd);
+        o = mk(Pp.x + lo.x, Pp.y + lo.y, Pp.z + lo.z);
+        d = normalize(mk(fp.x - o.x, fp.y - o.y, fp.z - o.z));
+    }
+    P.org[i] = f4(o, 0.0001f);
+    P.dir[i] = f4(d, 1e20f);
+    P.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);  // prevPdf = 1
+    P.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.st[i] = make_uint4(seed, 0u, FLAG_FIRST_HIT | FLAG_PREV_DELTA, pixel);
+    P.result[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    queue[i] = i;
+}
+
+// ------------------------------------------------------------- classify
+// Closest-hit results (queue[0, nC)): miss -> environment + flush
+// (trace.cu:143-148); hit -> category = material kind of the primitive
+// (unknown kinds join the emissive queue, whose shader flushes them like
+// the default case of shade.cu:79-81; an out-of-range material id ends the
+// path without a flush, shade.cu:35-38). Shadow results (queue[nC, nC+nS)):
+// add the NEE contribution when unoccluded, restore the direction, and
+// resume in the Lambertian shader (trace.cu:27-51).
+__global__ __launch_bounds__(256) void k_classify(SceneDev S, wrk_render_params prm, Pool P, const uint32_t* queue,
+                                                  uint32_t nC, uint32_t nS, uint8_t* cat) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nC + nS) return;
+    uint32_t p = queue[i];
+    uint32_t c;
+    if (i < nC) {
+        uint2 h = P.hit[p];
+        if (h.y == 0xFFFFFFFFu) {
+            float4 T = P.thr[p], L = P.rad[p];
+            f3 Lr = mk(L.x + T.x * S.env[0], L.y + T.y * S.env[1], L.z + T.z * S.env[2]);
+            f3 Lc = clamp_fireflies(Lr, prm.fireflyClamp);
+            P.rad[p] = f4(Lr, 0.0f);
+            P.result[p] = f4(Lc, 0.0f);
+            c = CAT_DONE;
+        } else {
+            uint32_t m = S.triMat[h.y];
+            if (m >= S.numMaterials) {
+                c = CAT_DONE;
+            } else {
+                uint32_t type = S.materials[m].type;
+                c = type <= WRK_MAT_GGX_TRANSMISSION ? type : (uint32_t)WRK_MAT_EMISSIVE;
+            }
+        }
+    } else {
+        float4 L = P.rad[p];
+        if (!P.occl[p]) {
+            float4 C = P.contrib[p];
+            L = make_float4(L.x + C.x, L.y + C.y, L.z + C.z, L.w);
+            P.rad[p] = L;
+        }
+        float4 sd = P.savedDir[p];
+        float4 d = P.dir[p];
+        P.dir[p] = make_float4(sd.x, sd.y, sd.z, d.w);
+        c = WRK_MAT_LAMBERTIAN;
+    }
+    cat[i] = (uint8_t)c;
+}
+
+// ----------------------------------------------------------------- shade
+struct Hit {
+    f3 position, normal, geometricNormal;
+    float u, v;
+    uint32_t materialId;
+};
+
+struct PathState {
+    f3 o, d;
+    float tMin, tMax;
+    f3 T, L;
+    float prevPdf;
+    uint32_t seed, dim, depth;
+    bool prevDelta, firstHit, neeDone;
+    uint32_t pixel;
+    bool flushed, done, toShadow;
+    f3 flushL;
+    f3 contrib, savedOrg, savedDir;
+};
+
+__device__ __forceinline__ f3 vert(const SceneDev& S, uint32_t i) {
+    return mk(S.vertices[i * 3 + 0], S.vertices[i * 3 + 1], S.vertices[i * 3 + 2]);
+}
+
+__device__ __forceinline__ f3 material_albedo(const SceneDev& S, const wrk_material& m, float u, float v) {
+    if (m.albedoTextureId != 0 && m.albedoTextureId <= S.numTextures && S.textures) {
+        return sample_tex(S.textures[m.albedoTextureId - 1], u, v);
+    }
+    return mk(m.albedo[0], m.albedo[1], m.albedo[2]);
+}
+
+__device__ __forceinline__ void terminate_ray(PathState& r, float maxL) {
+    r.flushL = clamp_fireflies(r.L, maxL);
+    r.flushed = true;
+    r.done = true;
+}
+
+// sampling.cuh:162-193 + material_common.cuh:39-52
+__device__ __forceinline__ bool apply_rr(PathState& r, const wrk_render_params& prm, bool isDelta) {
+    float xi = rnd_dim(r.seed, r.dim++);
+    float rrStart = prm.rrStartDepth;
+    if ((float)r.depth < rrStart) return true;
+    float lum = 0.2126f * r.T.x + 0.7152f * r.T.y + 0.0722f * r.T.z;
+    float depthFactor = 1.0f / (1.0f + 0.1f * ((float)r.depth - rrStart));
+    float minProb = isDelta ? 0.5f : 0.05f;
+    float q = fmaxf(minProb, fminf(0.95f, lum * depthFactor));
+    if (xi < q) {
+        r.T = mk(r.T.x / q, r.T.y / q, r.T.z / q);
+        return true;
+    }
+    terminate_ray(r, prm.fireflyClamp);
+    return false;
+}
+
+// material_common.cuh:55-71
+__device__ __forceinline__ void setup_next(PathState& r, const Hit& h, f3 nd, const wrk_render_params& prm) {
+    r.d = nd;
+    r.o = offset_origin(h.position, h.normal, nd);
+    r.tMin = 0.0f;
+    r.tMax = 1e20f;
+    r.depth++;
+    if (r.depth >= prm.maxBounces) terminate_ray(r, prm.fireflyClamp);
+}
+
+// sampling.cuh:116-135
+__device__ __forceinline__ uint32_t sample_emissive(const SceneDev& S, float u) {
+    uint32_t left = 0, right = S.numEmissive;
+    while (left < right) {
+        uint32_t mid = (left + right) / 2;
+        if (S.emissiveCdf[mid] < u) left = mid + 1; else right = mid;
+    }
+    return min(left, S.numEmissive - 1);
+}
+
+struct LightSample { f3 toLight; float distSq, dist, cosTheta, lightPdf; f3 Le; };
+
+// lambertian.cuh:17-70 / ggx_material.cuh:33-109
+__device__ __forceinline__ bool sample_light(const SceneDev& S, PathState& r, const Hit& h, LightSample& ls) {
+    float lightU = rnd_dim(r.seed, r.dim++);
+    uint32_t li = sample_emissive(S, lightU);
+    uint32_t tri = S.emissiveTris[li];
+    uint32_t i0 = S.indices[tri * 3 + 0], i1 = S.indices[tri * 3 + 1], i2 = S.indices[tri * 3 + 2];
+    f3 v0 = vert(S, i0), v1 = vert(S, i1), v2 = vert(S, i2);
+    float u1 = rnd_dim(r.seed, r.dim);
+    float u2 = rnd_dim(r.seed, r.dim + 1);
+    r.dim += 2;
+    // sampling.cuh:85-110
+    float su = sqrtf(u1);
+    float b0 = 1.0f - su;
+    float b1 = su * (1.0f - u2);
+    float b2 = su * u2;
+    f3 lp = mk(v0.x * b0 + v1.x * b1 + v2.x * b2, v0.y * b0 + v1.y * b1 + v2.y * b2,
+               v0.z * b0 + v1.z * b1 + v2.z * b2);
+    f3 e1 = mk(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z);
+    f3 e2 = mk(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z);
+    f3 cp = cross(e1, e2);
+    float area = 0.5f * length(cp);
+    f3 ln = normalize(cp);
+    f3 tl = mk(lp.x - h.position.x, lp.y - h.position.y, lp.z - h.position.z);
+    ls.distSq = dot(tl, tl);
+    ls.dist = sqrtf(ls.distSq);
+    ls.toLight = mk(tl.x / ls.dist, tl.y / ls.dist, tl.z / ls.dist);
+    ls.cosTheta = dot(h.normal, ls.toLight);
+    float cosLight = -dot(ln, ls.toLight);
+    if (!(ls.cosTheta > 0.0f && cosLight > 0.0f)) return false;
+    const wrk_material& lm = S.materials[S.triMat[tri]];
+    float pdfArea = 1.0f / ((float)S.numEmissive * area);
+    ls.lightPdf = pdfArea * ls.distSq / fmaxf(cosLight, 1e-8f);
+    ls.Le = mk(lm.emission[0], lm.emission[1], lm.emission[2]);
+    return true;
+}
+
+__device__ __forceinline__ bool valid_contrib(f3 c) {
+    return !isnan(c.x) && !isnan(c.y) && !isnan(c.z) && !isinf(c.x) && !isinf(c.y) && !isinf(c.z) &&
+           c.x >= 0.0f && c.y >= 0.0f && c.z >= 0.0f;
+}
+
+// materials/lambertian.cuh:8-147
+__device__ void shade_lambertian(const SceneDev& S, const wrk_render_params& prm, PathState& r, const Hit& h,
+                                 const wrk_material& mat) {
+    if (prm.useNEE && S.numEmissive > 0 && !r.neeDone) {
+        LightSample ls;
+        if (sample_light(S, r, h, ls)) {
+            float bsdfPdf = fmaxf(0.0f, ls.cosTheta / kPi);
+            float w = power_heuristic(ls.lightPdf, bsdfPdf);
+            f3 albedo = material_albedo(S, mat, h.u, h.v);
+            f3 bsdf = mk(albedo.x / kPi * ls.cosTheta, albedo.y / kPi * ls.cosTheta, albedo.z / kPi * ls.cosTheta);
+            f3 c = mk(r.T.x * bsdf.x * ls.Le.x * w / ls.lightPdf, r.T.y * bsdf.y * ls.Le.y * w / ls.lightPdf,
+                      r.T.z * bsdf.z * ls.Le.z * w / ls.lightPdf);
+            if (valid_contrib(c)) {
+                r.contrib = c;
+                r.savedDir = r.d;
+                r.savedOrg = r.o;
+                r.d = ls.toLight;
+                r.o = offset_origin(h.position, h.normal, ls.toLight);
+                r.tMin = 1e-5f;
+                r.tMax = ls.dist - 1e-5f;
+                r.neeDone = true;
+                r.toShadow = true;
+                return;
+            }
+        }
+    }
+    r.neeDone = false;
+    f3 tg, bt;
+    make_frame(h.normal, tg, bt);
+    float u1 = rnd_dim(r.seed, r.dim);
+    float u2 = rnd_dim(r.seed, r.dim + 1);
+    r.dim += 2;
+    float pdf;
+    f3 local = sample_cosine(u1, u2, pdf);
+    f3 wd = to_world(local, h.normal, tg, bt);
+    f3 albedo = material_albedo(S, mat, h.u, h.v);
+    r.T = r.T * albedo;
+    r.prevPdf = pdf;
+    r.prevDelta = false;
+    if (!apply_rr(r, prm, false)) return;
+    setup_next(r, h, wd, prm);
+}
+
+// materials/glass.cuh:10-140 == materials/ggx_transmission.cuh:12-107
+__device__ void shade_glass(const SceneDev& S, const wrk_render_params& prm, PathState& r, const Hit& h,
+                            const wrk_material& mat) {
+    f3 wo = -r.d;
+    float cosI = dot(wo, h.normal);
+    float etaI = 1.0f, etaT = mat.ior;
+    f3 n = h.normal;
+    if (cosI < 0.0f) { cosI = -cosI; n = -n; etaI = mat.ior; etaT = 1.0f; }
+    float F = fresnel_dielectric(cosI, etaI, etaT);
+    float rr = rnd_dim(r.seed, r.dim++);
+    if (rr < F) {
+        float dn = 2.0f * dot(r.d, n);
+        r.d = mk(r.d.x - n.x * dn, r.d.y - n.y * dn, r.d.z - n.z * dn);
+        r.T = r.T * material_albedo(S, mat, h.u, h.v);
+    } else {
+        float eta = etaI / etaT;
+        float sinT2 = eta * eta * (1.0f - cosI * cosI);
+        if (sinT2 >= 1.0f) {
+            float dn = 2.0f * dot(r.d, n);
+            r.d = mk(r.d.x - n.x * dn, r.d.y - n.y * dn, r.d.z - n.z * dn);
+            r.T = r.T * material_albedo(S, mat, h.u, h.v);
+        } else {
+            float cosT = sqrtf(fmaxf(0.0f, 1.0f - sinT2));
+            float k = eta * cosI - cosT;
+            f3 rf = mk(eta * r.d.x + n.x * k, eta * r.d.y + n.y * k, eta * r.d.z + n.z * k);
+            r.d = rf;
+            r.T = r.T * material_albedo(S, mat, h.u, h.v);
+            float corr = asym_correction(rf, h.normal, h.geometricNormal);
+            r.T = r.T * corr;
+        }
+    }
+    r.prevDelta = true;
+    if (!apply_rr(r, prm, true)) return;
+    setup_next(r, h, r.d, prm);
+}
+
+// materials/ggx_material.cuh:10-213
+__device__ void shade_ggx(const SceneDev& S, const wrk_render_params& prm, PathState& r, const Hit& h,
+                          const wrk_material& mat) {
+    f3 wo = -r.d;
+    f3 albedo = mk(mat.albedo[0], mat.albedo[1], mat.albedo[2]);
+    if (prm.useNEE && S.numEmissive > 0) {
+        LightSample ls;
+        if (sample_light(S, r, h, ls)) {
+            f3 brdf = eval_ggx(wo, ls.toLight, h.normal, albedo, mat.roughness, mat.metallic, mat.ior);
+            float bsdfPdf = ggx_pdf(wo, ls.toLight, h.normal, mat.roughness);
+            float w = power_heuristic(ls.lightPdf, bsdfPdf);
+            f3 c = mk(r.T.x * brdf.x * ls.Le.x * ls.cosTheta * w / ls.lightPdf,
+                      r.T.y * brdf.y * ls.Le.y * ls.cosTheta * w / ls.lightPdf,
+                      r.T.z * brdf.z * ls.Le.z * ls.cosTheta * w / ls.lightPdf);
+            if (valid_contrib(c)) r.L = mk(r.L.x + c.x, r.L.y + c.y, r.L.z + c.z);
+        }
+    }
+    f3 tg, bt;
+    make_frame(h.normal, tg, bt);
+    float u0 = rnd_dim(r.seed, r.dim);
+    float u1 = rnd_dim(r.seed, r.dim + 1);
+    r.dim += 2;
+    f3 wi = mk(0.0f, 0.0f, 0.0f);
+    float pdf;
+    sample_ggx(wo, h.normal, tg, bt, mat.roughness, u0, u1, wi, pdf);
+    float cosTheta = dot(h.normal, wi);
+    if (pdf < 1e-5f || cosTheta <= 0.0f) { terminate_ray(r, prm.fireflyClamp); return; }
+    f3 brdf = eval_ggx(wo, wi, h.normal, albedo, mat.roughness, mat.metallic, mat.ior);
+    if (isnan(brdf.x) || isnan(brdf.y) || isnan(brdf.z) || isinf(brdf.x) || isinf(brdf.y) || isinf(brdf.z)) {
+        terminate_ray(r, prm.fireflyClamp);
+        return;
+    }
+    r.T = mk(r.T.x * brdf.x * cosTheta / pdf, r.T.y * brdf.y * cosTheta / pdf, r.T.z * brdf.z * cosTheta / pdf);
+    r.prevPdf = pdf;
+    r.prevDelta = false;
+    if (!apply_rr(r, prm, false)) return;
+    setup_next(r, h, wi, prm);
+}
+
+// trace.cu:155-264 (hit attributes) reconstructed from (origin, direction,
+// t, primitive); trace.cu:289-316 emissive MIS is applied by the caller.
+__device__ __forceinline__ Hit reconstruct(const SceneDev& S, f3 o, f3 d, float t, uint32_t prim, f3* e1o,
+                                           f3* e2o) {
+    Hit hit;
+    f3 hitPos = mk(o.x + t * d.x, o.y + t * d.y, o.z + t * d.z);
+    uint32_t i0 = S.indices[prim * 3 + 0], i1 = S.indices[prim * 3 + 1], i2 = S.indices[prim * 3 + 2];
+    f3 v0 = vert(S, i0), v1 = vert(S, i1), v2 = vert(S, i2);
+    f3 e1 = v1 - v0, e2 = v2 - v0;
+    *e1o = e1; *e2o = e2;
+    f3 gN = normalize(cross(e1, e2));
+    f3 v0p = hitPos - v0;
+    float d00 = dot(e1, e1), d01 = dot(e1, e2), d11 = dot(e2, e2);
+    float d20 = dot(v0p, e1), d21 = dot(v0p, e2);
+    float denom = d00 * d11 - d01 * d01;
+    float vb = (d11 * d20 - d01 * d21) / fmaxf(denom, 1e-8f);
+    float wb = (d00 * d21 - d01 * d20) / fmaxf(denom, 1e-8f);
+    float ub = 1.0f - vb - wb;
+    f3 sN = gN;
+    if (S.normals) {
+        f3 n0 = mk(S.normals[i0 * 3 + 0], S.normals[i0 * 3 + 1], S.normals[i0 * 3 + 2]);
+        f3 n1 = mk(S.normals[i1 * 3 + 0], S.normals[i1 * 3 + 1], S.normals[i1 * 3 + 2]);
+        f3 n2 = mk(S.normals[i2 * 3 + 0], S.normals[i2 * 3 + 1], S.normals[i2 * 3 + 2]);
+        float l0 = length(n0), l1 = length(n1), l2 = length(n2);
+        if (l0 > 1e-6f && l1 > 1e-6f && l2 > 1e-6f) {
+            sN = normalize(ub * n0 + vb * n1 + wb * n2);
+            if (dot(sN, gN) < 0.0f) sN = -sN;
+        }
+    }
+    hit.geometricNormal = dot(gN, d) > 0.0f ? -gN : gN;
+    hit.normal = dot(sN, d) > 0.0f ? -sN : sN;
+    hit.position = hitPos;
+    hit.materialId = S.triMat[prim];
+    if (S.uvs) {
+        float a0 = S.uvs[i0 * 2 + 0], b0 = S.uvs[i0 * 2 + 1];
+        float a1 = S.uvs[i1 * 2 + 0], b1 = S.uvs[i1 * 2 + 1];
+        float a2 = S.uvs[i2 * 2 + 0], b2 = S.uvs[i2 * 2 + 1];
+        hit.u = ub * a0 + vb * a1 + wb * a2;
+        hit.v = ub * b0 + vb * b1 + wb * b2;
+    } else {
+        hit.u = 0.0f;
+        hit.v = 0.0f;
+    }
+    return hit;
+}
+
+// utils/normal_map.cuh:36-60
+__device__ __forceinline__ f3 effective_normal(const SceneDev& S, const wrk_material& mat, const Hit& h) {
+    if (mat.normalTextureId > 0 && mat.normalTextureId <= S.numTextures) {
+        const wrk_texture& tex = S.textures[mat.normalTextureId - 1];
+        if (tex.texels == 0) return h.normal;
+        f3 tn = sample_tex(tex, h.u, h.v);
+        f3 tg, bt;
+        make_frame(h.normal, tg, bt);
+        f3 n = mk(tn.x * 2.0f - 1.0f, tn.y * 2.0f - 1.0f, tn.z * 2.0f - 1.0f);
+        f3 w = mk(tg.x * n.x + bt.x * n.y + h.normal.x * n.z, tg.y * n.x + bt.y * n.y + h.normal.y * n.z,
+                  tg.z * n.x + bt.z * n.y + h.normal.z * n.z);
+        return normalize(w);
+    }
+    return h.normal;
+}
+
+// One launch over the material-sorted queue: blocks are assigned to the
+// five category segments in order, so every block runs one material
+// shader; the trailing blocks mark queue positions past the last segment
+// as finished.
+__global__ __launch_bounds__(256) void k_shade(SceneDev S, wrk_render_params prm, Pool P, const uint32_t* queue,
+                                               const uint32_t* __restrict__ offsets, uint32_t maxItems,
+                                               uint32_t lastIteration, uint8_t* nextStage) {
+    uint32_t off[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) off[k] = offsets[k];
+    uint32_t b = blockIdx.x;
+    int seg = -1;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        uint32_t nb = (off[k + 1] - off[k] + 255) / 256;
+        if (seg < 0) {
+            if (b < nb) seg = k; else b -= nb;
+        }
+    }
+    if (seg < 0) {
+        uint32_t j = off[5] + b * 256 + threadIdx.x;
+        if (j < maxItems) nextStage[j] = NEXT_DONE;
+        return;
+    }
+    uint32_t j = off[seg] + b * 256 + threadIdx.x;
+    if (j >= off[seg + 1]) return;
+    uint32_t p = queue[j];
+
+    PathState r;
+    uint4 st = P.st[p];
+    r.seed = st.x; r.dim = st.y;
+    r.depth = st.z & FLAG_DEPTH_MASK;
+    r.firstHit = (st.z & FLAG_FIRST_HIT) != 0;
+    r.prevDelta = (st.z & FLAG_PREV_DELTA) != 0;
+    r.neeDone = (st.z & FLAG_NEE_DONE) != 0;
+    r.pixel = st.w;
+    float4 o4 = r.neeDone ? P.savedOrg[p] : P.org[p];
+    float4 d4 = P.dir[p];
+    float4 T4 = P.thr[p], L4 = P.rad[p];
+    r.o = xyz(o4); r.tMin = o4.w; r.d = xyz(d4); r.tMax = d4.w;
+    r.T = xyz(T4); r.prevPdf = T4.w; r.L = xyz(L4);
+    r.flushed = false; r.done = false; r.toShadow = false;
+    uint2 hr = P.hit[p];
+    float t = __uint_as_float(hr.x);
+    f3 e1, e2;
+    Hit h = reconstruct(S, r.o, r.d, t, hr.y, &e1, &e2);
+    r.firstHit = false;  // first-hit guides (trace.cu:267-286) are not produced here
+    const wrk_material& mat = S.materials[h.materialId];
+
+    if (seg == WRK_MAT_EMISSIVE) {
+        // trace.cu:289-316: emission with MIS, then shadeEmissive / default
+        if (mat.type == WRK_MAT_EMISSIVE) {
+            float w = 1.0f;
+            if (prm.useNEE && r.depth > 0 && !r.prevDelta && S.numEmissive > 0) {
+                float area = 0.5f * length(cross(e1, e2));
+                float distSq = t * t;
+                float cosL = fabsf(dot(h.geometricNormal, r.d));
+                float pdfArea = 1.0f / ((float)S.numEmissive * area);
+                float lightPdf = pdfArea * distSq / fmaxf(cosL, 1e-8f);
+                w = power_heuristic(r.prevPdf, lightPdf);
+            }
+            f3 E = mk(mat.emission[0], mat.emission[1], mat.emission[2]);
+            f3 TE = r.T * E;
+            r.L = r.L + TE * w;
+        }
+        terminate_ray(r, prm.fireflyClamp);
+    } else {
+        Hit eh = h;
+        if (mat.normalTextureId > 0) eh.normal = effective_normal(S, mat, h);
+        if (seg == WRK_MAT_LAMBERTIAN) shade_lambertian(S, prm, r, eh, mat);
+        else if (seg == WRK_MAT_GGX_REFLECTION) shade_ggx(S, prm, r, eh, mat);
+        else shade_glass(S, prm, r, eh, mat);
+    }
+
+    uint32_t next;
+    if (r.done) {
+        next = NEXT_DONE;
+        if (r.flushed) P.result[p] = f4(r.flushL, 0.0f);
+    } else if (lastIteration) {
+        next = NEXT_DONE;  // renderer.cpp:213 cap: the path is dropped unflushed
+    } else if (r.toShadow) {
+        next = NEXT_SHADOW;
+        P.contrib[p] = f4(r.contrib, 0.0f);
+        P.savedOrg[p] = f4(r.savedOrg, r.tMin);
+        P.savedDir[p] = f4(r.savedDir, 0.0f);
+    } else {
+        next = NEXT_TRACE;
+    }
+    if (!r.done) {
+        P.org[p] = f4(r.o, r.tMin);
+        P.dir[p] = f4(r.d, r.tMax);
+        P.thr[p] = f4(r.T, r.prevPdf);
+        P.rad[p] = f4(r.L, 0.0f);
+        uint32_t flags = (r.depth & FLAG_DEPTH_MASK) | (r.prevDelta ? FLAG_PREV_DELTA : 0u) |
+                         (r.neeDone ? FLAG_NEE_DONE : 0u);
+        P.st[p] = make_uint4(r.seed, r.dim, flags, r.pixel);
+    }
+    nextStage[j] = (uint8_t)next;
+}
+
+// accum[pixel] += result[path] over the batch in sample order.
+__global__ __launch_bounds__(256) void k_accumulate(const float4* __restrict__ result, uint64_t first, uint32_t n,
+                                                    uint32_t npix, float* accum) {
+    uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    uint32_t cover = n < npix ? n : npix;
+    if (k >= cover) return;
+    uint64_t g0 = first + k;  // first path of this pixel in the batch
+    uint32_t pixel = (uint32_t)(g0 % npix);
+    float a0 = accum[pixel * 3 + 0], a1 = accum[pixel * 3 + 1], a2 = accum[pixel * 3 + 2];
+    for (uint64_t i = k; i < n; i += npix) {
+        float4 v = result[i];
+        a0 = a0 + v.x; a1 = a1 + v.y; a2 = a2 + v.z;
+    }
+    accum[pixel * 3 + 0] = a0; accum[pixel * 3 + 1] = a1; accum[pixel * 3 + 2] = a2;
+}
+
+__global__ __launch_bounds__(256) void k_average(const float* __restrict__ accum, uint32_t n3, float invSpp,
+                                                 float* out) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n3) out[i] = accum[i] * invSpp;
+}
+
+// libWR/test/helpers/image.cpp:14-49 (gamma in FP64, see oracle_capi.cpp)
+__global__ __launch_bounds__(256) void k_aces(const float* __restrict__ img, uint32_t n3, uint8_t* out) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n3) return;
+    const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
+    float x = img[i] * 2.2f;
+    float y = (x * (a * x + b)) / (x * (c * x + d) + e);
+    y = y < 0.0f ? 0.0f : (y > 1.0f ? 1.0f : y);
+    const float gamma = 1.0f / 2.2f;
+    float g = (float)pow((double)y, (double)gamma);
+    out[i] = (uint8_t)(g * 255.0f);
+}
+
+}  // namespace wrk
+
+using namespace wrk;
+
+extern "C" int wrk_generate(const wrk_camera* cam, const wrk_render_params* params, const wrk_paths* paths,
+                            uint64_t first, uint32_t n, uint32_t* queue, void* stream) {
+    WRK_REQUIRE(cam && params && paths && queue, "wrk_generate: null argument");
+    WRK_REQUIRE(n <= paths->capacity, "wrk_generate: n exceeds the path pool capacity");
+    WRK_REQUIRE(params->width > 0 && params->height > 0, "wrk_generate: empty image");
+    if (n == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_generate, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), *cam, *params,
+                       pool_of(paths), first, n, queue);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_classify(const wrk_scene_view* scene, const wrk_render_params* params, const wrk_paths* paths,
+                            const uint32_t* queue, uint32_t numClosest, uint32_t numShadow, uint8_t* category,
+                            void* stream) {
+    WRK_REQUIRE(scene && params && paths && queue && category, "wrk_classify: null argument");
+    uint32_t n = numClosest + numShadow;
+    if (n == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_classify, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), scene_of(scene), *params,
+                       pool_of(paths), queue, numClosest, numShadow, category);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_shade(const wrk_scene_view* scene, const wrk_render_params* params, const wrk_paths* paths,
+                         const uint32_t* queue, const uint32_t* offsets, uint32_t maxItems, uint32_t lastIteration,
+                         uint8_t* nextStage, void* stream) {
+    WRK_REQUIRE(scene && params && paths && queue && offsets && nextStage, "wrk_shade: null argument");
+    if (maxItems == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_shade, dim3(ceil_div(maxItems, 256) + 6), dim3(256), 0, as_stream(stream), scene_of(scene),
+                       *params, pool_of(paths), queue, offsets, maxItems, lastIteration, nextStage);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_accumulate(const wrk_paths* paths, uint64_t first, uint32_t n, uint32_t npix, float* accum,
+                              void* stream) {
+    WRK_REQUIRE(paths && accum && npix > 0, "wrk_accumulate: bad argument");
+    if (n == 0) return WRK_OK;
+    uint32_t cover = n < npix ? n : npix;
+    hipLaunchKernelGGL(k_accumulate, dim3(ceil_div(cover, 256)), dim3(256), 0, as_stream(stream),
+                       (const float4*)paths->result, first, n, npix, accum);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_average(const float* accum, uint32_t npix, uint32_t spp, float* out, void* stream) {
+    WRK_REQUIRE(accum && out && spp > 0, "wrk_average: bad argument");
+    uint32_t n3 = npix * 3;
+    if (n3 == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_average, dim3(ceil_div(n3, 256)), dim3(256), 0, as_stream(stream), accum, n3,
+                       1.0f / (float)spp, out);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_tonemap_aces(const float* image, uint32_t npix, uint8_t* rgb8, void* stream) {
+    WRK_REQUIRE(image && rgb8, "wrk_tonemap_aces: null argument");
+    uint32_t n3 = npix * 3;
+    if (n3 == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_aces, dim3(ceil_div(n3, 256)), dim3(256), 0, as_stream(stream), image, n3, rgb8);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}

@@ -1,0 +1,162 @@
+// wr_capi.cpp — C ABI over the wr:: C++ API (see include/wr_c.h).
+#include <stdexcept>
+#include <string>
+
+#include "../../include/wr/wr.h"
+#include "../../include/wr_c.h"
+
+namespace {
+thread_local std::string g_err;
+wr::Camera to_camera(const wr_c_camera* c) {
+    wr::Camera k;
+    k.position = wr::Vec3(c->position[0], c->position[1], c->position[2]);
+    k.target = wr::Vec3(c->target[0], c->target[1], c->target[2]);
+    k.up = wr::Vec3(c->up[0], c->up[1], c->up[2]);
+    k.fovY = c->fovY;
+    k.aspect = c->aspect;
+    k.focalDistance = c->focalDistance;
+    k.lensRadius = c->lensRadius;
+    return k;
+}
+wr::RenderParams to_params(const wr_c_render_params* p) {
+    wr::RenderParams r;
+    r.width = p->width;
+    r.height = p->height;
+    r.spp = p->spp;
+    r.useNEE = p->useNEE != 0;
+    r.maxBounces = p->maxBounces;
+    r.russianRouletteDepth = p->russianRouletteDepth;
+    r.fireflyClamp = p->fireflyClamp;
+    r.denoiser.enabled = p->denoiser != 0;
+    return r;
+}
+template <typename F> int guard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    } catch (...) {
+        g_err = "unknown exception";
+        return -1;
+    }
+}
+template <typename F> uint32_t guard_id(F&& f) {
+    uint32_t id = 0xFFFFFFFFu;
+    if (guard([&] { id = f(); }) != 0) return 0xFFFFFFFFu;
+    return id;
+}
+wr::Scene* S(void* p) { return static_cast<wr::Scene*>(p); }
+wr::Renderer* R(void* p) { return static_cast<wr::Renderer*>(p); }
+}  // namespace
+
+extern "C" {
+
+const char* wr_c_last_error(void) { return g_err.c_str(); }
+
+int wr_c_context_create(int deviceId, int logLevel) {
+    return guard([&] {
+        wr::ContextConfig cfg;
+        cfg.deviceId = deviceId;
+        cfg.logLevel = logLevel;
+        wr::Context ctx(cfg);
+    });
+}
+
+void* wr_c_scene_create(void) {
+    void* out = nullptr;
+    guard([&] { out = new wr::Scene(); });
+    return out;
+}
+void wr_c_scene_destroy(void* scene) { delete S(scene); }
+
+uint32_t wr_c_scene_add_lambertian(void* s, float r, float g, float b, uint32_t tex) {
+    return guard_id([&] { return S(s)->addLambertianMaterial(wr::Vec3(r, g, b), tex); });
+}
+uint32_t wr_c_scene_add_emissive(void* s, float r, float g, float b) {
+    return guard_id([&] { return S(s)->addEmissiveMaterial(wr::Vec3(r, g, b)); });
+}
+uint32_t wr_c_scene_add_glass(void* s, float r, float g, float b, float ior) {
+    return guard_id([&] { return S(s)->addGlassMaterial(wr::Vec3(r, g, b), ior); });
+}
+uint32_t wr_c_scene_add_ggx_reflection(void* s, float r, float g, float b, float rough, float metal) {
+    return guard_id([&] { return S(s)->addGGXReflectionMaterial(wr::Vec3(r, g, b), rough, metal); });
+}
+uint32_t wr_c_scene_add_ggx_transmission(void* s, float r, float g, float b, float rough, float ior) {
+    return guard_id([&] { return S(s)->addGGXTransmissionMaterial(wr::Vec3(r, g, b), rough, ior); });
+}
+int wr_c_scene_set_normal_map(void* s, uint32_t mat, uint32_t tex) {
+    return guard([&] { S(s)->setMaterialNormalMap(mat, tex); });
+}
+int wr_c_scene_add_mesh(void* s, const float* v, size_t nv, const uint32_t* i, size_t ni, const float* n, size_t nn,
+                        const float* uv, size_t nuv, uint32_t mat) {
+    return guard([&] {
+        S(s)->addTriangleMesh(std::span<const float>(v, nv), std::span<const uint32_t>(i, ni),
+                              std::span<const float>(n, n ? nn : 0), std::span<const float>(uv, uv ? nuv : 0), mat);
+    });
+}
+uint32_t wr_c_texture_create(void* s, const char* path) {
+    uint32_t id = 0;
+    if (guard([&] { id = wr::Texture2D::create(S(s), path); }) != 0) return 0;
+    return id;
+}
+void wr_c_texture_destroy(void* s, uint32_t id) { guard([&] { wr::Texture2D::destroy(S(s), id); }); }
+uint32_t wr_c_scene_add_texture_rgba8(void* s, const uint8_t* rgba, uint32_t w, uint32_t h) {
+    uint32_t id = 0;
+    if (guard([&] { id = S(s)->addTextureRGBA8(w, h, std::span<const uint8_t>(rgba, (size_t)w * h * 4)); }) != 0) return 0;
+    return id;
+}
+int wr_c_scene_set_environment_radiance(void* s, float r, float g, float b) {
+    return guard([&] { S(s)->setEnvironmentRadiance(wr::Vec3(r, g, b)); });
+}
+int wr_c_scene_set_environment_map(void* s, const char* path) {
+    return guard([&] { S(s)->setEnvironmentMap(path); });
+}
+int wr_c_scene_finalize(void* s) { return guard([&] { S(s)->finalize(); }); }
+int wr_c_scene_get_bvh(void* s, wrk_bvh* out) { return guard([&] { *out = S(s)->getBVH(); }); }
+int wr_c_scene_get_view(void* s, wrk_scene_view* out) { return guard([&] { *out = S(s)->getDeviceView(); }); }
+double wr_c_scene_build_seconds(void* s) { return S(s)->getBuildSeconds(); }
+
+void* wr_c_renderer_create(void) {
+    void* out = nullptr;
+    guard([&] { out = new wr::Renderer(); });
+    return out;
+}
+void wr_c_renderer_destroy(void* r) { delete R(r); }
+int wr_c_renderer_set_wavefront(void* r, uint32_t cap, uint32_t maxPaths, uint32_t flags) {
+    return guard([&] {
+        wr::WavefrontConfig c;
+        c.iterationCap = cap;
+        c.maxPathsInFlight = maxPaths;
+        c.timeKernels = (flags & 1u) != 0;
+        c.countTraversal = (flags & 2u) != 0;
+        R(r)->setWavefrontConfig(c);
+    });
+}
+int wr_c_render(void* r, void* s, const wr_c_camera* cam, const wr_c_render_params* p, float* out) {
+    return guard([&] { R(r)->render(S(s), to_camera(cam), reinterpret_cast<wr::Vec3*>(out), to_params(p)); });
+}
+int wr_c_render_paths(void* r, void* s, const wr_c_camera* cam, const wr_c_render_params* p, uint64_t first,
+                      uint32_t count, float* out) {
+    return guard([&] { R(r)->renderPaths(S(s), to_camera(cam), to_params(p), first, count, out); });
+}
+int wr_c_renderer_stats(void* r, wr_c_stats* out) {
+    return guard([&] {
+        const wr::RenderStats& st = R(r)->getStats();
+        out->seconds = st.seconds;
+        out->paths = st.paths;
+        out->closestRays = st.closestRays;
+        out->shadowRays = st.shadowRays;
+        out->iterations = st.iterations;
+        out->batches = st.batches;
+        out->msGenerate = st.msGenerate; out->msTraceClosest = st.msTraceClosest; out->msTraceAny = st.msTraceAny;
+        out->msClassify = st.msClassify; out->msPartition = st.msPartition; out->msShade = st.msShade;
+        out->msAccumulate = st.msAccumulate;
+        out->launchesTraceClosest = st.launchesTraceClosest; out->launchesTraceAny = st.launchesTraceAny;
+        out->closestNodeFetches = st.closestNodeFetches; out->closestTriFetches = st.closestTriFetches;
+        out->anyNodeFetches = st.anyNodeFetches; out->anyTriFetches = st.anyTriFetches;
+    });
+}
+
+}  // extern "C"

@@ -51,6 +51,7 @@ static_assert(sizeof(Node4) == 128, "Node4 must be 128 bytes");
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
 constexpr uint32_t kLeafMax = 4;        // triangles per leaf
+constexpr float kBoxPadRel = 9.5367431640625e-07f;  // 2^-20 outward box padding
 constexpr uint32_t kStackLds = 16;      // traversal stack entries per lane in LDS
 constexpr uint32_t kStackSpill = 176;   // further entries per lane in scratch
 constexpr uint32_t kStackTotal = kStackLds + kStackSpill;

@@ -69,6 +69,14 @@ uint32_t orc_scene_add_material(void* h, const wrk_material* m) {
     return (uint32_t)o->s.materials.size() - 1;
 }
 
+// scene.cpp:170-175; returns -1 for an invalid material id ("Invalid material ID").
+int orc_scene_set_normal_map(void* h, uint32_t mat, uint32_t tex) {
+    OScene* o = static_cast<OScene*>(h);
+    if (mat >= o->s.materials.size()) return -1;
+    o->s.materials[mat].normalTextureId = tex;
+    return 0;
+}
+
 // RGBA8 row-major texture; returns the 1-based id (scene.cpp:246-264).
 uint32_t orc_scene_add_texture(void* h, const uint8_t* rgba, uint32_t w, uint32_t ht) {
     OScene* o = static_cast<OScene*>(h);

@@ -120,20 +120,23 @@ struct Build {
     }
 };
 
-inline bool box_hit(const HNode& n, const double o[3], const double inv[3], double tmin, double tmax) {
-    // FP64 slab test, widened by a relative 1e-9 so that it can only be
-    // conservative with respect to the FP32 triangle test.
+inline bool box_hit(const HNode& n, const double o[3], const double inv[3], double tmin, double tmax,
+                    double ext) {
+    // FP64 slab test on a box padded outward by 2^-20 * (|coordinate| +
+    // scene extent), far distance clamped to the best hit and then widened
+    // by 1e-6: conservative with respect to the FP32 watertight triangle
+    // test, whose rounding can report edge/vertex hits slightly outside the
+    // exact bounds and ties at equal t.
     double t0 = tmin, t1 = tmax;
     for (int a = 0; a < 3; ++a) {
-        double lo = n.lo[a], hi = n.hi[a];
-        double pad = 1e-9 * (std::fabs(lo) + std::fabs(hi)) + 1e-30;
-        lo -= pad; hi += pad;
+        double pad = (std::max(std::fabs(n.lo[a]), std::fabs(n.hi[a])) + ext) * 9.5367431640625e-07;
+        double lo = n.lo[a] - pad, hi = n.hi[a] + pad;
         double ta = (lo - o[a]) * inv[a], tb = (hi - o[a]) * inv[a];
         if (std::isnan(ta) || std::isnan(tb)) { if (o[a] < lo || o[a] > hi) return false; continue; }
         if (ta > tb) std::swap(ta, tb);
-        t0 = std::max(t0, ta); t1 = std::min(t1, tb * (1 + 1e-9));
+        t0 = std::max(t0, ta); t1 = std::min(t1, tb);
     }
-    return t0 <= t1;
+    return t0 <= t1 * (1 + 1e-6) + 1e-30;
 }
 }  // namespace
 
@@ -154,6 +157,12 @@ void build_host_bvh(Scene& s) {
             b.bhi[t * 3 + a] = std::max(x0, std::max(x1, x2));
             b.cen[t * 3 + a] = (b.blo[t * 3 + a] + b.bhi[t * 3 + a]) * 0.5;
         }
+    }
+    s.extent = 0;
+    for (int a = 0; a < 3; ++a) {
+        double lo = 1e300, hi = -1e300;
+        for (uint32_t t = 0; t < nt; ++t) { lo = std::min(lo, b.blo[t * 3 + a]); hi = std::max(hi, b.bhi[t * 3 + a]); }
+        s.extent = std::max(s.extent, hi - lo);
     }
     s.nodes.reserve(2 * nt / 2 + 8);
     b.build(0, nt);
@@ -205,7 +214,7 @@ static Hit bvh_query(const Scene& s, const float o[3], const float d[3], float t
     stack[sp++] = 0;
     while (sp) {
         const HNode& n = s.nodes[stack[--sp]];
-        if (!box_hit(n, od, inv, tmin, h.t)) continue;
+        if (!box_hit(n, od, inv, tmin, h.t, s.extent)) continue;
         if (n.left == 0xFFFFFFFFu) {
             for (uint32_t i = n.start; i < n.start + n.count; ++i) {
                 uint32_t t = s.order[i];

@@ -142,6 +142,7 @@ struct Scene {
     // host BVH
     std::vector<HNode> nodes;
     std::vector<uint32_t> order;
+    double extent = 0;
     uint32_t numTris() const { return (uint32_t)(indices.size() / 3); }
 };
 
