@@ -77,6 +77,16 @@ def lib():
                                         C.c_uint32, C.c_int, C.c_int, C.c_int, _P, _P]),
             "orc_compact": (C.c_uint32, [_P, _P, C.c_uint32, _P]),
             "orc_aces": (None, [_P, C.c_uint32, _P]),
+            "orc_fresnel": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+            "orc_ggx_D": (C.c_float, [C.c_float, C.c_float]),
+            "orc_ggx_G1": (C.c_float, [C.c_float, C.c_float]),
+            "orc_ggx_eval": (None, [_P, _P, _P, _P, C.c_float, C.c_float, C.c_float, _P]),
+            "orc_ggx_pdf": (C.c_float, [_P, _P, _P, C.c_float]),
+            "orc_ggx_sample": (C.c_int, [_P, _P, C.c_float, C.c_float, C.c_float, _P, _P]),
+            "orc_sample_cosine": (None, [C.c_float, C.c_float, _P, _P]),
+            "orc_frame": (None, [_P, _P, _P]),
+            "orc_power_heuristic": (C.c_float, [C.c_float, C.c_float]),
+            "orc_cdf_search": (C.c_uint32, [_P, C.c_uint32, C.c_float]),
             "orc_wall_seconds": (C.c_double, []),
             "orc_max_threads": (C.c_int, []),
         }

@@ -242,7 +242,17 @@ bool any_bvh(const Scene& s, const float o[3], const float d[3], float tmin, flo
 }
 
 // --------------------------------------------------------- path tracing
-namespace {
+// sampling.cuh:116-135 — lower-bound binary search on cdf[mid] < u, clamped
+// to n-1.
+template <typename T>
+uint32_t cdf_search(const float* cdf, uint32_t n, T u) {
+    uint32_t left = 0, right = n;
+    while (left < right) {
+        uint32_t mid = (left + right) / 2;
+        if (T(cdf[mid]) < u) left = mid + 1; else right = mid;
+    }
+    return std::min(left, n - 1);
+}
 
 template <typename T>
 struct Tracer {
@@ -363,15 +373,8 @@ struct Tracer {
         normal = normalize(c);
         return pt;
     }
-    // sampling.cuh:116-135
     uint32_t sampleEmissive(T u) const {
-        uint32_t n = (uint32_t)s.emissiveCdf.size();
-        uint32_t left = 0, right = n;
-        while (left < right) {
-            uint32_t mid = (left + right) / 2;
-            if (T(s.emissiveCdf[mid]) < u) left = mid + 1; else right = mid;
-        }
-        return std::min(left, n - 1);
+        return cdf_search(s.emissiveCdf.data(), (uint32_t)s.emissiveCdf.size(), u);
     }
     // sampling.cuh:145-149
     static T powerHeuristic(T a, T b) {
@@ -845,8 +848,6 @@ struct Tracer {
     }
 };
 
-}  // namespace
-
 // One (pixel, sample) path through the reference host loop
 // (renderer.cpp:190-284): every iteration runs raygen/trace (trace.cu:17-153)
 // then shade (shade.cu:24-85) on the still-active path; the loop stops after
@@ -898,3 +899,44 @@ template void trace_path<float>(const Scene&, const Camera&, const Params&, uint
 template void trace_path<double>(const Scene&, const Camera&, const Params&, uint32_t, uint32_t, bool, PathOut*, double*);
 
 }  // namespace orc
+
+// ------------------------------------------------ math exports for tests
+// Thin wrappers over the FP32 restatement so tests/ can pin each function
+// against closed forms and invariants.
+extern "C" {
+using orc::V3;
+using TF = orc::Tracer<float>;
+float orc_fresnel(float cosI, float etaI, float etaT) { return TF::fresnelDielectric(cosI, etaI, etaT); }
+float orc_ggx_D(float NoH, float alpha) { return TF::ggxD(NoH, alpha); }
+float orc_ggx_G1(float NoV, float alpha) { return TF::ggxG1(NoV, alpha); }
+void orc_ggx_eval(const float* wo, const float* wi, const float* n, const float* albedo, float rough, float metal,
+                  float ior, float* out) {
+    V3<float> r = TF::evalGGX(orc::mk(wo[0], wo[1], wo[2]), orc::mk(wi[0], wi[1], wi[2]), orc::mk(n[0], n[1], n[2]),
+                              orc::mk(albedo[0], albedo[1], albedo[2]), rough, metal, ior);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+float orc_ggx_pdf(const float* wo, const float* wi, const float* n, float rough) {
+    return TF::ggxPdf(orc::mk(wo[0], wo[1], wo[2]), orc::mk(wi[0], wi[1], wi[2]), orc::mk(n[0], n[1], n[2]), rough);
+}
+// wi (3 floats) and pdf; returns 1 when wi was written (reference leaves it unset on failure)
+int orc_ggx_sample(const float* wo, const float* n, float rough, float u0, float u1, float* wi, float* pdf) {
+    V3<float> N = orc::mk(n[0], n[1], n[2]), tg, bt;
+    TF::frame(N, tg, bt);
+    V3<float> w = orc::mk(0.0f, 0.0f, 0.0f);
+    bool set = false;
+    TF::sampleGGX(orc::mk(wo[0], wo[1], wo[2]), N, tg, bt, rough, u0, u1, w, *pdf, set);
+    wi[0] = w.x; wi[1] = w.y; wi[2] = w.z;
+    return set ? 1 : 0;
+}
+void orc_sample_cosine(float u1, float u2, float* dir, float* pdf) {
+    V3<float> d = TF::sampleCosine(u1, u2, *pdf);
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+}
+void orc_frame(const float* n, float* t, float* b) {
+    V3<float> T, B;
+    TF::frame(orc::mk(n[0], n[1], n[2]), T, B);
+    t[0] = T.x; t[1] = T.y; t[2] = T.z; b[0] = B.x; b[1] = B.y; b[2] = B.z;
+}
+float orc_power_heuristic(float a, float b) { return TF::powerHeuristic(a, b); }
+uint32_t orc_cdf_search(const float* cdf, uint32_t n, float u) { return orc::cdf_search<float>(cdf, n, u); }
+}  // extern "C"

@@ -1,0 +1,53 @@
+"""The C-ABI library loads on a machine without a GPU and exports every entry
+point its public headers declare (no compute calls)."""
+import re
+import subprocess
+from pathlib import Path
+
+import offlinerenderer_amd as wr
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared(header):
+    text = (ROOT / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b((?:wrk|wr_c)_[a-z0-9_]+)\s*\(", text)))
+
+
+def _exported():
+    out = subprocess.run(["nm", "-D", "--defined-only", str(wr.LIB_PATH)], capture_output=True, text=True,
+                         check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if " T " in line}
+
+
+def test_headers_declare_entry_points():
+    assert len(_declared("wrk.h")) >= 18
+    assert len(_declared("wr_c.h")) >= 25
+
+
+def test_library_exports_every_declared_symbol():
+    exported = _exported()
+    for h in ("wrk.h", "wr_c.h"):
+        missing = [s for s in _declared(h) if s not in exported]
+        assert not missing, f"{h}: not exported: {missing}"
+
+
+def test_ctypes_binds_every_symbol():
+    L = wr.lib()
+    for name in wr._SIGS:
+        assert getattr(L, name) is not None
+    # no GPU needed for error plumbing
+    assert isinstance(L.wrk_last_error(), bytes)
+
+
+def test_cpp_api_symbols_exported():
+    out = subprocess.run(["nm", "-DC", "--defined-only", str(wr.LIB_PATH)], capture_output=True, text=True,
+                         check=True).stdout
+    for sig in ["wr::Context::Context(wr::ContextConfig const&)", "wr::Scene::finalize(wr::SceneBuildConfig const&)",
+                "wr::Renderer::render(wr::Scene*, wr::Camera const&, wr::Vec3*, wr::RenderParams const&)",
+                "wr::Texture2D::create(wr::Scene*, std::__cxx11::basic_string<char, std::char_traits<char>, "
+                "std::allocator<char> > const&)",
+                "wr::Scene::addTriangleMesh(std::span<float const, 18446744073709551615ul>, "
+                "std::span<unsigned int const, 18446744073709551615ul>, unsigned int)"]:
+        assert sig in out, sig
