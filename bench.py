@@ -47,7 +47,7 @@ def main():
     ap.add_argument("--spp", type=int, default=256)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--max-paths", type=int, default=1 << 23)
+    ap.add_argument("--max-paths", type=int, default=1 << 28, help="paths in flight (256M = ~46 GB of HBM)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target wall time of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

@@ -40,7 +40,7 @@ struct RenderParams {
 // (obeys maxBounces). maxPathsInFlight bounds the device path pool.
 struct WavefrontConfig {
     uint32_t iterationCap = 8;
-    uint32_t maxPathsInFlight = 1u << 23;
+    uint32_t maxPathsInFlight = 1u << 26;  // ~11 GB of path state
     bool timeKernels = false;     // per-kernel device time with HIP events
     bool countTraversal = false;  // node/triangle fetch counters (slower)
 };

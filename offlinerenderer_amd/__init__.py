@@ -20,7 +20,9 @@ from pathlib import Path
 import numpy as np
 
 PKG = Path(__file__).resolve().parent
-LIB_PATH = PKG / "lib" / "libwr_amd.so"
+# OFFLINERENDERER_AMD_LIB selects another build of the same library (used to
+# compare tuning variants of the kernels); default is the in-tree build.
+LIB_PATH = Path(os.environ.get("OFFLINERENDERER_AMD_LIB", PKG / "lib" / "libwr_amd.so"))
 
 _lib = None
 
@@ -210,7 +212,7 @@ class RenderParams:
 class WavefrontConfig:
     """iterationCap 8 = libWR's host loop (renderer.cpp:213); 0 = run to completion."""
     iterationCap: int = 8
-    maxPathsInFlight: int = 1 << 23
+    maxPathsInFlight: int = 1 << 26
     timeKernels: bool = False
     countTraversal: bool = False
 

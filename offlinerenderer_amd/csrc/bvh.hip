@@ -330,48 +330,79 @@ __global__ __launch_bounds__(256) void k_reorder_tris(const uint32_t* __restrict
 }
 
 // ------------------------------------------------------------- traversal
-// One lane per ray. The traversal stack keeps its top kStackLds entries in
-// LDS (a per-lane ring, column-major so a wave touches distinct banks) and
-// spills older entries to a per-lane scratch array. Children are visited
-// near to far; leaf children are intersected as soon as their node is
-// visited. Box test: slabs (lo - o) * (1/d) on boxes padded outward by
-// 2^-20 * (|coordinate| + scene extent), far distance (clamped to the best
-// hit) scaled by 1 + 2*gamma_3 (Ize 2013), so a box is never culled while it
-// may hold a triangle the watertight test reports at t <= best (ties
-// included). Ties in t resolve to the smaller primitive id, so the
-// result does not depend on the traversal order.
+// One lane per ray with per-wave dynamic ray fetch: each wave owns a chunk
+// of raysPerLane*64 queue entries and a lane whose ray finishes takes the
+// next entry of the chunk (ballot + popcount, no atomics), so lanes do not
+// idle until the slowest ray of a static 64-ray group is done. raysPerLane
+// is chosen per launch so the grid still fills every wave slot of the chip. The
+// traversal stack keeps its top kStackLds entries in LDS (a per-lane ring,
+// column-major so a wave touches distinct banks) and spills older entries
+// to a per-lane scratch array. Children are visited near to far; leaf
+// children are intersected as soon as their node is visited. Box test:
+// slabs (lo - o) * (1/d) on boxes padded outward by 2^-20 * (|coordinate| +
+// scene extent), far distance (clamped to the best hit) scaled by
+// 1 + 2*gamma_3 (Ize 2013), so a box is never culled while it may hold a
+// triangle the watertight test reports at t <= best (ties included). Ties
+// in t resolve to the smaller primitive id, so the result does not depend
+// on the traversal order.
 template <bool ANY, bool COUNT>
 __global__ __launch_bounds__(256) void k_trace(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
                                                const float4* __restrict__ org, const float4* __restrict__ dir,
                                                const uint32_t* __restrict__ queue, uint32_t n,
                                                const uint32_t* __restrict__ countDev, uint2* __restrict__ hits,
-                                               uint8_t* __restrict__ occl, unsigned long long* __restrict__ fetches) {
+                                               uint8_t* __restrict__ occl, unsigned long long* __restrict__ fetches,
+                                               uint32_t raysPerLane) {
     __shared__ uint32_t sNode[kStackLds][256];
     __shared__ float sT[kStackLds][256];
-    uint32_t qi = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t wlane = threadIdx.x & 63;
+    const uint64_t ltMask = wlane == 0 ? 0ull : (~0ull >> (64 - wlane));
     uint32_t cnt = n;
     if (countDev) cnt = min(cnt, *countDev);
-    if (qi >= cnt) return;
-    const uint32_t lane = threadIdx.x;
-    uint32_t ray = queue ? queue[qi] : qi;
-    float4 o = org[ray], d = dir[ray];
-    const float tmin = o.w;
-    float tbest = d.w;
-    uint32_t bestPrim = 0xFFFFFFFFu;
-    RayPre rp = ray_pre(o, d);
-    const float ix = 1.0f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x));
-    const float iy = 1.0f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y));
-    const float iz = 1.0f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
+    const uint64_t waveId = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+    const uint64_t chunkBegin = waveId * (64u * raysPerLane);
+    const uint64_t chunkEnd = min<uint64_t>(chunkBegin + 64u * raysPerLane, cnt);
+    uint64_t cursor = chunkBegin;  // wave-uniform
     const float kRobust = 1.0000005f;
 
     uint32_t spillN[kStackSpill];
     float spillT[kStackSpill];
-    uint32_t sp = 0;
-    uint32_t cur = 0;  // root
+    bool active = false;
+    uint32_t ray = 0, sp = 0, cur = kEmptyChild, bestPrim = 0xFFFFFFFFu;
+    float4 o = make_float4(0, 0, 0, 0);
+    float tmin = 0.0f, tbest = 0.0f, ix = 0.0f, iy = 0.0f, iz = 0.0f;
+    RayPre rp;
+    rp.kx = 0; rp.ky = 1; rp.kz = 2; rp.Sx = rp.Sy = rp.Sz = 0.0f; rp.o[0] = rp.o[1] = rp.o[2] = 0.0f;
     bool found = false;
     uint32_t nodeFetches = 0, triFetches = 0;
 
     for (;;) {
+        // refill lanes without a ray from the wave's chunk
+        uint64_t need = __ballot(!active);
+        if (need != 0 && cursor < chunkEnd) {
+            uint64_t qi = cursor + __popcll(need & ltMask);
+            if (!active && qi < chunkEnd) {
+                ray = queue ? queue[qi] : (uint32_t)qi;
+                o = org[ray];
+                float4 d = dir[ray];
+                tmin = o.w;
+                tbest = d.w;
+                bestPrim = 0xFFFFFFFFu;
+                rp = ray_pre(o, d);
+                ix = 1.0f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x));
+                iy = 1.0f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y));
+                iz = 1.0f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
+                sp = 0;
+                cur = 0;  // root
+                found = false;
+                active = true;
+            }
+            cursor += __popcll(need);
+        }
+        if (__ballot(active) == 0) break;
+        if (!active) continue;
+
+        // one traversal step: visit `cur` (if any), then pop if needed
         if (cur != kEmptyChild) {
             const Node4* nd = nodes + cur;
             if (COUNT) ++nodeFetches;
@@ -411,36 +442,39 @@ __global__ __launch_bounds__(256) void k_trace(const Node4* __restrict__ nodes, 
                             if (th < tbest || prim < bestPrim) { tbest = th; bestPrim = prim; }
                         }
                     }
-                    if (ANY && found) break;
                 }
             }
-            if (ANY && found) break;
-            // inner children near to far (sorting network on 4 keys)
+            if (ANY && found) {
+                cur = kEmptyChild;
+                sp = 0;
+            } else {
+                // inner children near to far (sorting network on 4 keys)
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-                if (key[c] > tbest * kRobust) key[c] = INFINITY;
+                for (int c = 0; c < 4; ++c)
+                    if (key[c] > tbest * kRobust) key[c] = INFINITY;
 #define WRK_CSWAP(a, b)                                                             \
     if (key[b] < key[a]) {                                                          \
         float tk = key[a]; key[a] = key[b]; key[b] = tk;                            \
         uint32_t tc = code[a]; code[a] = code[b]; code[b] = tc;                     \
     }
-            WRK_CSWAP(0, 1) WRK_CSWAP(2, 3) WRK_CSWAP(0, 2) WRK_CSWAP(1, 3) WRK_CSWAP(1, 2)
+                WRK_CSWAP(0, 1) WRK_CSWAP(2, 3) WRK_CSWAP(0, 2) WRK_CSWAP(1, 3) WRK_CSWAP(1, 2)
 #undef WRK_CSWAP
-            // push far children (farthest first), descend into the nearest
+                // push far children (farthest first), descend into the nearest
 #pragma unroll
-            for (int c = 3; c >= 1; --c) {
-                if (key[c] != INFINITY) {
-                    if (sp >= kStackLds) {
-                        uint32_t slot = sp & (kStackLds - 1);
-                        spillN[sp - kStackLds] = sNode[slot][lane];
-                        spillT[sp - kStackLds] = sT[slot][lane];
+                for (int c = 3; c >= 1; --c) {
+                    if (key[c] != INFINITY) {
+                        if (sp >= kStackLds) {
+                            uint32_t slot = sp & (kStackLds - 1);
+                            spillN[sp - kStackLds] = sNode[slot][lane];
+                            spillT[sp - kStackLds] = sT[slot][lane];
+                        }
+                        sNode[sp & (kStackLds - 1)][lane] = code[c];
+                        sT[sp & (kStackLds - 1)][lane] = key[c];
+                        ++sp;
                     }
-                    sNode[sp & (kStackLds - 1)][lane] = code[c];
-                    sT[sp & (kStackLds - 1)][lane] = key[c];
-                    ++sp;
                 }
+                cur = (key[0] != INFINITY) ? code[0] : kEmptyChild;
             }
-            cur = (key[0] != INFINITY) ? code[0] : kEmptyChild;
         }
         if (cur == kEmptyChild) {
             bool done = true;
@@ -455,13 +489,12 @@ __global__ __launch_bounds__(256) void k_trace(const Node4* __restrict__ nodes, 
                 }
                 if (tt <= tbest * kRobust) { cur = nn; done = false; break; }
             }
-            if (done) break;
+            if (done) {
+                if (ANY) occl[ray] = found ? 1 : 0;
+                else hits[ray] = make_uint2(__float_as_uint(tbest), bestPrim);
+                active = false;
+            }
         }
-    }
-    if (ANY) {
-        occl[ray] = found ? 1 : 0;
-    } else {
-        hits[ray] = make_uint2(__float_as_uint(tbest), bestPrim);
     }
     if (COUNT) {
         atomicAdd(&fetches[0], (unsigned long long)nodeFetches);
@@ -592,6 +625,21 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
     return WRK_OK;
 }
 
+// Rays per lane for a launch of n rays: enough waves for two full rounds of
+// every wave slot (CUs x 4 SIMDs x 8 waves) before any lane loops over rays.
+static uint32_t rays_per_lane(uint32_t n) {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    uint64_t slots = (uint64_t)cus * 4u * 8u * 64u * 2u;
+    uint64_t r = n / slots;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(r, kRaysPerLane));
+}
+
 template <bool ANY>
 static int launch_trace(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue, uint32_t n,
                         const uint32_t* countDev, uint32_t* hits, uint8_t* occl, unsigned long long* fetches,
@@ -599,14 +647,17 @@ static int launch_trace(const wrk_bvh* bvh, const void* org4, const void* dir4, 
     WRK_REQUIRE(bvh && bvh->nodes && bvh->tris && org4 && dir4, "wrk_trace: null argument");
     WRK_REQUIRE(3u * bvh->maxDepth + 4u <= kStackTotal, "wrk_trace: BVH deeper than the traversal stack");
     if (n == 0) return WRK_OK;
-    dim3 grid(ceil_div(n, 256)), block(256);
+    const uint32_t rpl = rays_per_lane(n);
+    dim3 grid(ceil_div(n, 256u * rpl)), block(256);
     hipStream_t s = as_stream(stream);
     if (fetches) {
         hipLaunchKernelGGL((k_trace<ANY, true>), grid, block, 0, s, (const Node4*)bvh->nodes, (const float4*)bvh->tris,
-                           (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl, fetches);
+                           (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl, fetches,
+                           rpl);
     } else {
         hipLaunchKernelGGL((k_trace<ANY, false>), grid, block, 0, s, (const Node4*)bvh->nodes, (const float4*)bvh->tris,
-                           (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl, nullptr);
+                           (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl, nullptr,
+                           rpl);
     }
     WRK_CHECK_LAUNCH();
     return WRK_OK;

@@ -52,8 +52,15 @@ constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
 constexpr uint32_t kLeafMax = 4;        // triangles per leaf
 constexpr float kBoxPadRel = 9.5367431640625e-07f;  // 2^-20 outward box padding
-constexpr uint32_t kStackLds = 16;      // traversal stack entries per lane in LDS
-constexpr uint32_t kStackSpill = 176;   // further entries per lane in scratch
+#ifndef WRK_STACK_LDS
+#define WRK_STACK_LDS 8
+#endif
+#ifndef WRK_RAYS_PER_LANE
+#define WRK_RAYS_PER_LANE 1
+#endif
+constexpr uint32_t kStackLds = WRK_STACK_LDS;        // traversal stack entries per lane in LDS (power of 2)
+constexpr uint32_t kStackSpill = 192 - WRK_STACK_LDS; // further entries per lane in scratch
+constexpr uint32_t kRaysPerLane = WRK_RAYS_PER_LANE;  // upper bound of rays per lane in a wave's fetch chunk
 constexpr uint32_t kStackTotal = kStackLds + kStackSpill;
 
 }  // namespace wrk
