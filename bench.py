@@ -13,6 +13,10 @@ dominant kernel (closest-hit traversal) and a CPU-baseline object (the
 oracle, OpenMP on the host cores, on a 1-spp sample of the same frame).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--spp 256]
+
+--workload large measures the incoherent 10.24M-triangle scene instead
+(2,000 displaced icospheres, 3840x2160, 16 spp, depth 8); the default
+workload is the one the headline metric is quoted on.
 """
 from __future__ import annotations
 
@@ -44,12 +48,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--spp", type=int, default=256)
-    ap.add_argument("--width", type=int, default=1920)
-    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--workload", choices=["material", "large"], default="material")
+    ap.add_argument("--spp", type=int, default=None)
+    ap.add_argument("--width", type=int, default=None)
+    ap.add_argument("--height", type=int, default=None)
     ap.add_argument("--max-paths", type=int, default=1 << 28, help="paths in flight (256M = ~46 GB of HBM)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target wall time of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the bench, run a plain device copy of this many MiB (calibrates HBM counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -64,15 +71,22 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     wr.Context(wr.ContextConfig(deviceId=0, logLevel=0))
-    W, H, spp = args.width, args.height, args.spp
-    log(f"[bench] building material scene {W}x{H} spp={spp}")
-    desc = scenes.material_scene(W, H, spp)
+    if args.workload == "material":
+        W, H, spp = args.width or 1920, args.height or 1080, args.spp or 256
+        desc = scenes.material_scene(W, H, spp)
+        workload = f"material_scene_{W}x{H}_{spp}spp_depth{desc.maxBounces}"
+    else:
+        W, H, spp = args.width or 3840, args.height or 2160, args.spp or 16
+        desc = scenes.large_scene(W, H, spp)
+        workload = f"large_scene_{desc.numTris}tris_{W}x{H}_{spp}spp_depth{desc.maxBounces}"
+    log(f"[bench] {workload}")
     scene = wr.Scene()
     desc.build(scene)
     scene.finalize()
     bvh = scene.getBVH()
     log(f"[bench] {desc.numTris} tris, BVH {bvh.numNodes} nodes depth {bvh.maxDepth}, "
         f"build {scene.getBuildSeconds() * 1e3:.1f} ms")
+    bvh_build_ms = scene.getBuildSeconds() * 1e3
     cam = wr.Camera(**desc.camera)
     prm = wr.RenderParams(width=W, height=H, spp=spp, maxBounces=desc.maxBounces,
                           russianRouletteDepth=desc.rrDepth)
@@ -123,6 +137,15 @@ def main():
     any_bytes_per_ray = ANY_FIXED + any_nodes * NODE_BYTES + any_tris * TRI_BYTES
     any_gbs = any_bytes_per_ray * rays_s / max(totals["msTraceAny"] / 1e3, 1e-9) / 1e9
 
+    if args.copy_probe_mib:
+        nbytes = args.copy_probe_mib << 20
+        src = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: 3 x {nbytes} bytes read + written")
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(desc, W, H, args.cpu_seconds)
@@ -139,13 +162,14 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded material scene generator)",
+        "data": f"synthetic (seeded {args.workload} scene generator)",
         "config": {
-            "workload": "material_scene_1920x1080_256spp_depth16",
+            "workload": workload,
             "width": W, "height": H, "spp": spp, "max_depth": desc.maxBounces, "triangles": desc.numTris,
             "iteration_cap": "none (paths run to completion)", "paths_in_flight": args.max_paths,
         },
         "mrays_per_s": round(mrays, 3),
+        "bvh_build_ms": round(bvh_build_ms, 2),
         "kernel_ms_per_step": {k: round(v / args.steps, 3) for k, v in totals.items()},
         "roofline": {
             "kernel": "k_trace<closest>",

@@ -4,8 +4,8 @@
 // nondeterministic order) with an ordered, atomic-free reduce-then-scan:
 //   1. k_part_count  — each 1024-item tile counts its items per category
 //                      with wave ballots (one popcount per wave per category);
-//   2. k_part_scan   — one block scans the K x tiles count matrix
-//                      (category-major), giving every (category, tile) its
+//   2. k_scan_*      — a three-launch scan of the K x tiles count matrix
+//                      (category-major) gives every (category, tile) its
 //                      output offset;
 //   3. k_part_scatter— each tile ranks its items stably (ballot prefix inside
 //                      the wave, LDS prefix across the 4 waves, running total
@@ -53,38 +53,92 @@ __global__ __launch_bounds__(256) void k_part_count(const uint8_t* __restrict__ 
     }
 }
 
-// Exclusive scan over m = K * numTiles counts by one 1024-lane block;
-// offsets[k] = first output slot of category k, offsets[K] = total kept.
-__global__ __launch_bounds__(1024) void k_part_scan(uint32_t* __restrict__ counts, uint32_t m, uint32_t numTiles,
-                                                    uint32_t K, uint32_t* __restrict__ offsets) {
-    __shared__ uint32_t waveSum[16];
-    __shared__ uint32_t carry;
-    const uint32_t per = (m + 1023) / 1024;
-    const uint32_t begin = threadIdx.x * per;
-    const uint32_t end = min(begin + per, m);
-    uint32_t local = 0;
-    for (uint32_t i = begin; i < end; ++i) local += counts[i];
-    // block exclusive scan of `local`
+// Exclusive scan over m = K * numTiles counts (category-major), in three
+// launches: per-segment totals, one block scanning the (few) segment totals,
+// then each segment rescanned with its carry. offsets[k] = first output slot
+// of category k, offsets[K] = total kept.
+constexpr uint32_t kScanSeg = 1024 * 8;  // counts per segment (1024 lanes x 8)
+
+__device__ __forceinline__ uint32_t block_exclusive_scan_1024(uint32_t v, uint32_t* waveSum, uint32_t* total) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t incl = local;
+    uint32_t incl = v;
 #pragma unroll
     for (uint32_t off = 1; off < 64; off <<= 1) {
-        uint32_t v = __shfl_up(incl, off);
-        if (lane >= off) incl += v;
+        uint32_t u = __shfl_up(incl, off);
+        if (lane >= off) incl += u;
     }
     if (lane == 63) waveSum[wave] = incl;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t acc = 0;
         for (uint32_t w = 0; w < 16; ++w) { uint32_t t = waveSum[w]; waveSum[w] = acc; acc += t; }
-        carry = acc;
+        *total = acc;
     }
     __syncthreads();
-    uint32_t run = waveSum[wave] + incl - local;
-    for (uint32_t i = begin; i < end; ++i) { uint32_t c = counts[i]; counts[i] = run; run += c; }
+    uint32_t r = waveSum[wave] + incl - v;
     __syncthreads();
-    if (threadIdx.x < K) offsets[threadIdx.x] = (numTiles > 0) ? counts[threadIdx.x * numTiles] : 0u;
+    return r;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_segments(const uint32_t* __restrict__ counts, uint32_t m,
+                                                        uint32_t* __restrict__ segTotals) {
+    __shared__ uint32_t waveSum[16];
+    __shared__ uint32_t total;
+    const uint32_t base = blockIdx.x * kScanSeg + threadIdx.x * 8;
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i)
+        if (base + i < m) local += counts[base + i];
+    block_exclusive_scan_1024(local, waveSum, &total);
+    if (threadIdx.x == 0) segTotals[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_totals(uint32_t* __restrict__ segTotals, uint32_t numSeg,
+                                                      uint32_t K, uint32_t* __restrict__ offsets) {
+    __shared__ uint32_t waveSum[16];
+    __shared__ uint32_t total;
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t b = 0; b < numSeg; b += 1024) {
+        uint32_t i = b + threadIdx.x;
+        uint32_t v = i < numSeg ? segTotals[i] : 0u;
+        uint32_t ex = block_exclusive_scan_1024(v, waveSum, &total);
+        if (i < numSeg) segTotals[i] = carry + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += total;
+        __syncthreads();
+    }
     if (threadIdx.x == 0) offsets[K] = carry;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_apply(uint32_t* __restrict__ counts, uint32_t m,
+                                                     const uint32_t* __restrict__ segOffsets, uint32_t numTiles,
+                                                     uint32_t K, uint32_t* __restrict__ offsets) {
+    __shared__ uint32_t waveSum[16];
+    __shared__ uint32_t total;
+    const uint32_t base = blockIdx.x * kScanSeg + threadIdx.x * 8;
+    uint32_t v[8];
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        v[i] = base + i < m ? counts[base + i] : 0u;
+        local += v[i];
+    }
+    uint32_t run = segOffsets[blockIdx.x] + block_exclusive_scan_1024(local, waveSum, &total);
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        uint32_t e = base + i;
+        if (e < m) {
+            counts[e] = run;
+            if (e % numTiles == 0) offsets[e / numTiles] = run;  // first tile of category e/numTiles
+        }
+        run += v[i];
+    }
+}
+
+__global__ void k_zero_offsets(uint32_t* offsets, uint32_t n) {
+    if (threadIdx.x < n) offsets[threadIdx.x] = 0;
 }
 
 template <uint32_t K>
@@ -131,15 +185,21 @@ template <uint32_t K>
 static int partition_impl(const uint8_t* category, const uint32_t* in, uint32_t n, uint32_t* out,
                           uint32_t* offsetsOut, void* scratch, uint64_t scratchBytes, hipStream_t s) {
     const uint32_t numTiles = ceil_div(n, kTile);
-    WRK_REQUIRE((uint64_t)K * numTiles * 4u <= scratchBytes, "wrk_partition: scratch too small");
+    const uint32_t m = K * numTiles;
+    const uint32_t numSeg = ceil_div(m, kScanSeg);
+    WRK_REQUIRE(((uint64_t)m + numSeg) * 4u <= scratchBytes, "wrk_partition: scratch too small");
+    if (numTiles == 0) {
+        hipLaunchKernelGGL(k_zero_offsets, dim3(1), dim3(64), 0, s, offsetsOut, K + 1);
+        WRK_CHECK_LAUNCH();
+        return WRK_OK;
+    }
     uint32_t* counts = static_cast<uint32_t*>(scratch);
-    if (numTiles > 0) {
-        hipLaunchKernelGGL(k_part_count<K>, dim3(numTiles), dim3(256), 0, s, category, n, numTiles, counts);
-    }
-    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, counts, K * numTiles, numTiles, K, offsetsOut);
-    if (numTiles > 0) {
-        hipLaunchKernelGGL(k_part_scatter<K>, dim3(numTiles), dim3(256), 0, s, category, in, n, numTiles, counts, out);
-    }
+    uint32_t* segTotals = counts + m;
+    hipLaunchKernelGGL(k_part_count<K>, dim3(numTiles), dim3(256), 0, s, category, n, numTiles, counts);
+    hipLaunchKernelGGL(k_scan_segments, dim3(numSeg), dim3(1024), 0, s, counts, m, segTotals);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, s, segTotals, numSeg, K, offsetsOut);
+    hipLaunchKernelGGL(k_scan_apply, dim3(numSeg), dim3(1024), 0, s, counts, m, segTotals, numTiles, K, offsetsOut);
+    hipLaunchKernelGGL(k_part_scatter<K>, dim3(numTiles), dim3(256), 0, s, category, in, n, numTiles, counts, out);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }
@@ -151,7 +211,8 @@ using namespace wrk;
 extern "C" uint64_t wrk_partition_scratch_bytes(uint32_t n, uint32_t numCategories) {
     uint64_t tiles = (n + kTile - 1) / kTile;
     uint32_t k = numCategories < 1 ? 1 : numCategories;
-    return (tiles * k + 64) * 4;
+    uint64_t m = tiles * k;
+    return (m + (m + kScanSeg - 1) / kScanSeg + 64) * 4;
 }
 
 extern "C" int wrk_partition(const uint8_t* category, const uint32_t* in, uint32_t n, uint32_t numCategories,
@@ -189,14 +250,14 @@ extern "C" int wrk_compact(const uint8_t* keep, const uint32_t* in, uint32_t n, 
     // keep bytes are used directly as categories when they are 0/1: map
     // nonzero -> 0 into the tail of the scratch buffer.
     const uint32_t numTiles = ceil_div(n, kTile);
-    uint64_t countsBytes = ((uint64_t)numTiles + 64) * 4;
+    uint64_t countsBytes = ((uint64_t)numTiles + ceil_div(numTiles, kScanSeg) + 64) * 4;
     uint64_t catOff = (countsBytes + 255) & ~uint64_t(255);
     WRK_REQUIRE(scratchBytes >= catOff + n, "wrk_compact: scratch too small (need wrk_compact_scratch_bytes)");
     hipStream_t s = as_stream(stream);
     uint8_t* cat = static_cast<uint8_t*>(scratch) + catOff;
     if (n > 0) hipLaunchKernelGGL(k_keep_to_cat, dim3(ceil_div(n, 256)), dim3(256), 0, s, keep, n, cat);
     // offsets for K=1: [0, total] -> total lands in countOut via a 2-word tail
-    uint32_t* offs = static_cast<uint32_t*>(scratch) + numTiles + 8;
+    uint32_t* offs = static_cast<uint32_t*>(scratch) + numTiles + ceil_div(numTiles, kScanSeg) + 8;
     int rc = partition_impl<1>(cat, in, n, out, offs, scratch, countsBytes, s);
     if (rc != WRK_OK) return rc;
     WRK_HIP_TRY(hipMemcpyAsync(countOut, offs + 1, 4, hipMemcpyDeviceToDevice, s));
@@ -205,6 +266,6 @@ extern "C" int wrk_compact(const uint8_t* keep, const uint32_t* in, uint32_t n, 
 
 extern "C" uint64_t wrk_compact_scratch_bytes(uint32_t n) {
     uint64_t tiles = (n + kTile - 1) / kTile;
-    uint64_t countsBytes = (tiles + 64) * 4;
+    uint64_t countsBytes = (tiles + (tiles + kScanSeg - 1) / kScanSeg + 64) * 4;
     return ((countsBytes + 255) & ~uint64_t(255)) + n;
 }

@@ -1,0 +1,55 @@
+"""Ordered compaction / K-way partition kernels against the oracle's
+copy_if and numpy stable partition (bit-exact queues), including empty,
+ragged and multi-segment sizes."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import offlinerenderer_amd as wr
+from tests import gpu_util as G
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n", [0, 1, 1023, 1024, 1025, 77777, 9_000_001])
+def test_compact_matches_copy_if(gpu, oracle, n):
+    rng = np.random.default_rng(n)
+    keep = (rng.uniform(size=n) < 0.37).astype(np.uint8)
+    items = rng.integers(0, 1 << 31, size=n, dtype=np.uint32)
+    L = wr.lib()
+    sb = L.wrk_compact_scratch_bytes(n)
+    scratch = torch.empty(max(sb, 1), dtype=torch.uint8, device="cuda")
+    out = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    kd = G.dev(keep) if n else torch.zeros(1, dtype=torch.uint8, device="cuda")
+    it = G.dev(items) if n else torch.zeros(1, dtype=torch.int32, device="cuda")
+    G.check(L.wrk_compact(G.vp(kd), G.vp(it), n, G.vp(out), G.vp(cnt), G.vp(scratch), sb, G.stream()))
+    torch.cuda.synchronize()
+    c = int(cnt.item())
+    ref = oracle.compact(keep, items) if n else np.zeros(0, np.uint32)
+    assert c == ref.size
+    assert np.array_equal(out.cpu().numpy().view(np.uint32)[:c], ref)
+
+
+@pytest.mark.parametrize("K", [2, 5, 8])
+@pytest.mark.parametrize("n", [5000, 3_000_017])
+def test_partition_is_stable(gpu, K, n):
+    rng = np.random.default_rng(K * 1000 + n % 997)
+    cat = rng.integers(0, K + 2, size=n).astype(np.uint8)   # categories >= K are dropped
+    items = rng.permutation(n).astype(np.uint32)
+    L = wr.lib()
+    sb = L.wrk_partition_scratch_bytes(n, K)
+    scratch = torch.empty(sb, dtype=torch.uint8, device="cuda")
+    out = torch.zeros(n, dtype=torch.int32, device="cuda")
+    offs = torch.zeros(K + 1, dtype=torch.int32, device="cuda")
+    G.check(L.wrk_partition(G.vp(G.dev(cat)), G.vp(G.dev(items)), n, K, G.vp(out), G.vp(offs), G.vp(scratch), sb,
+                            G.stream()))
+    torch.cuda.synchronize()
+    o = offs.cpu().numpy()
+    got = out.cpu().numpy().view(np.uint32)
+    expect_offs = np.concatenate([[0], np.cumsum([np.count_nonzero(cat == k) for k in range(K)])])
+    assert np.array_equal(o, expect_offs)
+    for k in range(K):
+        assert np.array_equal(got[o[k]:o[k + 1]], items[cat == k])

@@ -60,3 +60,9 @@ def test_cornell_image_matches_oracle_accumulation(gpu, oracle):
     assert rel.max() <= REL_TOL, rel.max()
     st = r.getStats()
     assert st["paths"] == 48 * 48 * 6 and st["batches"] >= 3
+
+
+def test_large_scene_product_mode_paths(gpu, oracle):
+    d = scenes.large_scene(64, 36, 1, num_spheres=200, num_emitters=16)
+    exact, bad, L_o = _paths_parity(gpu, oracle, d, cap=0, use_bvh=True, spp=1)
+    assert bad.sum() == 0

@@ -59,3 +59,20 @@ def test_queue_indirection(gpu, oracle):
     t_g, p_g = G.trace_closest(ps.getBVH(), org4, dir4, queue=queue)
     t_o, p_o = oracle.trace_closest(os_, org4[queue], dir4[queue], use_bvh=False)
     _compare(t_g[queue], p_g[queue], t_o, p_o)
+
+
+def test_large_incoherent_scene_trace(gpu, oracle):
+    """200 displaced icospheres (1.02M triangles) in a 100^3 box: GPU LBVH +
+    traversal against the oracle's BVH (itself pinned to brute force)."""
+    d = scenes.large_scene(64, 36, 1, num_spheres=200, num_emitters=16)
+    ps = G.product_scene(d)
+    os_ = oracle.build_oracle_scene(d, bvh=True)
+    bvh = ps.getBVH()
+    assert bvh.numTris == d.numTris
+    org4, dir4 = make_rays(d, 40_000, 5)
+    t_g, p_g = G.trace_closest(bvh, org4, dir4)
+    t_o, p_o = oracle.trace_closest(os_, org4, dir4, use_bvh=True)
+    _compare(t_g, p_g, t_o, p_o)
+    t_b, p_b = oracle.trace_closest(os_, org4[:300], dir4[:300], use_bvh=False)
+    _compare(t_g[:300], p_g[:300], t_b, p_b)
+    assert np.array_equal(G.trace_any(bvh, org4, dir4), oracle.trace_any(os_, org4, dir4, use_bvh=True))
