@@ -54,44 +54,90 @@ __global__ void k_init_counters(uint32_t* c) {
     }
 }
 
+// Per-triangle boxes; each block writes its partial scene and centroid
+// bounds (12 floats), reduced by k_reduce_bounds.
 __global__ __launch_bounds__(256) void k_tri_bounds(const float* __restrict__ verts, const uint32_t* __restrict__ idx,
-                                                    uint32_t n, float4* triLo, float4* triHi, uint32_t* counters) {
+                                                    uint32_t n, float4* triLo, float4* triHi, float* partials) {
+    __shared__ float red[4][12];
     uint32_t t = blockIdx.x * 256 + threadIdx.x;
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float v[12];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { v[a] = INFINITY; v[3 + a] = -INFINITY; v[6 + a] = INFINITY; v[9 + a] = -INFINITY; }
     if (t < n) {
         uint32_t i0 = idx[t * 3 + 0], i1 = idx[t * 3 + 1], i2 = idx[t * 3 + 2];
+        float lo[3], hi[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             float x0 = verts[i0 * 3 + a], x1 = verts[i1 * 3 + a], x2 = verts[i2 * 3 + a];
             lo[a] = fminf(x0, fminf(x1, x2));
             hi[a] = fmaxf(x0, fmaxf(x1, x2));
             float c = (lo[a] + hi[a]) * 0.5f;
-            cmin[a] = c; cmax[a] = c;
+            v[a] = c; v[3 + a] = c; v[6 + a] = lo[a]; v[9 + a] = hi[a];
         }
         triLo[t] = make_float4(lo[0], lo[1], lo[2], 0.0f);
         triHi[t] = make_float4(hi[0], hi[1], hi[2], 0.0f);
     }
-    // wave reduction then one atomic per wave per value
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        for (int off = 32; off > 0; off >>= 1) {
-            cmin[a] = fminf(cmin[a], __shfl_xor(cmin[a], off));
-            cmax[a] = fmaxf(cmax[a], __shfl_xor(cmax[a], off));
-            lo[a] = fminf(lo[a], __shfl_xor(lo[a], off));
-            hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off));
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            v[a] = fminf(v[a], __shfl_xor(v[a], off));
+            v[3 + a] = fmaxf(v[3 + a], __shfl_xor(v[3 + a], off));
+            v[6 + a] = fminf(v[6 + a], __shfl_xor(v[6 + a], off));
+            v[9 + a] = fmaxf(v[9 + a], __shfl_xor(v[9 + a], off));
+        }
+    }
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) red[wave][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        uint32_t k = threadIdx.x;
+        bool isMin = (k % 6) < 3;
+        float r = red[0][k];
+        for (int w = 1; w < 4; ++w) r = isMin ? fminf(r, red[w][k]) : fmaxf(r, red[w][k]);
+        partials[(size_t)blockIdx.x * 12 + k] = r;
+    }
+}
+
+// One block folds the per-block partials into the counters record
+// (ordered-uint min/max of centroids and of the scene box).
+__global__ __launch_bounds__(1024) void k_reduce_bounds(const float* __restrict__ partials, uint32_t numBlocks,
+                                                        uint32_t* counters) {
+    __shared__ float red[16][12];
+    float v[12];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { v[a] = INFINITY; v[3 + a] = -INFINITY; v[6 + a] = INFINITY; v[9 + a] = -INFINITY; }
+    for (uint32_t b = threadIdx.x; b < numBlocks; b += 1024) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            float x = partials[(size_t)b * 12 + k];
+            v[k] = ((k % 6) < 3) ? fminf(v[k], x) : fmaxf(v[k], x);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            float x = __shfl_xor(v[k], off);
+            v[k] = ((k % 6) < 3) ? fminf(v[k], x) : fmaxf(v[k], x);
         }
     }
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (cmin[a] <= cmax[a]) {
-                atomicMin(&counters[C_CMIN + a], ordered(cmin[a]));
-                atomicMax(&counters[C_CMAX + a], ordered(cmax[a]));
-                atomicMin(&counters[C_BMIN + a], ordered(lo[a]));
-                atomicMax(&counters[C_BMAX + a], ordered(hi[a]));
-            }
-        }
+        for (int k = 0; k < 12; ++k) red[threadIdx.x >> 6][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        uint32_t k = threadIdx.x;
+        bool isMin = (k % 6) < 3;
+        float r = red[0][k];
+        for (int w = 1; w < 16; ++w) r = isMin ? fminf(r, red[w][k]) : fmaxf(r, red[w][k]);
+        // k: 0-2 centroid min, 3-5 centroid max, 6-8 box min, 9-11 box max
+        uint32_t slot = (k < 3) ? C_CMIN + k : (k < 6) ? C_CMAX + (k - 3) : (k < 9) ? C_BMIN + (k - 6) : C_BMAX + (k - 9);
+        counters[slot] = ordered(r);
     }
 }
 
@@ -133,8 +179,7 @@ __device__ __forceinline__ int delta(const uint32_t* keys, int n, int i, int j) 
     return __clz(ki ^ kj);
 }
 
-__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ keys, int n, BinNode* bnodes,
-                                                uint32_t* parent) {
+__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ keys, int n, BinNode* bnodes) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n - 1) return;
     int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) > 0 ? 1 : -1;
@@ -161,61 +206,67 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ key
     b.hi.w = __uint_as_float(right);
     b.first = (uint32_t)first;
     b.last = (uint32_t)last;
-    // parent table: internal nodes at [0, n-1), leaves at n-1+k
-    parent[(left & kLeafBit) ? (uint32_t)(n - 1) + (left & ~kLeafBit) : left] = (uint32_t)i;
-    parent[(right & kLeafBit) ? (uint32_t)(n - 1) + (right & ~kLeafBit) : right] = (uint32_t)i;
-    if (i == 0) parent[0] = 0xFFFFFFFFu;
 }
 
-__device__ __forceinline__ void store_box(BinNode* b, const float lo[3], const float hi[3]) {
-    uint32_t* L = reinterpret_cast<uint32_t*>(&b->lo);
-    uint32_t* H = reinterpret_cast<uint32_t*>(&b->hi);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        __hip_atomic_store(L + a, __float_as_uint(lo[a]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(H + a, __float_as_uint(hi[a]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
-__device__ __forceinline__ void load_child_box(const BinNode* bnodes, const float4* triLo, const float4* triHi,
-                                               const uint32_t* vals, uint32_t ref, float lo[3], float hi[3]) {
+__device__ __forceinline__ void child_box(const BinNode* bnodes, const float4* triLo, const float4* triHi,
+                                          const uint32_t* vals, uint32_t ref, float lo[3], float hi[3]) {
     if (ref & kLeafBit) {
         uint32_t t = vals[ref & ~kLeafBit];
         float4 a = triLo[t], b = triHi[t];
         lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
         hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
     } else {
-        const uint32_t* L = reinterpret_cast<const uint32_t*>(&bnodes[ref].lo);
-        const uint32_t* H = reinterpret_cast<const uint32_t*>(&bnodes[ref].hi);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = __uint_as_float(__hip_atomic_load(L + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            hi[a] = __uint_as_float(__hip_atomic_load(H + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        }
+        float4 a = bnodes[ref].lo, b = bnodes[ref].hi;
+        lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
+        hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
     }
 }
 
-// Bottom-up boxes: the second child to arrive at a node computes its box.
-__global__ __launch_bounds__(256) void k_refit(int n, BinNode* bnodes, const uint32_t* __restrict__ parent,
-                                               uint32_t* flags, const float4* __restrict__ triLo,
-                                               const float4* __restrict__ triHi, const uint32_t* __restrict__ vals) {
-    int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
-    uint32_t p = parent[(uint32_t)(n - 1) + (uint32_t)k];
-    while (p != 0xFFFFFFFFu) {
-        __threadfence();
-        uint32_t old = atomicAdd(&flags[p], 1u);
-        if (old == 0) return;
-        __threadfence();
-        BinNode* b = &bnodes[p];
-        uint32_t l = __float_as_uint(b->lo.w), r = __float_as_uint(b->hi.w);
-        float llo[3], lhi[3], rlo[3], rhi[3], lo[3], hi[3];
-        load_child_box(bnodes, triLo, triHi, vals, l, llo, lhi);
-        load_child_box(bnodes, triLo, triHi, vals, r, rlo, rhi);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) { lo[a] = fminf(llo[a], rlo[a]); hi[a] = fmaxf(lhi[a], rhi[a]); }
-        store_box(b, lo, hi);
-        p = parent[p];
+// Top-down: the internal children of one level's nodes form the next level
+// (one append per wave).
+__global__ __launch_bounds__(256) void k_expand_level(const uint32_t* __restrict__ level, uint32_t count,
+                                                      const BinNode* __restrict__ bnodes, uint32_t* next,
+                                                      uint32_t* nextCount) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    uint32_t k0 = kEmptyChild, k1 = kEmptyChild;
+    uint32_t num = 0;
+    if (i < count) {
+        const BinNode& b = bnodes[level[i]];
+        uint32_t l = __float_as_uint(b.lo.w), r = __float_as_uint(b.hi.w);
+        if (!(l & kLeafBit)) { k0 = l; num = 1; }
+        if (!(r & kLeafBit)) { if (num) k1 = r; else k0 = r; ++num; }
     }
+    uint32_t incl = num;
+    const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        uint32_t u = __shfl_up(incl, off);
+        if (lane >= off) incl += u;
+    }
+    uint32_t total = __shfl(incl, 63);
+    uint32_t base = 0;
+    if (lane == 0 && total) base = atomicAdd(nextCount, total);
+    base = __shfl(base, 0);
+    uint32_t pos = base + incl - num;
+    if (num > 0) next[pos] = k0;
+    if (num > 1) next[pos + 1] = k1;
+}
+
+// Bottom-up: boxes of one level from its children, whose boxes (deeper
+// levels) were written by earlier launches.
+__global__ __launch_bounds__(256) void k_refit_level(const uint32_t* __restrict__ level, uint32_t count,
+                                                     BinNode* bnodes, const float4* __restrict__ triLo,
+                                                     const float4* __restrict__ triHi,
+                                                     const uint32_t* __restrict__ vals) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    BinNode* b = &bnodes[level[i]];
+    uint32_t l = __float_as_uint(b->lo.w), r = __float_as_uint(b->hi.w);
+    float llo[3], lhi[3], rlo[3], rhi[3];
+    child_box(bnodes, triLo, triHi, vals, l, llo, lhi);
+    child_box(bnodes, triLo, triHi, vals, r, rlo, rhi);
+    b->lo = make_float4(fminf(llo[0], rlo[0]), fminf(llo[1], rlo[1]), fminf(llo[2], rlo[2]), __uint_as_float(l));
+    b->hi = make_float4(fmaxf(lhi[0], rhi[0]), fmaxf(lhi[1], rhi[1]), fmaxf(lhi[2], rhi[2]), __uint_as_float(r));
 }
 
 __device__ __forceinline__ float half_area(const float lo[3], const float hi[3]) {
@@ -508,7 +559,7 @@ using namespace wrk;
 
 namespace {
 struct BuildLayout {
-    size_t triLo, triHi, keysA, keysB, valsA, valsB, bnodes, parent, flags, tasksA, tasksB, counters, sortTemp;
+    size_t triLo, triHi, keysA, keysB, valsA, valsB, bnodes, levels, partials, tasksA, tasksB, counters, sortTemp;
     size_t sortTempBytes, total;
 };
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -524,8 +575,8 @@ BuildLayout layout_for(uint32_t n) {
     L.valsA = take(nn * 4);
     L.valsB = take(nn * 4);
     L.bnodes = take(nn * sizeof(BinNode));
-    L.parent = take(2 * nn * 4);
-    L.flags = take(nn * 4);
+    L.levels = take(nn * 4);
+    L.partials = take(((nn + 255) / 256) * 12 * 4);
     L.tasksA = take(nn * 16);
     L.tasksB = take(nn * 16);
     L.counters = take(C_COUNT * 4);
@@ -564,8 +615,8 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
     uint32_t* valsA = (uint32_t*)(base + L.valsA);
     uint32_t* valsB = (uint32_t*)(base + L.valsB);
     BinNode* bnodes = (BinNode*)(base + L.bnodes);
-    uint32_t* parent = (uint32_t*)(base + L.parent);
-    uint32_t* flags = (uint32_t*)(base + L.flags);
+    uint32_t* levels = (uint32_t*)(base + L.levels);
+    float* partials = (float*)(base + L.partials);
     uint4* tasksA = (uint4*)(base + L.tasksA);
     uint4* tasksB = (uint4*)(base + L.tasksB);
     uint32_t* counters = (uint32_t*)(base + L.counters);
@@ -573,7 +624,8 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
     const uint32_t g = ceil_div(n, 256);
 
     hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(64), 0, s, counters);
-    hipLaunchKernelGGL(k_tri_bounds, dim3(g), dim3(256), 0, s, vertices, indices, n, triLo, triHi, counters);
+    hipLaunchKernelGGL(k_tri_bounds, dim3(g), dim3(256), 0, s, vertices, indices, n, triLo, triHi, partials);
+    hipLaunchKernelGGL(k_reduce_bounds, dim3(1), dim3(1024), 0, s, partials, g, counters);
     hipLaunchKernelGGL(k_morton, dim3(g), dim3(256), 0, s, triLo, triHi, n, counters, keysA, valsA);
     WRK_CHECK_LAUNCH();
     size_t tmp = L.sortTempBytes;
@@ -581,9 +633,35 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
                                                    30u, s);
     WRK_HIP_TRY(sortErr);
     if (n > 1) {
-        WRK_HIP_TRY(hipMemsetAsync(flags, 0, (size_t)n * 4, s));
-        hipLaunchKernelGGL(k_karras, dim3(ceil_div(n - 1, 256)), dim3(256), 0, s, keysB, (int)n, bnodes, parent);
-        hipLaunchKernelGGL(k_refit, dim3(g), dim3(256), 0, s, (int)n, bnodes, parent, flags, triLo, triHi, valsB);
+        hipLaunchKernelGGL(k_karras, dim3(ceil_div(n - 1, 256)), dim3(256), 0, s, keysB, (int)n, bnodes);
+        WRK_CHECK_LAUNCH();
+        // binary levels top-down (each internal node lands in exactly one level) ...
+        uint32_t zero = 0;
+        WRK_HIP_TRY(hipMemcpyAsync(levels, &zero, 4, hipMemcpyHostToDevice, s));
+        std::vector<uint32_t> off = {0u, 1u};
+        uint32_t next = 0;
+        for (;;) {
+            size_t L0 = off.size() - 2;
+            uint32_t cnt = off[L0 + 1] - off[L0];
+            WRK_HIP_TRY(hipMemsetAsync(counters + C_TASKS_NEXT, 0, 4, s));
+            hipLaunchKernelGGL(k_expand_level, dim3(ceil_div(cnt, 256)), dim3(256), 0, s, levels + off[L0], cnt,
+                               bnodes, levels + off[L0 + 1], counters + C_TASKS_NEXT);
+            WRK_CHECK_LAUNCH();
+            WRK_HIP_TRY(hipMemcpyAsync(&next, counters + C_TASKS_NEXT, 4, hipMemcpyDeviceToHost, s));
+            WRK_HIP_TRY(hipStreamSynchronize(s));
+            if (next == 0) break;
+            if ((uint64_t)off.back() + next > (uint64_t)(n - 1)) {
+                set_error("wrk_bvh_build: malformed binary hierarchy");
+                return WRK_ERR_INVALID;
+            }
+            off.push_back(off.back() + next);
+        }
+        // ... then boxes bottom-up, one launch per level
+        for (size_t L0 = off.size() - 1; L0-- > 0;) {
+            uint32_t cnt = off[L0 + 1] - off[L0];
+            hipLaunchKernelGGL(k_refit_level, dim3(ceil_div(cnt, 256)), dim3(256), 0, s, levels + off[L0], cnt, bnodes,
+                               triLo, triHi, valsB);
+        }
         WRK_CHECK_LAUNCH();
     }
     // level-by-level collapse; root 4-wide node is slot 0

@@ -44,8 +44,8 @@ def test_partition_is_stable(gpu, K, n):
     scratch = torch.empty(sb, dtype=torch.uint8, device="cuda")
     out = torch.zeros(n, dtype=torch.int32, device="cuda")
     offs = torch.zeros(K + 1, dtype=torch.int32, device="cuda")
-    G.check(L.wrk_partition(G.vp(G.dev(cat)), G.vp(G.dev(items)), n, K, G.vp(out), G.vp(offs), G.vp(scratch), sb,
-                            G.stream()))
+    cat_d, items_d = G.dev(cat), G.dev(items)  # keep the device buffers alive across the launch
+    G.check(L.wrk_partition(G.vp(cat_d), G.vp(items_d), n, K, G.vp(out), G.vp(offs), G.vp(scratch), sb, G.stream()))
     torch.cuda.synchronize()
     o = offs.cpu().numpy()
     got = out.cpu().numpy().view(np.uint32)
