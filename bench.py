@@ -146,6 +146,15 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] copy probe: 3 x {nbytes} bytes read + written")
 
+    # HBM bytes per launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+    # (profiles/hbm_traffic_pmc.md): measured bytes per ray x rays per launch
+    traffic = None
+    tfile = ROOT / "profiles" / "hbm_traffic.json"
+    if tfile.exists():
+        t = json.loads(tfile.read_text()).get(args.workload)
+        if t:
+            traffic = round(t["hbm_bytes_per_ray"] * rays_c / max(launches_c, 1))
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(desc, W, H, args.cpu_seconds)
@@ -178,7 +187,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE+WRITE_SIZE, calibrated)",
+            "algorithmic_bytes_per_launch": round(bytes_per_ray * rays_c / max(launches_c, 1)),
             "bytes_per_ray": round(bytes_per_ray, 2),
             "nodes_per_ray": round(nodes_per_ray, 3),
             "tris_per_ray": round(tris_per_ray, 3),
