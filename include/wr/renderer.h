@@ -61,6 +61,7 @@ struct RenderStats {
 };
 
 struct RendererImpl;
+struct ServerImpl;
 
 class Renderer {
 public:
@@ -85,6 +86,50 @@ public:
 private:
     RendererImpl* m_impl;
     RendererConfig m_config;
+};
+
+// --- additive: continuous-batching render server -----------------------------
+// Requests of different scenes (up to 4 resident), resolutions and spp share
+// one wavefront: each iteration traces and shades every in-flight path of
+// every admitted request, and new requests are admitted (in chunks of at
+// most maxChunkPaths paths) as soon as path slots and the memory budget
+// allow. A request's image does not depend on what else is in flight.
+struct ServerConfig {
+    uint32_t maxPathsInFlight = 1u << 26;  // path pool
+    uint32_t maxChunkPaths = 1u << 24;     // admission unit (paths per request chunk)
+    uint64_t memoryBudgetBytes = 0;        // accumulation + image buffers of admitted requests; 0 = unlimited
+    uint32_t iterationCap = 0;             // as WavefrontConfig::iterationCap
+    bool timeKernels = false;
+};
+
+struct RequestStatus {
+    bool done = false;
+    bool admitted = false;
+    double submitTime = 0, admitTime = 0, doneTime = 0;  // RenderServer::clock() seconds
+    uint64_t paths = 0;
+};
+
+class RenderServer {
+public:
+    explicit RenderServer(const ServerConfig& config = {});
+    ~RenderServer();
+    RenderServer(const RenderServer&) = delete;
+    RenderServer& operator=(const RenderServer&) = delete;
+
+    uint64_t submit(Scene* scene, const Camera& camera, const RenderParams& params);
+    // Admission, one wavefront iteration and completions; true while work remains.
+    bool step();
+    bool isDone(uint64_t id) const;
+    RequestStatus status(uint64_t id) const;
+    // Averaged image (width*height Vec3) of a finished request; releases it.
+    void fetch(uint64_t id, Vec3* out);
+    uint32_t numPending() const;
+    uint32_t numInFlight() const;
+    const RenderStats& getStats() const;
+    static double clock();
+
+private:
+    ServerImpl* m_impl;
 };
 
 }  // namespace wr

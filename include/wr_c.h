@@ -72,6 +72,32 @@ int wr_c_render_paths(void* renderer, void* scene, const wr_c_camera* cam, const
                       uint64_t firstPath, uint32_t count, float* out);
 int wr_c_renderer_stats(void* renderer, wr_c_stats* out);
 
+/* --- continuous-batching render server (wr::RenderServer) --- */
+typedef struct wr_c_server_config {
+    uint32_t maxPathsInFlight, maxChunkPaths;
+    uint64_t memoryBudgetBytes;
+    uint32_t iterationCap, flags; /* flags bit0: time kernels */
+} wr_c_server_config;
+
+typedef struct wr_c_request_status {
+    uint32_t done, admitted;
+    double submitTime, admitTime, doneTime;
+    uint64_t paths;
+} wr_c_request_status;
+
+void* wr_c_server_create(const wr_c_server_config* cfg);            /* RenderServer(ServerConfig) */
+void wr_c_server_destroy(void* server);
+/* returns the request id, 0 on error */
+uint64_t wr_c_server_submit(void* server, void* scene, const wr_c_camera* cam, const wr_c_render_params* params);
+/* 1 = work remains, 0 = idle, -1 = error */
+int wr_c_server_step(void* server);
+int wr_c_server_status(void* server, uint64_t id, wr_c_request_status* out);
+int wr_c_server_fetch(void* server, uint64_t id, float* out);      /* width*height*3 floats */
+int wr_c_server_stats(void* server, wr_c_stats* out);
+uint32_t wr_c_server_pending(void* server);
+uint32_t wr_c_server_in_flight(void* server);
+double wr_c_server_clock(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -198,27 +198,49 @@ int wrk_partition(const uint8_t* category, const uint32_t* in, uint32_t n, uint3
                   void* stream);
 
 /* ---------------------------------------------------------- wavefront --- */
-/* Paths i < n of a batch start as camera rays for (pixel = (first + i) % npix,
- * sample = (first + i) / npix) with npix = width*height (trace.cu:53-125). */
-int wrk_generate(const wrk_camera* cam, const wrk_render_params* params, const wrk_paths* paths,
-                 uint64_t first, uint32_t n, uint32_t* queue, void* stream);
-/* Resolves the trace results of the active queue [closest | shadow] and
- * assigns each path a shade category (trace.cu:27-51 shadow resolve,
- * trace.cu:143-148 miss flush, gpu_types.h:11-17 material kinds). */
-int wrk_classify(const wrk_scene_view* scene, const wrk_render_params* params, const wrk_paths* paths,
+/* Path state flags (uint4 state.z): depth bits 0..15, isFirstHit 16,
+ * prevWasDelta 17, neeDone 18, host-loop iterations 19..23, request slot
+ * 24..31. A slot indexes a device table of wrk_render_params, so paths of
+ * different requests (resolution, spp, depth, iteration cap) share launches. */
+#define WRK_MAX_SLOTS 256
+
+/* Camera rays for paths k < n of one chunk: global path g = first + k of the
+ * request (pixel = g % (width*height), sample = g / (width*height)),
+ * stored at pool position poolBase + k, tagged with `slot`
+ * (trace.cu:53-125). queue[k] = poolBase + k; if category is non-NULL,
+ * category[k] = categoryValue. */
+int wrk_generate(const wrk_camera* cam, const wrk_render_params* params, const wrk_paths* paths, uint64_t first,
+                 uint32_t n, uint32_t poolBase, uint32_t slot, uint32_t* queue, uint8_t* category,
+                 uint32_t categoryValue, void* stream);
+/* Resolves the trace results of the active queue [closest | shadow] of one
+ * scene and assigns each path a shade category 0..4 (material kinds of
+ * gpu_types.h:11-17) or 5 = finished (trace.cu:27-51 shadow resolve,
+ * trace.cu:143-148 miss flush). paramsTable is a device array indexed by the
+ * path's slot. */
+int wrk_classify(const wrk_scene_view* scene, const wrk_render_params* paramsTable, const wrk_paths* paths,
                  const uint32_t* queue, uint32_t numClosest, uint32_t numShadow, uint8_t* category,
                  void* stream);
-/* Runs the material shaders over the material-sorted queue (offsets as from
- * wrk_partition with 5 categories): shade.cu:24-85 and the materials/ headers.
- * nextStage[i] = 0 trace, 1 shadow, 2 finished, for queue position i.
- * lastIteration != 0 drops paths that are still alive (renderer.cpp:213). */
-int wrk_shade(const wrk_scene_view* scene, const wrk_render_params* params, const wrk_paths* paths,
-              const uint32_t* queue, const uint32_t* offsets, uint32_t maxItems, uint32_t lastIteration,
-              uint8_t* nextStage, void* stream);
-/* accum[pixel] += result[path] for the batch paths in sample order
- * (renderer.cpp:190 serial samples; accumulation order is per pixel). */
-int wrk_accumulate(const wrk_paths* paths, uint64_t first, uint32_t n, uint32_t npix, float* accum,
-                   void* stream);
+/* Runs the material shaders over the material-sorted queue of one scene
+ * (offsets as from wrk_partition with 5 categories): shade.cu:24-85 and the
+ * materials/ headers. nextCategory[i] = categoryBase + 0 (trace next) or
+ * categoryBase + 1 (shadow test next) or 0xFF (finished) for queue
+ * positions i < maxItems. Paths reaching their slot's iterationCap are
+ * dropped unflushed (renderer.cpp:213). */
+int wrk_shade(const wrk_scene_view* scene, const wrk_render_params* paramsTable, const wrk_paths* paths,
+              const uint32_t* queue, const uint32_t* offsets, uint32_t maxItems, uint32_t categoryBase,
+              uint8_t* nextCategory, void* stream);
+/* Live paths per slot (no counterpart in libWR, whose host loop renders one
+ * batch at a time): counts[ranges[numRanges + r]] = number of queue entries
+ * inside pool range r, for the first n = min(*countDev, maxItems) entries
+ * (n = maxItems when countDev is NULL). ranges[0 .. numRanges) are the
+ * ascending first pool positions of the disjoint ranges that hold every
+ * live path; counts[WRK_MAX_SLOTS] is cleared first. */
+int wrk_slot_count(const uint32_t* queue, uint32_t maxItems, const uint32_t* countDev, const uint32_t* ranges,
+                   uint32_t numRanges, uint32_t* counts, void* stream);
+/* accum[pixel] += result[poolBase + k] for the chunk's paths in sample order
+ * (renderer.cpp:190 serial samples; per-pixel accumulation order is fixed). */
+int wrk_accumulate(const wrk_paths* paths, uint32_t poolBase, uint64_t first, uint32_t n, uint32_t npix,
+                   float* accum, void* stream);
 /* out = accum * (1/spp) (renderer.cpp:308-313). */
 int wrk_average(const float* accum, uint32_t npix, uint32_t spp, float* out, void* stream);
 /* Exposure 2.2, ACES (Narkowicz), gamma 1/2.2, truncation to uint8

@@ -68,6 +68,16 @@ class wr_c_stats(C.Structure):
                 ("closestTriFetches", C.c_uint64), ("anyNodeFetches", C.c_uint64), ("anyTriFetches", C.c_uint64)]
 
 
+class wr_c_server_config(C.Structure):
+    _fields_ = [("maxPathsInFlight", C.c_uint32), ("maxChunkPaths", C.c_uint32), ("memoryBudgetBytes", C.c_uint64),
+                ("iterationCap", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class wr_c_request_status(C.Structure):
+    _fields_ = [("done", C.c_uint32), ("admitted", C.c_uint32), ("submitTime", C.c_double),
+                ("admitTime", C.c_double), ("doneTime", C.c_double), ("paths", C.c_uint64)]
+
+
 _P = C.c_void_p
 _SIGS = {
     # wr_c_*
@@ -99,6 +109,16 @@ _SIGS = {
     "wr_c_render_paths": (C.c_int, [_P, _P, C.POINTER(wr_c_camera), C.POINTER(wr_c_render_params), C.c_uint64,
                                     C.c_uint32, _P]),
     "wr_c_renderer_stats": (C.c_int, [_P, C.POINTER(wr_c_stats)]),
+    "wr_c_server_create": (_P, [C.POINTER(wr_c_server_config)]),
+    "wr_c_server_destroy": (None, [_P]),
+    "wr_c_server_submit": (C.c_uint64, [_P, _P, C.POINTER(wr_c_camera), C.POINTER(wr_c_render_params)]),
+    "wr_c_server_step": (C.c_int, [_P]),
+    "wr_c_server_status": (C.c_int, [_P, C.c_uint64, C.POINTER(wr_c_request_status)]),
+    "wr_c_server_fetch": (C.c_int, [_P, C.c_uint64, _P]),
+    "wr_c_server_stats": (C.c_int, [_P, C.POINTER(wr_c_stats)]),
+    "wr_c_server_pending": (C.c_uint32, [_P]),
+    "wr_c_server_in_flight": (C.c_uint32, [_P]),
+    "wr_c_server_clock": (C.c_double, []),
     # wrk_*
     "wrk_last_error": (C.c_char_p, []),
     "wrk_device_count": (C.c_int, [C.POINTER(C.c_int)]),
@@ -113,10 +133,12 @@ _SIGS = {
     "wrk_compact_scratch_bytes": (C.c_uint64, [C.c_uint32]),
     "wrk_compact": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P, C.c_uint64, _P]),
     "wrk_partition": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint64, _P]),
-    "wrk_generate": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_uint32, _P, _P]),
+    "wrk_generate": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32,
+                               _P]),
     "wrk_classify": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_shade": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, _P]),
-    "wrk_accumulate": (C.c_int, [_P, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P]),
+    "wrk_slot_count": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, _P, _P]),
+    "wrk_accumulate": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_average": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_tonemap_aces": (C.c_int, [_P, C.c_uint32, _P, _P]),
 }
@@ -382,6 +404,78 @@ class Renderer:
         s = wr_c_stats()
         _check(lib().wr_c_renderer_stats(self._h, C.byref(s)), "getStats")
         return {name: getattr(s, name) for name, _ in wr_c_stats._fields_}
+
+
+@dataclass
+class ServerConfig:
+    """Continuous-batching server (wr::RenderServer): path pool, admission
+    chunk, memory budget for accumulation/image buffers (0 = unlimited)."""
+    maxPathsInFlight: int = 1 << 26
+    maxChunkPaths: int = 1 << 24
+    memoryBudgetBytes: int = 0
+    iterationCap: int = 0
+    timeKernels: bool = False
+
+
+class RenderServer:
+    """Requests of different scenes, resolutions and spp share one wavefront;
+    submit() queues, step() advances every in-flight request by one bounce
+    iteration and admits new ones as path slots and memory free up."""
+
+    def __init__(self, cfg: ServerConfig | None = None):
+        c = cfg or ServerConfig()
+        cc = wr_c_server_config(int(c.maxPathsInFlight), int(c.maxChunkPaths), int(c.memoryBudgetBytes),
+                                int(c.iterationCap), 1 if c.timeKernels else 0)
+        self._h = lib().wr_c_server_create(C.byref(cc))
+        if not self._h:
+            raise RuntimeError(f"RenderServer: {_err()}")
+        self._shapes = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.wr_c_server_destroy(h)
+            self._h = None
+
+    def submit(self, scene: Scene, camera: Camera, params: RenderParams) -> int:
+        cam, prm = camera.to_c(), params.to_c()
+        rid = lib().wr_c_server_submit(self._h, scene._h, C.byref(cam), C.byref(prm))
+        if rid == 0:
+            raise RuntimeError(f"submit: {_err()}")
+        self._shapes[rid] = (params.height, params.width)
+        return rid
+
+    def step(self) -> bool:
+        r = lib().wr_c_server_step(self._h)
+        if r < 0:
+            raise RuntimeError(f"step: {_err()}")
+        return r == 1
+
+    def status(self, rid: int) -> dict:
+        st = wr_c_request_status()
+        _check(lib().wr_c_server_status(self._h, rid, C.byref(st)), "status")
+        return {name: getattr(st, name) for name, _ in wr_c_request_status._fields_}
+
+    def fetch(self, rid: int) -> np.ndarray:
+        h, w = self._shapes.pop(rid)
+        out = np.zeros((h, w, 3), dtype=np.float32)
+        _check(lib().wr_c_server_fetch(self._h, rid, _ptr(out)), "fetch")
+        return out
+
+    def getStats(self) -> dict:
+        s = wr_c_stats()
+        _check(lib().wr_c_server_stats(self._h, C.byref(s)), "getStats")
+        return {name: getattr(s, name) for name, _ in wr_c_stats._fields_}
+
+    def numPending(self) -> int:
+        return int(lib().wr_c_server_pending(self._h))
+
+    def numInFlight(self) -> int:
+        return int(lib().wr_c_server_in_flight(self._h))
+
+    @staticmethod
+    def clock() -> float:
+        return float(lib().wr_c_server_clock())
 
 
 def aces_tonemap(image: np.ndarray) -> np.ndarray:

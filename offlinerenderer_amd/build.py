@@ -26,6 +26,7 @@ SOURCES = [
     "partition.hip",
     "wavefront.hip",
     "wrk_api.cpp",
+    "engine.cpp",
     "wr_host.cpp",
     "wr_capi.cpp",
     "image_io.cpp",

@@ -1,0 +1,144 @@
+// engine.h — the wavefront scheduler shared by wr::Renderer and
+// wr::RenderServer. One path pool holds paths of several requests (up to
+// WRK_MAX_SLOTS admitted chunks, up to 4 resident scenes); every iteration
+// traces, classifies and shades each scene's share of the merged queue and
+// rebuilds the queue with one ordered partition. Replaces the host loop of
+// libWR/src/internal/renderer.cpp:190-284.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "../../include/wr/wr.h"
+#include "../../include/wrk.h"
+
+namespace wr {
+namespace detail {
+
+void hip_check(hipError_t e, const char* what);
+void wrk_check(int rc, const char* what);
+
+// Owning device allocation.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); ptr = o.ptr; bytes = o.bytes; o.ptr = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release();
+    void alloc(size_t n);
+    void upload(const void* src, size_t n);
+    uint64_t addr() const { return (uint64_t)(uintptr_t)ptr; }
+    template <typename T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+wrk_camera camera_record(const Camera& c);
+
+struct EngineConfig {
+    uint32_t capacity = 1u << 26;       // paths in flight
+    uint32_t maxChunkPaths = 0;          // 0 = capacity
+    uint64_t memoryBudgetBytes = 0;      // 0 = unlimited (beyond the pool itself)
+    bool timeKernels = false;
+    bool countTraversal = false;
+};
+
+class Engine {
+public:
+    enum class Mode { Accumulate, Paths };
+    struct RequestState {
+        uint64_t id = 0;
+        Scene* scene = nullptr;
+        uint32_t sceneIdx = 0;
+        wrk_camera cam{};
+        wrk_render_params prm{};
+        Mode mode = Mode::Accumulate;
+        uint64_t firstPath = 0, endPath = 0, nextPath = 0;
+        uint32_t spp = 1, npix = 0;
+        float* pathsOut = nullptr;  // host, Mode::Paths
+        DevBuf accum, image;
+        bool inFlight = false, done = false, admitted = false;
+        double tSubmit = 0, tAdmit = 0, tDone = 0;
+        uint64_t footprint = 0;
+    };
+
+    explicit Engine(const EngineConfig& cfg);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    void setConfig(const EngineConfig& cfg);
+    void ensureCapacity(uint32_t cap) { reserve(cap); }
+    const EngineConfig& config() const { return cfg_; }
+
+    // Queues a request for paths [firstPath, firstPath + numPaths) of the
+    // image described by params (numPaths = W*H*spp for a whole render).
+    uint64_t submit(Scene* scene, const wrk_camera& cam, const wrk_render_params& prm, uint32_t spp,
+                    uint64_t firstPath, uint64_t numPaths, Mode mode, float* pathsOut);
+    // Admission + one wavefront iteration + completions. Returns true while
+    // any request is pending or in flight.
+    bool step();
+    bool isDone(uint64_t id) const;
+    const RequestState& request(uint64_t id) const;
+    // Copies the averaged image (W*H*3 floats) of a finished Accumulate
+    // request to host memory and releases the request.
+    void fetch(uint64_t id, float* out);
+    void release(uint64_t id);
+    size_t numPending() const { return pending_.size(); }
+    size_t numActiveSlots() const;
+    RenderStats& stats() { return stats_; }
+    static double now();
+
+private:
+    struct Slot {
+        bool active = false;
+        uint64_t request = 0;
+        uint32_t poolBase = 0, count = 0;
+        uint64_t firstPath = 0;
+    };
+    void reserve(uint32_t cap);
+    uint32_t sceneIndex(Scene* s);
+    bool allocRange(uint32_t n, uint32_t* base);
+    void freeRange(uint32_t base, uint32_t n);
+    uint32_t largestFree() const;
+    void complete(uint32_t slot);
+    template <typename F> void timed(int category, F&& launch);
+
+    EngineConfig cfg_;
+    RenderStats stats_;
+    hipStream_t stream_ = nullptr;
+    uint32_t capacity_ = 0;
+    DevBuf org_, dir_, thr_, rad_, st_, contrib_, savedOrg_, savedDir_, hit_, occl_, result_;
+    DevBuf qA_, qB_, cat_, scratch_, offs_, prmTable_, live_, ranges_, fetch_;
+    wrk_paths paths_{};
+    uint32_t* hostOffs_ = nullptr;        // pinned
+    uint32_t* hostLive_ = nullptr;        // pinned
+    wrk_render_params* hostPrm_ = nullptr;  // pinned staging for the slot table
+    uint32_t* hostRanges_ = nullptr;      // pinned: [first pool positions | slots], ascending
+    bool rangesDirty_ = true;
+    std::vector<Scene*> scenes_;
+    std::vector<uint32_t> segC_, segS_;   // per-scene closest/shadow queue sizes
+    std::map<uint32_t, uint32_t> free_;   // pool ranges: base -> length
+    Slot slots_[WRK_MAX_SLOTS];
+    std::vector<uint32_t> freeSlots_;
+    std::deque<uint64_t> pending_;
+    std::map<uint64_t, std::unique_ptr<RequestState>> requests_;
+    uint64_t nextId_ = 1;
+    uint64_t footprintUsed_ = 0;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks_;
+    std::vector<hipEvent_t> eventPool_;
+    size_t eventNext_ = 0;
+};
+
+}  // namespace detail
+}  // namespace wr

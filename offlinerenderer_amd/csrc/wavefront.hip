@@ -64,11 +64,13 @@ static SceneDev scene_of(const wrk_scene_view* v) {
 // ------------------------------------------------------------- generate
 // trace.cu:53-125: camera ray for (pixel, sample), path state reset.
 __global__ __launch_bounds__(256) void k_generate(wrk_camera cam, wrk_render_params prm, Pool P, uint64_t first,
-                                                  uint32_t n, uint32_t* queue) {
-    uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+                                                  uint32_t n, uint32_t poolBase, uint32_t slot, uint32_t* queue,
+                                                  uint8_t* cat, uint32_t catValue) {
+    uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = poolBase + k;
     const uint64_t npix = (uint64_t)prm.width * prm.height;
-    const uint64_t g = first + i;
+    const uint64_t g = first + k;
     const uint32_t pixel = (uint32_t)(g % npix);
     const uint32_t sample = (uint32_t)(g / npix);
     const uint32_t px = pixel % prm.width;
@@ -105,9 +107,10 @@ __global__ __launch_bounds__(256) void k_generate(wrk_camera cam, wrk_render_par
     P.dir[i] = f4(d, 1e20f);
     P.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);  // prevPdf = 1
     P.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.st[i] = make_uint4(seed, 0u, FLAG_FIRST_HIT | FLAG_PREV_DELTA, pixel);
+    P.st[i] = make_uint4(seed, 0u, FLAG_FIRST_HIT | FLAG_PREV_DELTA | (slot << FLAG_SLOT_SHIFT), pixel);
     P.result[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    queue[i] = i;
+    queue[k] = i;
+    if (cat) cat[k] = (uint8_t)catValue;
 }
 
 // ------------------------------------------------------------- classify
@@ -118,8 +121,8 @@ __global__ __launch_bounds__(256) void k_generate(wrk_camera cam, wrk_render_par
 // path without a flush, shade.cu:35-38). Shadow results (queue[nC, nC+nS)):
 // add the NEE contribution when unoccluded, restore the direction, and
 // resume in the Lambertian shader (trace.cu:27-51).
-__global__ __launch_bounds__(256) void k_classify(SceneDev S, wrk_render_params prm, Pool P, const uint32_t* queue,
-                                                  uint32_t nC, uint32_t nS, uint8_t* cat) {
+__global__ __launch_bounds__(256) void k_classify(SceneDev S, const wrk_render_params* __restrict__ prmTable, Pool P,
+                                                  const uint32_t* queue, uint32_t nC, uint32_t nS, uint8_t* cat) {
     uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= nC + nS) return;
     uint32_t p = queue[i];
@@ -128,8 +131,9 @@ __global__ __launch_bounds__(256) void k_classify(SceneDev S, wrk_render_params 
         uint2 h = P.hit[p];
         if (h.y == 0xFFFFFFFFu) {
             float4 T = P.thr[p], L = P.rad[p];
+            const uint32_t slot = P.st[p].z >> FLAG_SLOT_SHIFT;
             f3 Lr = mk(L.x + T.x * S.env[0], L.y + T.y * S.env[1], L.z + T.z * S.env[2]);
-            f3 Lc = clamp_fireflies(Lr, prm.fireflyClamp);
+            f3 Lc = clamp_fireflies(Lr, prmTable[slot].fireflyClamp);
             P.rad[p] = f4(Lr, 0.0f);
             P.result[p] = f4(Lc, 0.0f);
             c = CAT_DONE;
@@ -458,9 +462,9 @@ __device__ __forceinline__ f3 effective_normal(const SceneDev& S, const wrk_mate
 // five category segments in order, so every block runs one material
 // shader; the trailing blocks mark queue positions past the last segment
 // as finished.
-__global__ __launch_bounds__(256) void k_shade(SceneDev S, wrk_render_params prm, Pool P, const uint32_t* queue,
-                                               const uint32_t* __restrict__ offsets, uint32_t maxItems,
-                                               uint32_t lastIteration, uint8_t* nextStage) {
+__global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_params* __restrict__ prmTable, Pool P,
+                                               const uint32_t* queue, const uint32_t* __restrict__ offsets,
+                                               uint32_t maxItems, uint32_t catBase, uint8_t* nextStage) {
     uint32_t off[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) off[k] = offsets[k];
@@ -475,7 +479,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, wrk_render_params prm
     }
     if (seg < 0) {
         uint32_t j = off[5] + b * 256 + threadIdx.x;
-        if (j < maxItems) nextStage[j] = NEXT_DONE;
+        if (j < maxItems) nextStage[j] = 0xFFu;
         return;
     }
     uint32_t j = off[seg] + b * 256 + threadIdx.x;
@@ -484,6 +488,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, wrk_render_params prm
 
     PathState r;
     uint4 st = P.st[p];
+    const uint32_t slot = st.z >> FLAG_SLOT_SHIFT;
+    const wrk_render_params prm = prmTable[slot];
+    const uint32_t iter = ((st.z & FLAG_ITER_MASK) >> FLAG_ITER_SHIFT) + 1u;
     r.seed = st.x; r.dim = st.y;
     r.depth = st.z & FLAG_DEPTH_MASK;
     r.firstHit = (st.z & FLAG_FIRST_HIT) != 0;
@@ -532,7 +539,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, wrk_render_params prm
     if (r.done) {
         next = NEXT_DONE;
         if (r.flushed) P.result[p] = f4(r.flushL, 0.0f);
-    } else if (lastIteration) {
+    } else if (prm.iterationCap != 0 && iter >= prm.iterationCap) {
         next = NEXT_DONE;  // renderer.cpp:213 cap: the path is dropped unflushed
     } else if (r.toShadow) {
         next = NEXT_SHADOW;
@@ -542,29 +549,81 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, wrk_render_params prm
     } else {
         next = NEXT_TRACE;
     }
-    if (!r.done) {
+    if (next != NEXT_DONE) {
         P.org[p] = f4(r.o, r.tMin);
         P.dir[p] = f4(r.d, r.tMax);
         P.thr[p] = f4(r.T, r.prevPdf);
         P.rad[p] = f4(r.L, 0.0f);
         uint32_t flags = (r.depth & FLAG_DEPTH_MASK) | (r.prevDelta ? FLAG_PREV_DELTA : 0u) |
-                         (r.neeDone ? FLAG_NEE_DONE : 0u);
+                         (r.neeDone ? FLAG_NEE_DONE : 0u) | (min(iter, 31u) << FLAG_ITER_SHIFT) |
+                         (slot << FLAG_SLOT_SHIFT);
         P.st[p] = make_uint4(r.seed, r.dim, flags, r.pixel);
     }
-    nextStage[j] = (uint8_t)next;
+    nextStage[j] = next == NEXT_DONE ? (uint8_t)0xFFu : (uint8_t)(catBase + next);
+}
+
+// ------------------------------------------------------------ slot count
+// Live paths per slot = queue entries inside the slot's pool range. A block
+// counts 256 * perThread entries: each lane merges runs of one range before
+// adding them to an LDS histogram, and the block adds its nonzero bins to
+// counts[], so the device atomics per launch stay near one per block.
+__global__ __launch_bounds__(256) void k_clear_counts(uint32_t* counts) { counts[threadIdx.x] = 0u; }
+
+__global__ __launch_bounds__(256) void k_slot_count(const uint32_t* __restrict__ queue, uint32_t maxItems,
+                                                    const uint32_t* __restrict__ countDev,
+                                                    const uint32_t* __restrict__ ranges, uint32_t numRanges,
+                                                    uint32_t perThread, uint32_t* counts) {
+    __shared__ uint32_t base[kMaxSlots];
+    __shared__ uint32_t hist[kMaxSlots];
+    const uint32_t t = threadIdx.x;
+    base[t] = t < numRanges ? ranges[t] : 0xFFFFFFFFu;
+    hist[t] = 0u;
+    __syncthreads();
+    uint32_t n = maxItems;
+    if (countDev) n = min(*countDev, maxItems);
+    const uint64_t start = (uint64_t)blockIdx.x * 256u * perThread;
+    uint32_t cur = 0u, run = 0u;
+    constexpr uint32_t kBatch = 8;
+    for (uint32_t k0 = 0; k0 < perThread; k0 += kBatch) {
+        uint32_t p[kBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+            uint64_t i = start + (uint64_t)(k0 + u) * 256u + t;
+            p[u] = (k0 + u < perThread && i < n) ? queue[i] : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+            if (p[u] == 0xFFFFFFFFu) continue;
+            // last range whose first position is <= p
+            uint32_t lo = 0u, hi = numRanges;
+            while (hi - lo > 1u) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (base[mid] <= p[u]) lo = mid; else hi = mid;
+            }
+            if (lo != cur && run) {
+                atomicAdd(&hist[cur], run);
+                run = 0u;
+            }
+            cur = lo;
+            ++run;
+        }
+    }
+    if (run) atomicAdd(&hist[cur], run);
+    __syncthreads();
+    if (t < numRanges && hist[t]) atomicAdd(&counts[ranges[numRanges + t]], hist[t]);
 }
 
 // accum[pixel] += result[path] over the batch in sample order.
-__global__ __launch_bounds__(256) void k_accumulate(const float4* __restrict__ result, uint64_t first, uint32_t n,
-                                                    uint32_t npix, float* accum) {
+__global__ __launch_bounds__(256) void k_accumulate(const float4* __restrict__ result, uint32_t poolBase,
+                                                    uint64_t first, uint32_t n, uint32_t npix, float* accum) {
     uint32_t k = blockIdx.x * 256 + threadIdx.x;
     uint32_t cover = n < npix ? n : npix;
     if (k >= cover) return;
-    uint64_t g0 = first + k;  // first path of this pixel in the batch
+    uint64_t g0 = first + k;  // first path of this pixel in the chunk
     uint32_t pixel = (uint32_t)(g0 % npix);
     float a0 = accum[pixel * 3 + 0], a1 = accum[pixel * 3 + 1], a2 = accum[pixel * 3 + 2];
     for (uint64_t i = k; i < n; i += npix) {
-        float4 v = result[i];
+        float4 v = result[poolBase + i];
         a0 = a0 + v.x; a1 = a1 + v.y; a2 = a2 + v.z;
     }
     accum[pixel * 3 + 0] = a0; accum[pixel * 3 + 1] = a1; accum[pixel * 3 + 2] = a2;
@@ -594,47 +653,69 @@ __global__ __launch_bounds__(256) void k_aces(const float* __restrict__ img, uin
 using namespace wrk;
 
 extern "C" int wrk_generate(const wrk_camera* cam, const wrk_render_params* params, const wrk_paths* paths,
-                            uint64_t first, uint32_t n, uint32_t* queue, void* stream) {
+                            uint64_t first, uint32_t n, uint32_t poolBase, uint32_t slot, uint32_t* queue,
+                            uint8_t* category, uint32_t categoryValue, void* stream) {
     WRK_REQUIRE(cam && params && paths && queue, "wrk_generate: null argument");
-    WRK_REQUIRE(n <= paths->capacity, "wrk_generate: n exceeds the path pool capacity");
+    WRK_REQUIRE((uint64_t)poolBase + n <= paths->capacity, "wrk_generate: range exceeds the path pool capacity");
     WRK_REQUIRE(params->width > 0 && params->height > 0, "wrk_generate: empty image");
+    WRK_REQUIRE(slot < kMaxSlots, "wrk_generate: slot out of range");
     if (n == 0) return WRK_OK;
     hipLaunchKernelGGL(k_generate, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), *cam, *params,
-                       pool_of(paths), first, n, queue);
+                       pool_of(paths), first, n, poolBase, slot, queue, category, categoryValue);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }
 
-extern "C" int wrk_classify(const wrk_scene_view* scene, const wrk_render_params* params, const wrk_paths* paths,
+extern "C" int wrk_classify(const wrk_scene_view* scene, const wrk_render_params* paramsTable, const wrk_paths* paths,
                             const uint32_t* queue, uint32_t numClosest, uint32_t numShadow, uint8_t* category,
                             void* stream) {
-    WRK_REQUIRE(scene && params && paths && queue && category, "wrk_classify: null argument");
+    WRK_REQUIRE(scene && paramsTable && paths && queue && category, "wrk_classify: null argument");
     uint32_t n = numClosest + numShadow;
     if (n == 0) return WRK_OK;
-    hipLaunchKernelGGL(k_classify, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), scene_of(scene), *params,
-                       pool_of(paths), queue, numClosest, numShadow, category);
+    hipLaunchKernelGGL(k_classify, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), scene_of(scene),
+                       paramsTable, pool_of(paths), queue, numClosest, numShadow, category);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }
 
-extern "C" int wrk_shade(const wrk_scene_view* scene, const wrk_render_params* params, const wrk_paths* paths,
-                         const uint32_t* queue, const uint32_t* offsets, uint32_t maxItems, uint32_t lastIteration,
-                         uint8_t* nextStage, void* stream) {
-    WRK_REQUIRE(scene && params && paths && queue && offsets && nextStage, "wrk_shade: null argument");
+extern "C" int wrk_shade(const wrk_scene_view* scene, const wrk_render_params* paramsTable, const wrk_paths* paths,
+                         const uint32_t* queue, const uint32_t* offsets, uint32_t maxItems, uint32_t categoryBase,
+                         uint8_t* nextCategory, void* stream) {
+    WRK_REQUIRE(scene && paramsTable && paths && queue && offsets && nextCategory, "wrk_shade: null argument");
+    WRK_REQUIRE(categoryBase + 1u < 0xFFu, "wrk_shade: categoryBase out of range");
     if (maxItems == 0) return WRK_OK;
     hipLaunchKernelGGL(k_shade, dim3(ceil_div(maxItems, 256) + 6), dim3(256), 0, as_stream(stream), scene_of(scene),
-                       *params, pool_of(paths), queue, offsets, maxItems, lastIteration, nextStage);
+                       paramsTable, pool_of(paths), queue, offsets, maxItems, categoryBase, nextCategory);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }
 
-extern "C" int wrk_accumulate(const wrk_paths* paths, uint64_t first, uint32_t n, uint32_t npix, float* accum,
-                              void* stream) {
+extern "C" int wrk_slot_count(const uint32_t* queue, uint32_t maxItems, const uint32_t* countDev,
+                              const uint32_t* ranges, uint32_t numRanges, uint32_t* counts, void* stream) {
+    WRK_REQUIRE(counts, "wrk_slot_count: null counts");
+    WRK_REQUIRE(numRanges <= kMaxSlots, "wrk_slot_count: more ranges than slots");
+    hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(kMaxSlots), 0, as_stream(stream), counts);
+    WRK_CHECK_LAUNCH();
+    if (maxItems == 0 || numRanges == 0) return WRK_OK;
+    WRK_REQUIRE(queue && ranges, "wrk_slot_count: null argument");
+    // about 2048 blocks for large queues, one entry per lane for small ones
+    uint32_t perThread = (uint32_t)ceil_div(maxItems, 256ull * 2048ull);
+    perThread = perThread < 1u ? 1u : (perThread > 256u ? 256u : perThread);
+    uint32_t blocks = ceil_div(maxItems, 256ull * perThread);
+    hipLaunchKernelGGL(k_slot_count, dim3(blocks), dim3(256), 0, as_stream(stream), queue, maxItems, countDev,
+                       ranges, numRanges, perThread, counts);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_accumulate(const wrk_paths* paths, uint32_t poolBase, uint64_t first, uint32_t n, uint32_t npix,
+                              float* accum, void* stream) {
     WRK_REQUIRE(paths && accum && npix > 0, "wrk_accumulate: bad argument");
+    WRK_REQUIRE((uint64_t)poolBase + n <= paths->capacity, "wrk_accumulate: range exceeds the path pool");
     if (n == 0) return WRK_OK;
     uint32_t cover = n < npix ? n : npix;
     hipLaunchKernelGGL(k_accumulate, dim3(ceil_div(cover, 256)), dim3(256), 0, as_stream(stream),
-                       (const float4*)paths->result, first, n, npix, accum);
+                       (const float4*)paths->result, poolBase, first, n, npix, accum);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }

@@ -48,6 +48,21 @@ template <typename F> uint32_t guard_id(F&& f) {
     return id;
 }
 wr::Scene* S(void* p) { return static_cast<wr::Scene*>(p); }
+wr::RenderServer* SV(void* p) { return static_cast<wr::RenderServer*>(p); }
+void fill_stats(const wr::RenderStats& st, wr_c_stats* out) {
+    out->seconds = st.seconds;
+    out->paths = st.paths;
+    out->closestRays = st.closestRays;
+    out->shadowRays = st.shadowRays;
+    out->iterations = st.iterations;
+    out->batches = st.batches;
+    out->msGenerate = st.msGenerate; out->msTraceClosest = st.msTraceClosest; out->msTraceAny = st.msTraceAny;
+    out->msClassify = st.msClassify; out->msPartition = st.msPartition; out->msShade = st.msShade;
+    out->msAccumulate = st.msAccumulate;
+    out->launchesTraceClosest = st.launchesTraceClosest; out->launchesTraceAny = st.launchesTraceAny;
+    out->closestNodeFetches = st.closestNodeFetches; out->closestTriFetches = st.closestTriFetches;
+    out->anyNodeFetches = st.anyNodeFetches; out->anyTriFetches = st.anyTriFetches;
+}
 wr::Renderer* R(void* p) { return static_cast<wr::Renderer*>(p); }
 }  // namespace
 
@@ -142,21 +157,52 @@ int wr_c_render_paths(void* r, void* s, const wr_c_camera* cam, const wr_c_rende
     return guard([&] { R(r)->renderPaths(S(s), to_camera(cam), to_params(p), first, count, out); });
 }
 int wr_c_renderer_stats(void* r, wr_c_stats* out) {
+    return guard([&] { fill_stats(R(r)->getStats(), out); });
+}
+
+void* wr_c_server_create(const wr_c_server_config* cfg) {
+    void* out = nullptr;
+    guard([&] {
+        wr::ServerConfig c;
+        c.maxPathsInFlight = cfg->maxPathsInFlight;
+        c.maxChunkPaths = cfg->maxChunkPaths;
+        c.memoryBudgetBytes = cfg->memoryBudgetBytes;
+        c.iterationCap = cfg->iterationCap;
+        c.timeKernels = (cfg->flags & 1u) != 0;
+        out = new wr::RenderServer(c);
+    });
+    return out;
+}
+void wr_c_server_destroy(void* sv) { delete SV(sv); }
+uint64_t wr_c_server_submit(void* sv, void* scene, const wr_c_camera* cam, const wr_c_render_params* p) {
+    uint64_t id = 0;
+    if (guard([&] { id = SV(sv)->submit(S(scene), to_camera(cam), to_params(p)); }) != 0) return 0;
+    return id;
+}
+int wr_c_server_step(void* sv) {
+    int more = 0;
+    if (guard([&] { more = SV(sv)->step() ? 1 : 0; }) != 0) return -1;
+    return more;
+}
+int wr_c_server_status(void* sv, uint64_t id, wr_c_request_status* out) {
     return guard([&] {
-        const wr::RenderStats& st = R(r)->getStats();
-        out->seconds = st.seconds;
+        wr::RequestStatus st = SV(sv)->status(id);
+        out->done = st.done;
+        out->admitted = st.admitted;
+        out->submitTime = st.submitTime;
+        out->admitTime = st.admitTime;
+        out->doneTime = st.doneTime;
         out->paths = st.paths;
-        out->closestRays = st.closestRays;
-        out->shadowRays = st.shadowRays;
-        out->iterations = st.iterations;
-        out->batches = st.batches;
-        out->msGenerate = st.msGenerate; out->msTraceClosest = st.msTraceClosest; out->msTraceAny = st.msTraceAny;
-        out->msClassify = st.msClassify; out->msPartition = st.msPartition; out->msShade = st.msShade;
-        out->msAccumulate = st.msAccumulate;
-        out->launchesTraceClosest = st.launchesTraceClosest; out->launchesTraceAny = st.launchesTraceAny;
-        out->closestNodeFetches = st.closestNodeFetches; out->closestTriFetches = st.closestTriFetches;
-        out->anyNodeFetches = st.anyNodeFetches; out->anyTriFetches = st.anyTriFetches;
     });
 }
+int wr_c_server_fetch(void* sv, uint64_t id, float* out) {
+    return guard([&] { SV(sv)->fetch(id, reinterpret_cast<wr::Vec3*>(out)); });
+}
+int wr_c_server_stats(void* sv, wr_c_stats* out) {
+    return guard([&] { fill_stats(SV(sv)->getStats(), out); });
+}
+uint32_t wr_c_server_pending(void* sv) { return SV(sv)->numPending(); }
+uint32_t wr_c_server_in_flight(void* sv) { return SV(sv)->numInFlight(); }
+double wr_c_server_clock(void) { return wr::RenderServer::clock(); }
 
 }  // extern "C"

@@ -16,55 +16,20 @@
 
 #include "../../include/wr/wr.h"
 #include "../../include/wrk.h"
+#include "engine.h"
 #include "image_io.h"
 
 namespace wr {
+
+using detail::DevBuf;
+using detail::hip_check;
+using detail::wrk_check;
 
 namespace {
 
 int g_logLevel = 4;
 bool g_contextReady = false;
 int g_deviceId = 0;
-
-void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
-}
-void wrk_check(int rc, const char* what) {
-    if (rc != WRK_OK) throw std::runtime_error(std::string(what) + ": " + wrk_last_error());
-}
-
-// Owning device allocation.
-struct DevBuf {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
-    DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) { release(); ptr = o.ptr; bytes = o.bytes; o.ptr = nullptr; o.bytes = 0; }
-        return *this;
-    }
-    ~DevBuf() { release(); }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-    }
-    void alloc(size_t n) {
-        if (n <= bytes && ptr) return;
-        release();
-        if (n == 0) return;
-        hip_check(hipMalloc(&ptr, n), "hipMalloc");
-        bytes = n;
-    }
-    void upload(const void* src, size_t n) {
-        alloc(n);
-        if (n) hip_check(hipMemcpy(ptr, src, n, hipMemcpyHostToDevice), "hipMemcpy H2D");
-    }
-    uint64_t addr() const { return (uint64_t)(uintptr_t)ptr; }
-    template <typename T> T* as() const { return static_cast<T*>(ptr); }
-};
 
 void ensure_context() {
     if (!g_contextReady) throw std::runtime_error("wr::Context must be created before scenes and renderers");
@@ -383,60 +348,10 @@ double Scene::getBuildSeconds() const { return m_impl->buildSeconds; }
 struct RendererImpl {
     WavefrontConfig wf;
     RenderStats stats;
-    hipStream_t stream = nullptr;
-    uint32_t capacity = 0;
-    DevBuf org, dir, thr, rad, st, contrib, savedOrg, savedDir, hit, occl, result;
-    DevBuf qA, qB, cat, scratch, offs;
-    DevBuf accum, image;
-    uint32_t* hostCounts = nullptr;
-    wrk_paths paths{};
-    DevBuf fetchCounters;  // 4 x uint64: closest nodes/tris, any nodes/tris
-    hipEvent_t ev[16] = {};
+    std::unique_ptr<detail::Engine> eng;
 
-    RendererImpl() {
-        ensure_context();
-        hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
-        hip_check(hipHostMalloc((void**)&hostCounts, 64, hipHostMallocDefault), "hipHostMalloc");
-        for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
-        fetchCounters.alloc(64);
-    }
-    ~RendererImpl() {
-        for (auto& e : ev) if (e) (void)hipEventDestroy(e);
-        if (hostCounts) (void)hipHostFree(hostCounts);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
+    RendererImpl() { ensure_context(); }
 
-    void reserve(uint32_t cap) {
-        if (cap <= capacity) return;
-        size_t c = cap;
-        org.alloc(c * 16); dir.alloc(c * 16); thr.alloc(c * 16); rad.alloc(c * 16); st.alloc(c * 16);
-        contrib.alloc(c * 16); savedOrg.alloc(c * 16); savedDir.alloc(c * 16); hit.alloc(c * 8);
-        occl.alloc(c); result.alloc(c * 16);
-        qA.alloc(c * 4); qB.alloc(c * 4); cat.alloc(c);
-        scratch.alloc(wrk_partition_scratch_bytes(cap, 8));
-        offs.alloc(64);
-        capacity = cap;
-        paths.org = org.addr(); paths.dir = dir.addr(); paths.thr = thr.addr(); paths.rad = rad.addr();
-        paths.state = st.addr(); paths.contrib = contrib.addr(); paths.savedOrg = savedOrg.addr();
-        paths.savedDir = savedDir.addr(); paths.hit = hit.addr(); paths.occluded = occl.addr();
-        paths.result = result.addr(); paths.capacity = cap;
-    }
-
-    static wrk_camera camera_of(const Camera& c) {
-        // renderer.cpp:144-156
-        Vec3 f = c.getForward(), r = c.getRight(), u = c.getUp();
-        wrk_camera k;
-        std::memset(&k, 0, sizeof(k));
-        k.position[0] = c.position.x; k.position[1] = c.position.y; k.position[2] = c.position.z;
-        k.forward[0] = f.x; k.forward[1] = f.y; k.forward[2] = f.z;
-        k.right[0] = r.x; k.right[1] = r.y; k.right[2] = r.z;
-        k.up[0] = u.x; k.up[1] = u.y; k.up[2] = u.z;
-        k.tanHalfFovY = std::tan(c.fovY * 0.5f);
-        k.aspect = c.aspect;
-        k.focalDistance = c.focalDistance;
-        k.lensRadius = c.lensRadius;
-        return k;
-    }
     wrk_render_params params_of(const RenderParams& p) const {
         wrk_render_params k;
         std::memset(&k, 0, sizeof(k));
@@ -444,151 +359,54 @@ struct RendererImpl {
         k.rrStartDepth = p.russianRouletteDepth; k.fireflyClamp = p.fireflyClamp; k.iterationCap = wf.iterationCap;
         return k;
     }
-
-    // One batch of paths [first, first+n) through the wavefront loop
-    // (renderer.cpp:190-284 with device-resident, ordered queues).
-    float elapsed(int a, int b) {
-        float ms = 0.0f;
-        hip_check(hipEventElapsedTime(&ms, ev[a], ev[b]), "hipEventElapsedTime");
-        return ms;
+    detail::Engine& engine(uint64_t paths) {
+        detail::EngineConfig c;
+        c.capacity = (uint32_t)std::max<uint64_t>(1024, std::min<uint64_t>(paths, std::max(wf.maxPathsInFlight, 1024u)));
+        c.maxChunkPaths = std::max(wf.maxPathsInFlight, 1024u);
+        c.timeKernels = wf.timeKernels;
+        c.countTraversal = wf.countTraversal;
+        if (!eng) eng = std::make_unique<detail::Engine>(c);
+        c.capacity = std::max(c.capacity, eng->config().capacity);
+        eng->setConfig(c);
+        eng->ensureCapacity(c.capacity);
+        eng->stats() = RenderStats{};
+        return *eng;
     }
-    void mark(int i) {
-        if (wf.timeKernels) hip_check(hipEventRecord(ev[i], stream), "hipEventRecord");
-    }
-
-    // One batch of paths [first, first+n) through the wavefront loop
-    // (renderer.cpp:190-284 with device-resident, ordered queues). One
-    // host synchronisation per iteration reads the next queue sizes.
-    void run_batch(const Scene* scene, const wrk_camera& cam, const wrk_render_params& prm, uint64_t first, uint32_t n) {
-        const wrk_scene_view& view = scene->getDeviceView();
-        const wrk_bvh& bvh = scene->getBVH();
-        uint32_t* qa = qA.as<uint32_t>();
-        uint32_t* qb = qB.as<uint32_t>();
-        uint32_t* off = offs.as<uint32_t>();
-        unsigned long long* fc = fetchCounters.as<unsigned long long>();
-        mark(0);
-        wrk_check(wrk_generate(&cam, &prm, &paths, first, n, qa, stream), "wrk_generate");
-        mark(1);
-        if (wf.timeKernels) {
-            hip_check(hipEventSynchronize(ev[1]), "hipEventSynchronize");
-            stats.msGenerate += elapsed(0, 1);
-        }
-        uint32_t nC = n, nS = 0;
-        const uint32_t cap = prm.iterationCap;
-        const uint32_t safety = 4 * prm.maxBounces + 16;
-        for (uint32_t it = 0;; ++it) {
-            if (cap && it >= cap) break;
-            if (it >= safety) throw std::runtime_error("wavefront loop did not drain");
-            if (nC + nS == 0) break;
-            uint32_t active = nC + nS;
-            mark(2);
-            if (nC) {
-                if (wf.countTraversal)
-                    wrk_check(wrk_trace_closest_counted(&bvh, org.ptr, dir.ptr, qa, nC, nullptr, hit.as<uint32_t>(), fc,
-                                                        stream), "wrk_trace_closest_counted");
-                else
-                    wrk_check(wrk_trace_closest(&bvh, org.ptr, dir.ptr, qa, nC, nullptr, hit.as<uint32_t>(), stream),
-                              "wrk_trace_closest");
-            }
-            mark(3);
-            if (nS) {
-                if (wf.countTraversal)
-                    wrk_check(wrk_trace_any_counted(&bvh, org.ptr, dir.ptr, qa + nC, nS, nullptr, occl.as<uint8_t>(),
-                                                    fc + 2, stream), "wrk_trace_any_counted");
-                else
-                    wrk_check(wrk_trace_any(&bvh, org.ptr, dir.ptr, qa + nC, nS, nullptr, occl.as<uint8_t>(), stream),
-                              "wrk_trace_any");
-            }
-            mark(4);
-            stats.closestRays += nC;
-            stats.shadowRays += nS;
-            if (nC) stats.launchesTraceClosest++;
-            if (nS) stats.launchesTraceAny++;
-            wrk_check(wrk_classify(&view, &prm, &paths, qa, nC, nS, cat.as<uint8_t>(), stream), "wrk_classify");
-            mark(5);
-            wrk_check(wrk_partition(cat.as<uint8_t>(), qa, active, 5, qb, off, scratch.ptr, scratch.bytes, stream),
-                      "wrk_partition");
-            mark(6);
-            uint32_t last = (cap && it + 1 == cap) ? 1u : 0u;
-            wrk_check(wrk_shade(&view, &prm, &paths, qb, off, active, last, cat.as<uint8_t>(), stream), "wrk_shade");
-            mark(7);
-            wrk_check(wrk_partition(cat.as<uint8_t>(), qb, active, 2, qa, off + 8, scratch.ptr, scratch.bytes, stream),
-                      "wrk_partition");
-            mark(8);
-            hip_check(hipMemcpyAsync(hostCounts, off + 8, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
-                      "hipMemcpyAsync");
-            hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-            if (wf.timeKernels) {
-                stats.msTraceClosest += elapsed(2, 3);
-                stats.msTraceAny += elapsed(3, 4);
-                stats.msClassify += elapsed(4, 5);
-                stats.msPartition += elapsed(5, 6) + elapsed(7, 8);
-                stats.msShade += elapsed(6, 7);
-            }
-            nC = hostCounts[1] - hostCounts[0];
-            nS = hostCounts[2] - hostCounts[1];
-            stats.iterations++;
-        }
-        stats.batches++;
-    }
-
-    void render(Scene* scene, const Camera& camera, const RenderParams& rp, float* outHost, bool accumulateOnly) {
+    // renderer.cpp:132-316 — the whole image as one request, in chunks of
+    // at most maxPathsInFlight paths, accumulated per pixel in sample order.
+    void render(Scene* scene, const Camera& camera, const RenderParams& rp, float* outHost) {
         if (!scene) throw std::runtime_error("render: null scene");
         if (rp.width == 0 || rp.height == 0 || rp.spp == 0) throw std::runtime_error("render: empty image");
-        scene->finalize();
-        const uint64_t npix = (uint64_t)rp.width * rp.height;
-        const uint64_t total = npix * rp.spp;
-        uint32_t cap = (uint32_t)std::min<uint64_t>(total, std::max<uint32_t>(wf.maxPathsInFlight, 1024u));
-        reserve(cap);
-        accum.alloc(npix * 3 * sizeof(float));
-        image.alloc(npix * 3 * sizeof(float));
-        hip_check(hipMemsetAsync(accum.ptr, 0, npix * 3 * sizeof(float), stream), "hipMemsetAsync");
-        wrk_camera cam = camera_of(camera);
-        wrk_render_params prm = params_of(rp);
-        stats = RenderStats{};
-        hip_check(hipMemsetAsync(fetchCounters.ptr, 0, 32, stream), "hipMemsetAsync");
-        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        const uint64_t total = (uint64_t)rp.width * rp.height * rp.spp;
+        detail::Engine& e = engine(total);
         auto t0 = std::chrono::steady_clock::now();
-        for (uint64_t first = 0; first < total; first += cap) {
-            uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - first);
-            run_batch(scene, cam, prm, first, n);
-            mark(9);
-            wrk_check(wrk_accumulate(&paths, first, n, (uint32_t)npix, accum.as<float>(), stream), "wrk_accumulate");
-            mark(10);
-            if (wf.timeKernels) {
-                hip_check(hipEventSynchronize(ev[10]), "hipEventSynchronize");
-                stats.msAccumulate += elapsed(9, 10);
-            }
+        uint64_t id = e.submit(scene, detail::camera_record(camera), params_of(rp), rp.spp, 0, total,
+                               detail::Engine::Mode::Accumulate, nullptr);
+        while (e.step()) {
         }
-        wrk_check(wrk_average(accum.as<float>(), (uint32_t)npix, rp.spp, image.as<float>(), stream), "wrk_average");
-        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        std::vector<float> tmp;
+        float* dst = outHost;
+        if (!dst) {
+            tmp.resize((size_t)rp.width * rp.height * 3);
+            dst = tmp.data();
+        }
+        e.fetch(id, dst);
+        stats = e.stats();
         stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         stats.paths = total;
-        if (wf.countTraversal) {
-            unsigned long long c[4];
-            hip_check(hipMemcpy(c, fetchCounters.ptr, sizeof(c), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-            stats.closestNodeFetches = c[0]; stats.closestTriFetches = c[1];
-            stats.anyNodeFetches = c[2]; stats.anyTriFetches = c[3];
-        }
-        if (outHost && !accumulateOnly)
-            hip_check(hipMemcpy(outHost, image.ptr, npix * 3 * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        const uint64_t chunk = std::min<uint64_t>(e.config().capacity, e.config().maxChunkPaths);
+        stats.batches = (uint32_t)((total + chunk - 1) / chunk);
     }
-
     void render_paths(Scene* scene, const Camera& camera, const RenderParams& rp, uint64_t firstPath, uint32_t count,
                       float* out) {
         if (!scene) throw std::runtime_error("renderPaths: null scene");
-        scene->finalize();
-        uint32_t cap = std::max<uint32_t>(std::min<uint32_t>(count, wf.maxPathsInFlight), 1u);
-        reserve(cap);
-        wrk_camera cam = camera_of(camera);
-        wrk_render_params prm = params_of(rp);
-        stats = RenderStats{};
-        for (uint64_t done = 0; done < count; done += cap) {
-            uint32_t n = (uint32_t)std::min<uint64_t>(cap, count - done);
-            run_batch(scene, cam, prm, firstPath + done, n);
-            hip_check(hipMemcpyAsync(out + done * 4, result.ptr, (size_t)n * 16, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
-            hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        detail::Engine& e = engine(count);
+        uint64_t id = e.submit(scene, detail::camera_record(camera), params_of(rp), rp.spp, firstPath, count,
+                               detail::Engine::Mode::Paths, out);
+        while (e.step()) {
         }
+        e.release(id);
+        stats = e.stats();
     }
 };
 
@@ -607,7 +425,7 @@ void Renderer::render(Scene* scene, const Camera& camera, Vec3* outputBuffer, co
     if (g_logLevel >= 4)
         std::cout << "[Renderer] Starting render: " << params.width << "x" << params.height << " @ " << params.spp
                   << " spp" << std::endl;
-    m_impl->render(scene, camera, params, reinterpret_cast<float*>(outputBuffer), false);
+    m_impl->render(scene, camera, params, reinterpret_cast<float*>(outputBuffer));
     if (params.denoiser.enabled && g_logLevel >= 2)
         std::cout << "[Renderer] denoiser requested: not available in this build, returning the averaged image" << std::endl;
     if (g_logLevel >= 4) std::cout << "[Renderer] Render complete" << std::endl;
@@ -633,5 +451,54 @@ void Renderer::renderPaths(Scene* scene, const Camera& camera, const RenderParam
     m_impl->render_paths(scene, camera, params, firstPath, count, out);
 }
 const RenderStats& Renderer::getStats() const { return m_impl->stats; }
+
+// ----------------------------------------------------------- RenderServer
+struct ServerImpl {
+    ServerConfig cfg;
+    std::unique_ptr<detail::Engine> eng;
+    explicit ServerImpl(const ServerConfig& c) : cfg(c) {
+        ensure_context();
+        detail::EngineConfig e;
+        e.capacity = std::max(c.maxPathsInFlight, 1024u);
+        e.maxChunkPaths = c.maxChunkPaths;
+        e.memoryBudgetBytes = c.memoryBudgetBytes;
+        e.timeKernels = c.timeKernels;
+        eng = std::make_unique<detail::Engine>(e);
+        eng->ensureCapacity(e.capacity);
+    }
+};
+
+RenderServer::RenderServer(const ServerConfig& config) : m_impl(new ServerImpl(config)) {}
+RenderServer::~RenderServer() { delete m_impl; }
+
+uint64_t RenderServer::submit(Scene* scene, const Camera& camera, const RenderParams& params) {
+    if (params.width == 0 || params.height == 0 || params.spp == 0) throw std::runtime_error("render: empty image");
+    wrk_render_params k;
+    std::memset(&k, 0, sizeof(k));
+    k.width = params.width; k.height = params.height; k.useNEE = params.useNEE ? 1u : 0u;
+    k.maxBounces = params.maxBounces; k.rrStartDepth = params.russianRouletteDepth;
+    k.fireflyClamp = params.fireflyClamp; k.iterationCap = m_impl->cfg.iterationCap;
+    uint64_t total = (uint64_t)params.width * params.height * params.spp;
+    return m_impl->eng->submit(scene, detail::camera_record(camera), k, params.spp, 0, total,
+                               detail::Engine::Mode::Accumulate, nullptr);
+}
+bool RenderServer::step() { return m_impl->eng->step(); }
+bool RenderServer::isDone(uint64_t id) const { return m_impl->eng->isDone(id); }
+RequestStatus RenderServer::status(uint64_t id) const {
+    const auto& r = m_impl->eng->request(id);
+    RequestStatus s;
+    s.done = r.done;
+    s.admitted = r.admitted;
+    s.submitTime = r.tSubmit;
+    s.admitTime = r.tAdmit;
+    s.doneTime = r.tDone;
+    s.paths = r.endPath - r.firstPath;
+    return s;
+}
+void RenderServer::fetch(uint64_t id, Vec3* out) { m_impl->eng->fetch(id, reinterpret_cast<float*>(out)); }
+uint32_t RenderServer::numPending() const { return (uint32_t)m_impl->eng->numPending(); }
+uint32_t RenderServer::numInFlight() const { return (uint32_t)m_impl->eng->numActiveSlots(); }
+const RenderStats& RenderServer::getStats() const { return m_impl->eng->stats(); }
+double RenderServer::clock() { return detail::Engine::now(); }
 
 }  // namespace wr

@@ -338,5 +338,11 @@ constexpr uint32_t FLAG_DEPTH_MASK = 0xFFFFu;
 constexpr uint32_t FLAG_FIRST_HIT = 1u << 16;
 constexpr uint32_t FLAG_PREV_DELTA = 1u << 17;
 constexpr uint32_t FLAG_NEE_DONE = 1u << 18;
+// host-loop iterations the path took part in (renderer.cpp:213 cap), 5 bits
+constexpr uint32_t FLAG_ITER_SHIFT = 19;
+constexpr uint32_t FLAG_ITER_MASK = 31u << FLAG_ITER_SHIFT;
+// request slot of the path (index into the per-slot parameter table)
+constexpr uint32_t FLAG_SLOT_SHIFT = 24;
+constexpr uint32_t kMaxSlots = 256;
 
 }  // namespace wrk

@@ -53,3 +53,25 @@ def test_partition_is_stable(gpu, K, n):
     assert np.array_equal(o, expect_offs)
     for k in range(K):
         assert np.array_equal(got[o[k]:o[k + 1]], items[cat == k])
+
+
+@pytest.mark.parametrize("n", [0, 1, 1000, 3000017])
+def test_slot_count_matches_histogram(gpu, n):
+    rng = np.random.default_rng(1234 + n)
+    cap = max(2 * n, 64)
+    nr = 37
+    firsts = np.concatenate([[0], np.sort(rng.choice(np.arange(1, cap), nr - 1, replace=False))]).astype(np.uint32)
+    slots = rng.permutation(256)[:nr].astype(np.uint32)
+    queue = rng.permutation(cap)[:n].astype(np.uint32)
+    queue[: n // 2] = np.sort(queue[: n // 2])  # long runs of one range, then scattered entries
+    m = max(n - 5, 0)  # device-side count shorter than the launch size
+    q = G.dev(queue if n else np.zeros(1, np.uint32))
+    ranges = G.dev(np.concatenate([firsts, slots]))
+    cnt = G.dev(np.array([m], np.uint32))
+    counts = G.dev(np.full(256, 7, np.uint32))
+    G.check(wr.lib().wrk_slot_count(G.vp(q), n, G.vp(cnt), G.vp(ranges), nr, G.vp(counts), G.stream()))
+    torch.cuda.synchronize()
+    got = counts.cpu().numpy().view(np.uint32)
+    r = np.searchsorted(firsts, queue[:m], side="right") - 1
+    want = np.bincount(slots[r], minlength=256).astype(np.uint32)
+    assert np.array_equal(got, want)
