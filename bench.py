@@ -17,6 +17,15 @@ oracle, OpenMP on the host cores, on a 1-spp sample of the same frame).
 --workload large measures the incoherent 10.24M-triangle scene instead
 (2,000 displaced icospheres, 3840x2160, 16 spp, depth 8); the default
 workload is the one the headline metric is quoted on.
+
+--workload serving measures the continuous-batching server: the Cornell,
+material and large scenes resident on the device, a seeded stream of
+requests (random camera; 256x256 / 512x512 / 1024x768 / 1920x1080;
+1 / 4 / 16 / 64 spp; depth 8). It reports aggregate Mpaths/s with every
+request submitted at once against the same requests rendered one after
+another, and p50/p99 latency (completion - arrival) at a fixed Poisson
+arrival rate for the server and for serial FIFO rendering, and checks that
+each served image equals its stand-alone render bit for bit.
 """
 from __future__ import annotations
 
@@ -48,11 +57,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["material", "large"], default="material")
+    ap.add_argument("--workload", choices=["material", "large", "serving"], default="material")
     ap.add_argument("--spp", type=int, default=None)
     ap.add_argument("--width", type=int, default=None)
     ap.add_argument("--height", type=int, default=None)
-    ap.add_argument("--max-paths", type=int, default=1 << 28, help="paths in flight (256M = ~46 GB of HBM)")
+    ap.add_argument("--max-paths", type=int, default=None,
+                    help="paths in flight (default 256M = ~46 GB of HBM; 128M for serving)")
+    ap.add_argument("--max-chunk", type=int, default=None, help="paths per admitted chunk (default: whole pool; "
+                    "8M for serving)")
+    ap.add_argument("--requests", type=int, default=256, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrival rate, requests/s")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target wall time of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -71,6 +85,11 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     wr.Context(wr.ContextConfig(deviceId=0, logLevel=0))
+    if args.workload == "serving":
+        serving(args)
+        return
+    if args.max_paths is None:
+        args.max_paths = 1 << 28
     if args.workload == "material":
         W, H, spp = args.width or 1920, args.height or 1080, args.spp or 256
         desc = scenes.material_scene(W, H, spp)
@@ -104,7 +123,8 @@ def main():
     log(f"[bench] closest: {nodes_per_ray:.2f} nodes, {tris_per_ray:.2f} tris per ray; "
         f"any: {any_nodes:.2f} nodes, {any_tris:.2f} tris per ray")
 
-    r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=args.max_paths, timeKernels=True))
+    r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=args.max_paths,
+                                            maxChunkPaths=args.max_chunk or 0, timeKernels=True))
     for i in range(args.warmup):
         r.render(scene, cam, prm)
         log(f"[bench] warmup {i + 1}/{args.warmup}: {r.getStats()['seconds']:.3f} s")
@@ -176,6 +196,7 @@ def main():
             "workload": workload,
             "width": W, "height": H, "spp": spp, "max_depth": desc.maxBounces, "triangles": desc.numTris,
             "iteration_cap": "none (paths run to completion)", "paths_in_flight": args.max_paths,
+            "chunk_paths": args.max_chunk or args.max_paths,
         },
         "mrays_per_s": round(mrays, 3),
         "bvh_build_ms": round(bvh_build_ms, 2),
@@ -200,6 +221,134 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def serving(args):
+    """Continuous batching against serial per-request rendering."""
+    import hashlib
+
+    import numpy as np
+
+    import offlinerenderer_amd as wr
+    from offlinerenderer_amd import scenes
+
+    pool = args.max_paths or (1 << 27)
+    chunk = args.max_chunk or (1 << 23)
+    descs = {"cornell": scenes.cornell(), "material": scenes.material_scene(), "large": scenes.large_scene()}
+    resident = {}
+    for name, d in descs.items():
+        s = wr.Scene()
+        d.build(s)
+        s.finalize()
+        resident[name] = s
+        log(f"[serve] scene {name}: {d.numTris} tris, BVH build {s.getBuildSeconds() * 1e3:.1f} ms")
+    reqs = scenes.serving_requests(args.requests, seed=7)
+    jobs = [(resident[q["scene"]], wr.Camera(**q["camera"]),
+             wr.RenderParams(width=q["width"], height=q["height"], spp=q["spp"], maxBounces=q["maxBounces"],
+                             russianRouletteDepth=q["rrDepth"])) for q in reqs]
+    paths = np.array([p.width * p.height * p.spp for _, _, p in jobs], dtype=np.float64)
+    total_paths = float(paths.sum())
+    rng = np.random.default_rng(11)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=len(jobs)))
+    arrivals -= arrivals[0]
+
+    def digest(img):
+        return hashlib.sha1(np.ascontiguousarray(img).tobytes()).hexdigest()
+
+    # serial: one request after another on the same pool size
+    r = wr.Renderer()
+    r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=pool))
+    r.render(*jobs[0])  # warmup
+    service, ref = np.zeros(len(jobs)), []
+    for i, job in enumerate(jobs):
+        t0 = time.perf_counter()
+        img = r.render(*job)
+        service[i] = time.perf_counter() - t0
+        ref.append(digest(img))
+    del r
+    serial_mpaths = total_paths / service.sum() / 1e6
+    log(f"[serve] serial: {service.sum():.2f} s busy, {serial_mpaths:.1f} Mpaths/s")
+    # serial FIFO latency under the same arrivals (one request at a time)
+    finish, t = np.zeros(len(jobs)), 0.0
+    for i in range(len(jobs)):
+        t = max(t, arrivals[i]) + service[i]
+        finish[i] = t
+    serial_lat = finish - arrivals
+
+    def run_server(arrive):
+        srv = wr.RenderServer(wr.ServerConfig(maxPathsInFlight=pool, maxChunkPaths=chunk))
+        ids, done_t, equal = [None] * len(jobs), np.zeros(len(jobs)), 0
+        open_ids = {}
+        t0 = srv.clock()
+        nxt, steps = 0, 0
+        while nxt < len(jobs) or open_ids:
+            now = srv.clock() - t0
+            while nxt < len(jobs) and arrive[nxt] <= now:
+                ids[nxt] = srv.submit(*jobs[nxt])
+                open_ids[ids[nxt]] = nxt
+                nxt += 1
+            if not open_ids:
+                time.sleep(max(0.0, arrive[nxt] - (srv.clock() - t0)))
+                continue
+            srv.step()
+            steps += 1
+            for rid, i in list(open_ids.items()):
+                st = srv.status(rid)
+                if st["done"]:
+                    done_t[i] = st["doneTime"] - t0
+                    equal += digest(srv.fetch(rid)) == ref[i]
+                    del open_ids[rid]
+        stats = srv.getStats()
+        return done_t, equal, steps, stats
+
+    # saturation: every request at t = 0
+    done_sat, eq_sat, steps_sat, st_sat = run_server(np.zeros(len(jobs)))
+    makespan = float(done_sat.max())
+    server_mpaths = total_paths / makespan / 1e6
+    log(f"[serve] server, all at once: {makespan:.2f} s, {server_mpaths:.1f} Mpaths/s, {steps_sat} steps")
+    # open loop at the fixed arrival rate
+    done_ol, eq_ol, steps_ol, _ = run_server(arrivals)
+    server_lat = done_ol - arrivals
+    log(f"[serve] server at {args.rate} req/s: p50 {np.percentile(server_lat, 50) * 1e3:.0f} ms, "
+        f"p99 {np.percentile(server_lat, 99) * 1e3:.0f} ms")
+
+    def pct(a):
+        return {"p50_ms": round(float(np.percentile(a, 50)) * 1e3, 1),
+                "p99_ms": round(float(np.percentile(a, 99)) * 1e3, 1),
+                "mean_ms": round(float(a.mean()) * 1e3, 1)}
+
+    line = {
+        "metric": "Mpaths/s @1080p; time-to-relMSE<=1e-2 learned vs plain; p50/p99 req latency",
+        "value": round(server_mpaths, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": round(makespan * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded request stream over the Cornell, material and large scenes)",
+        "config": {
+            "workload": f"serving_{len(jobs)}req_poisson{args.rate:g}",
+            "requests": len(jobs), "total_paths": int(total_paths), "resolutions": list(scenes.SERVING_RESOLUTIONS),
+            "spp": list(scenes.SERVING_SPP), "max_depth": 8, "arrival_rate_per_s": args.rate,
+            "paths_in_flight": pool, "chunk_paths": chunk,
+        },
+        "serial_mpaths_per_s": round(serial_mpaths, 3),
+        "speedup_vs_serial": round(server_mpaths / serial_mpaths, 3),
+        "latency_server": pct(server_lat),
+        "latency_serial_fifo": pct(serial_lat),
+        "offered_load": round(float(paths.sum() / arrivals[-1]) / (serial_mpaths * 1e6), 3) if arrivals[-1] > 0 else None,
+        "server_steps": {"saturated": steps_sat, "open_loop": steps_ol},
+        "mixed_equals_alone": {"saturated": f"{eq_sat}/{len(jobs)}", "open_loop": f"{eq_ol}/{len(jobs)}"},
+        "mrays_per_s_saturated": round((st_sat["closestRays"] + st_sat["shadowRays"]) / makespan / 1e6, 3),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    assert eq_sat == len(jobs) and eq_ol == len(jobs), "a served image differs from its stand-alone render"
 
 
 def cpu_baseline(desc, W, H, target_s):

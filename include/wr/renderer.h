@@ -41,6 +41,9 @@ struct RenderParams {
 struct WavefrontConfig {
     uint32_t iterationCap = 8;
     uint32_t maxPathsInFlight = 1u << 26;  // ~11 GB of path state
+    // paths per admitted chunk (0 = maxPathsInFlight); smaller chunks refill
+    // the pool while the deep paths of earlier chunks finish
+    uint32_t maxChunkPaths = 0;
     bool timeKernels = false;     // per-kernel device time with HIP events
     bool countTraversal = false;  // node/triangle fetch counters (slower)
 };

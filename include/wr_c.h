@@ -64,7 +64,8 @@ double wr_c_scene_build_seconds(void* scene);
 void* wr_c_renderer_create(void);                                    /* Context::createRenderer */
 void wr_c_renderer_destroy(void* renderer);
 /* flags: bit0 time kernels with HIP events, bit1 count traversal fetches */
-int wr_c_renderer_set_wavefront(void* renderer, uint32_t iterationCap, uint32_t maxPathsInFlight, uint32_t flags);
+int wr_c_renderer_set_wavefront(void* renderer, uint32_t iterationCap, uint32_t maxPathsInFlight, uint32_t maxChunkPaths,
+                                uint32_t flags);
 /* Renderer::render; out = width*height*3 floats (averaged linear RGB) */
 int wr_c_render(void* renderer, void* scene, const wr_c_camera* cam, const wr_c_render_params* params, float* out);
 /* Renderer::renderPaths; out = count*4 floats */

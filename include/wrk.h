@@ -123,8 +123,9 @@ typedef struct wrk_render_params {
     uint32_t _pad;
 } wrk_render_params;
 
-/* Structure-of-arrays path pool for one wavefront batch (capacity paths).
- * Replaces the 116-byte RayState + 56-byte HitInfo AoS of gpu_types.h:48-86. */
+/* Structure-of-arrays path pool (capacity paths). Replaces the 116-byte
+ * RayState + 56-byte HitInfo AoS of gpu_types.h:48-86. Paths of one chunk
+ * occupy a contiguous range, so queue-ordered accesses stay coalesced. */
 typedef struct wrk_paths {
     uint64_t org;      /* float4: origin.xyz, tMin */
     uint64_t dir;      /* float4: direction.xyz, tMax */
@@ -132,8 +133,8 @@ typedef struct wrk_paths {
     uint64_t rad;      /* float4: radiance.xyz, - */
     uint64_t state;    /* uint4: seed, rngDimension, flags, pixelIndex */
     uint64_t contrib;  /* float4: pending NEE contribution.xyz */
-    uint64_t savedOrg; /* float4: origin of the ray that produced the hit */
-    uint64_t savedDir; /* float4: savedDirection.xyz */
+    uint64_t savedOrg; /* float4: shadow ray origin.xyz, tMin */
+    uint64_t savedDir; /* float4: shadow ray direction.xyz, tMax */
     uint64_t hit;      /* uint2: t (float bits), primitive id (0xFFFFFFFF = miss) */
     uint64_t occluded; /* uint8 */
     uint64_t result;   /* float4: value added to the accumulation buffer */
@@ -190,7 +191,7 @@ uint64_t wrk_compact_scratch_bytes(uint32_t n);
  * atomic-per-survivor order is nondeterministic; this one is ordered. */
 int wrk_compact(const uint8_t* keep, const uint32_t* in, uint32_t n, uint32_t* out,
                 uint32_t* countOut, void* scratch, uint64_t scratchBytes, void* stream);
-/* Stable K-way partition (K <= 8): items with category k, k < numCategories,
+/* Stable K-way partition (K <= 16): items with category k, k < numCategories,
  * land in out[offsets[k] ...] in input order; categories >= numCategories
  * are dropped. offsetsOut (device) receives K+1 exclusive offsets. */
 int wrk_partition(const uint8_t* category, const uint32_t* in, uint32_t n, uint32_t numCategories,
@@ -212,31 +213,36 @@ int wrk_partition(const uint8_t* category, const uint32_t* in, uint32_t n, uint3
 int wrk_generate(const wrk_camera* cam, const wrk_render_params* params, const wrk_paths* paths, uint64_t first,
                  uint32_t n, uint32_t poolBase, uint32_t slot, uint32_t* queue, uint8_t* category,
                  uint32_t categoryValue, void* stream);
-/* Resolves the trace results of the active queue [closest | shadow] of one
- * scene and assigns each path a shade category 0..4 (material kinds of
- * gpu_types.h:11-17) or 5 = finished (trace.cu:27-51 shadow resolve,
- * trace.cu:143-148 miss flush). paramsTable is a device array indexed by the
- * path's slot. */
+/* Resolves the trace results of the active queue of one scene,
+ * [trace numTrace | trace + shadow numTraceShadow | shadow numShadow]
+ * (closest-hit results for the first two parts, any-hit results of the
+ * shadow ray held in savedOrg/savedDir for the last two), and assigns each
+ * path a shade category 0..4 (material kinds of gpu_types.h:11-17) or
+ * 5 = finished (trace.cu:27-51 shadow resolve, trace.cu:143-148 miss flush).
+ * paramsTable is a device array indexed by the path's slot. */
 int wrk_classify(const wrk_scene_view* scene, const wrk_render_params* paramsTable, const wrk_paths* paths,
-                 const uint32_t* queue, uint32_t numClosest, uint32_t numShadow, uint8_t* category,
-                 void* stream);
+                 const uint32_t* queue, uint32_t numTrace, uint32_t numTraceShadow, uint32_t numShadow,
+                 uint8_t* category, void* stream);
 /* Runs the material shaders over the material-sorted queue of one scene
- * (offsets as from wrk_partition with 5 categories): shade.cu:24-85 and the
- * materials/ headers. nextCategory[i] = categoryBase + 0 (trace next) or
- * categoryBase + 1 (shadow test next) or 0xFF (finished) for queue
- * positions i < maxItems. Paths reaching their slot's iterationCap are
+ * (offsets as from wrk_partition with 6 categories, the sixth holding the
+ * paths classify finished): shade.cu:24-85 and the materials/ headers.
+ * nextCategory[i] = categoryBase + 0 (trace the continuation), + 1 (trace
+ * the continuation and a shadow ray), + 2 (trace a shadow ray only) or
+ * finishedCategory (path ended) for queue positions i < maxItems.
+ * With iterationCap = 0 a Lambertian hit emits its shadow ray and its
+ * continuation together; with a cap the continuation waits for the shadow
+ * result as in libWR's stage machine, and paths reaching the cap are
  * dropped unflushed (renderer.cpp:213). */
 int wrk_shade(const wrk_scene_view* scene, const wrk_render_params* paramsTable, const wrk_paths* paths,
               const uint32_t* queue, const uint32_t* offsets, uint32_t maxItems, uint32_t categoryBase,
-              uint8_t* nextCategory, void* stream);
-/* Live paths per slot (no counterpart in libWR, whose host loop renders one
- * batch at a time): counts[ranges[numRanges + r]] = number of queue entries
- * inside pool range r, for the first n = min(*countDev, maxItems) entries
- * (n = maxItems when countDev is NULL). ranges[0 .. numRanges) are the
- * ascending first pool positions of the disjoint ranges that hold every
- * live path; counts[WRK_MAX_SLOTS] is cleared first. */
-int wrk_slot_count(const uint32_t* queue, uint32_t maxItems, const uint32_t* countDev, const uint32_t* ranges,
-                   uint32_t numRanges, uint32_t* counts, void* stream);
+              uint32_t finishedCategory, uint8_t* nextCategory, void* stream);
+/* Finished paths per slot (no counterpart in libWR, whose host loop renders
+ * one batch at a time): live[slot] -= number of the first
+ * n = min(*countDev, maxItems) entries of `finished` (pool entries of paths
+ * that ended this iteration; n = maxItems when countDev is NULL) whose
+ * state carries that slot. */
+int wrk_slot_finish(const uint32_t* finished, uint32_t maxItems, const uint32_t* countDev, const wrk_paths* paths,
+                    uint32_t* live, void* stream);
 /* accum[pixel] += result[poolBase + k] for the chunk's paths in sample order
  * (renderer.cpp:190 serial samples; per-pixel accumulation order is fixed). */
 int wrk_accumulate(const wrk_paths* paths, uint32_t poolBase, uint64_t first, uint32_t n, uint32_t npix,

@@ -68,6 +68,13 @@ class wr_c_stats(C.Structure):
                 ("closestTriFetches", C.c_uint64), ("anyNodeFetches", C.c_uint64), ("anyTriFetches", C.c_uint64)]
 
 
+class wrk_paths(C.Structure):
+    """include/wrk.h wrk_paths: device addresses of the path pool arrays."""
+    _fields_ = [(name, C.c_uint64) for name in ("org", "dir", "thr", "rad", "state", "contrib", "savedOrg",
+                                                 "savedDir", "hit", "occluded", "result")] + \
+               [("capacity", C.c_uint32), ("_pad", C.c_uint32)]
+
+
 class wr_c_server_config(C.Structure):
     _fields_ = [("maxPathsInFlight", C.c_uint32), ("maxChunkPaths", C.c_uint32), ("memoryBudgetBytes", C.c_uint64),
                 ("iterationCap", C.c_uint32), ("flags", C.c_uint32)]
@@ -104,7 +111,7 @@ _SIGS = {
     "wr_c_scene_build_seconds": (C.c_double, [_P]),
     "wr_c_renderer_create": (_P, []),
     "wr_c_renderer_destroy": (None, [_P]),
-    "wr_c_renderer_set_wavefront": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "wr_c_renderer_set_wavefront": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "wr_c_render": (C.c_int, [_P, _P, C.POINTER(wr_c_camera), C.POINTER(wr_c_render_params), _P]),
     "wr_c_render_paths": (C.c_int, [_P, _P, C.POINTER(wr_c_camera), C.POINTER(wr_c_render_params), C.c_uint64,
                                     C.c_uint32, _P]),
@@ -135,9 +142,9 @@ _SIGS = {
     "wrk_partition": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint64, _P]),
     "wrk_generate": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32,
                                _P]),
-    "wrk_classify": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, _P]),
-    "wrk_shade": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, _P]),
-    "wrk_slot_count": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, _P, _P]),
+    "wrk_classify": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]),
+    "wrk_shade": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]),
+    "wrk_slot_finish": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
     "wrk_accumulate": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_average": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_tonemap_aces": (C.c_int, [_P, C.c_uint32, _P, _P]),
@@ -235,6 +242,7 @@ class WavefrontConfig:
     """iterationCap 8 = libWR's host loop (renderer.cpp:213); 0 = run to completion."""
     iterationCap: int = 8
     maxPathsInFlight: int = 1 << 26
+    maxChunkPaths: int = 0
     timeKernels: bool = False
     countTraversal: bool = False
 
@@ -382,7 +390,8 @@ class Renderer:
 
     def setWavefrontConfig(self, cfg: WavefrontConfig):
         flags = (1 if cfg.timeKernels else 0) | (2 if cfg.countTraversal else 0)
-        _check(lib().wr_c_renderer_set_wavefront(self._h, int(cfg.iterationCap), int(cfg.maxPathsInFlight), flags),
+        _check(lib().wr_c_renderer_set_wavefront(self._h, int(cfg.iterationCap), int(cfg.maxPathsInFlight),
+                                                      int(cfg.maxChunkPaths), flags),
                "setWavefrontConfig")
 
     def render(self, scene: Scene, camera: Camera, params: RenderParams) -> np.ndarray:

@@ -54,6 +54,10 @@ double Engine::now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+constexpr uint32_t kMaxScenes = 4;
+// final queue: [finished | per scene: trace, trace + shadow, shadow]
+constexpr uint32_t kFinalCategories = 1 + 3 * kMaxScenes;
+
 enum TimedCat { T_CLOSEST, T_ANY, T_CLASSIFY, T_PARTITION, T_SHADE, T_GENERATE, T_ACCUM };
 
 Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
@@ -63,13 +67,12 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
               "hipHostMalloc");
     hip_check(hipHostMalloc((void**)&hostPrm_, WRK_MAX_SLOTS * sizeof(wrk_render_params), hipHostMallocDefault),
               "hipHostMalloc");
-    hip_check(hipHostMalloc((void**)&hostRanges_, 2 * WRK_MAX_SLOTS * sizeof(uint32_t), hipHostMallocDefault),
+    hip_check(hipHostMalloc((void**)&hostLiveInit_, WRK_MAX_SLOTS * sizeof(uint32_t), hipHostMallocDefault),
               "hipHostMalloc");
     std::memset(hostPrm_, 0, WRK_MAX_SLOTS * sizeof(wrk_render_params));
     offs_.alloc(64 * sizeof(uint32_t));
     prmTable_.alloc(WRK_MAX_SLOTS * sizeof(wrk_render_params));
     live_.alloc(WRK_MAX_SLOTS * sizeof(uint32_t));
-    ranges_.alloc(2 * WRK_MAX_SLOTS * sizeof(uint32_t));
     fetch_.alloc(64);
     hip_check(hipMemsetAsync(fetch_.ptr, 0, 64, stream_), "hipMemsetAsync");
     for (uint32_t s = WRK_MAX_SLOTS; s-- > 0;) freeSlots_.push_back(s);
@@ -80,7 +83,7 @@ Engine::~Engine() {
     if (hostOffs_) (void)hipHostFree(hostOffs_);
     if (hostLive_) (void)hipHostFree(hostLive_);
     if (hostPrm_) (void)hipHostFree(hostPrm_);
-    if (hostRanges_) (void)hipHostFree(hostRanges_);
+    if (hostLiveInit_) (void)hipHostFree(hostLiveInit_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -100,12 +103,13 @@ void Engine::reserve(uint32_t cap) {
     contrib_.alloc(c * 16); savedOrg_.alloc(c * 16); savedDir_.alloc(c * 16); hit_.alloc(c * 8);
     occl_.alloc(c); result_.alloc(c * 16);
     qA_.alloc(c * 4); qB_.alloc(c * 4); cat_.alloc(c);
-    scratch_.alloc(wrk_partition_scratch_bytes(cap, 8));
+    scratch_.alloc(wrk_partition_scratch_bytes(cap, kFinalCategories));
     capacity_ = cap;
     paths_.org = org_.addr(); paths_.dir = dir_.addr(); paths_.thr = thr_.addr(); paths_.rad = rad_.addr();
     paths_.state = st_.addr(); paths_.contrib = contrib_.addr(); paths_.savedOrg = savedOrg_.addr();
     paths_.savedDir = savedDir_.addr(); paths_.hit = hit_.addr(); paths_.occluded = occl_.addr();
     paths_.result = result_.addr(); paths_.capacity = cap;
+    deadCount_ = 0;
     free_.clear();
     free_[0] = cap;
 }
@@ -118,14 +122,15 @@ uint32_t Engine::sceneIndex(Scene* s) {
         bool used = false;
         for (auto& kv : requests_)
             if (kv.second->sceneIdx == i && !kv.second->done) used = true;
-        if (!used && segC_[i] == 0 && segS_[i] == 0) {
+        if (!used && segT_[i] == 0 && segTS_[i] == 0 && segS_[i] == 0) {
             scenes_[i] = s;
             return i;
         }
     }
-    if (scenes_.size() >= 4) throw std::runtime_error("at most 4 scenes can be resident in one wavefront");
+    if (scenes_.size() >= kMaxScenes) throw std::runtime_error("at most 4 scenes can be resident in one wavefront");
     scenes_.push_back(s);
-    segC_.push_back(0);
+    segT_.push_back(0);
+    segTS_.push_back(0);
     segS_.push_back(0);
     return (uint32_t)scenes_.size() - 1;
 }
@@ -181,6 +186,7 @@ uint64_t Engine::submit(Scene* scene, const wrk_camera& cam, const wrk_render_pa
     r->mode = mode;
     r->firstPath = firstPath;
     r->nextPath = firstPath;
+    r->accumNext = firstPath;
     r->endPath = firstPath + numPaths;
     r->spp = spp;
     r->npix = prm.width * prm.height;
@@ -239,35 +245,117 @@ template <typename F> void Engine::timed(int category, F&& launch) {
     marks_.push_back({category, {a, b}});
 }
 
-// Chunk completion: accumulate (or copy out) the chunk's per-path results,
-// free its pool range and slot, queue the request's next chunk or finish it.
+// Frees a chunk's pool range and slot.
+void Engine::retire(uint32_t s) {
+    Slot& sl = slots_[s];
+    freeRange(sl.poolBase, sl.count);
+    sl.active = false;
+    sl.finished = false;
+    freeSlots_.push_back(s);
+}
+
+// Chunk completion. Paths mode copies the per-path results out at once;
+// accumulate mode adds every finished chunk that continues the sample order
+// (renderer.cpp:190: samples in order) and holds the others.
 void Engine::complete(uint32_t s) {
     Slot& sl = slots_[s];
     RequestState& r = *requests_.at(sl.request);
-    if (r.mode == Mode::Accumulate) {
-        timed(T_ACCUM, [&] {
-            wrk_check(wrk_accumulate(&paths_, sl.poolBase, sl.firstPath, sl.count, r.npix, r.accum.as<float>(), stream_),
-                      "wrk_accumulate");
-        });
-    } else {
+    sl.finished = true;
+    r.chunksInFlight--;
+    r.finishedPaths += sl.count;
+    if (r.mode == Mode::Paths) {
         float* dst = r.pathsOut + (sl.firstPath - r.firstPath) * 4;
         hip_check(hipMemcpyAsync(dst, result_.as<float4>() + sl.poolBase, (size_t)sl.count * 16,
                                  hipMemcpyDeviceToHost, stream_),
                   "hipMemcpyAsync");
-    }
-    freeRange(sl.poolBase, sl.count);
-    sl.active = false;
-    freeSlots_.push_back(s);
-    rangesDirty_ = true;
-    r.inFlight = false;
-    if (r.nextPath < r.endPath) {
-        pending_.push_front(r.id);  // keeps its place ahead of later arrivals
+        retire(s);
     } else {
+        r.held[sl.firstPath] = s;
+        while (!r.held.empty() && r.held.begin()->first == r.accumNext) {
+            const uint32_t hs = r.held.begin()->second;
+            Slot& h = slots_[hs];
+            timed(T_ACCUM, [&] {
+                wrk_check(wrk_accumulate(&paths_, h.poolBase, h.firstPath, h.count, r.npix, r.accum.as<float>(),
+                                         stream_),
+                          "wrk_accumulate");
+            });
+            r.accumNext += h.count;
+            r.held.erase(r.held.begin());
+            retire(hs);
+        }
+    }
+    if (r.finishedPaths == r.endPath - r.firstPath) {
         if (r.mode == Mode::Accumulate)
             wrk_check(wrk_average(r.accum.as<float>(), r.npix, r.spp, r.image.as<float>(), stream_), "wrk_average");
         r.done = true;
         r.tDone = -1;  // stamped after the step's synchronisation
     }
+}
+
+// Admits the next chunk of r into a free pool range and slot and generates
+// its camera rays behind the queue entries already placed. Returns false
+// when no slot, no memory or no large enough range is free.
+bool Engine::admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase) {
+    if (freeSlots_.empty() || capacity_ == 0) return false;
+    if (!r.admitted && cfg_.memoryBudgetBytes && footprintUsed_ != 0 &&
+        footprintUsed_ + r.footprint > cfg_.memoryBudgetBytes)
+        return false;  // back-pressure: wait for memory to be released
+    const uint64_t want = r.endPath - r.nextPath;
+    const uint64_t chunkCap = cfg_.maxChunkPaths ? std::min<uint64_t>(cfg_.maxChunkPaths, capacity_) : capacity_;
+    const uint64_t full = std::min<uint64_t>(want, chunkCap);  // the chunk an empty pool would take
+    // smaller chunks wait for more paths to end, unless nothing is in flight
+    const uint64_t floor = cfg_.minChunkPaths ? cfg_.minChunkPaths
+                                              : std::max<uint64_t>(capacity_ / 32, std::min<uint64_t>(65536, capacity_ / 8));
+    const uint64_t minChunk = numActiveSlots() == 0 ? 1 : std::min<uint64_t>(full, floor);
+    uint32_t n = (uint32_t)full;
+    uint32_t base = 0;
+    if (!allocRange(n, &base)) {
+        const uint32_t m = largestFree();
+        if (m == 0 || m < minChunk) return false;
+        n = m;
+        if (!allocRange(n, &base)) return false;
+    }
+    if (!r.admitted) {
+        if (r.mode == Mode::Accumulate) {
+            r.accum.alloc((size_t)r.npix * 3 * sizeof(float));
+            r.image.alloc((size_t)r.npix * 3 * sizeof(float));
+            hip_check(hipMemsetAsync(r.accum.ptr, 0, (size_t)r.npix * 3 * sizeof(float), stream_), "hipMemsetAsync");
+        }
+        r.admitted = true;
+        r.tAdmit = now();
+        footprintUsed_ += r.footprint;
+    }
+    const uint32_t s = freeSlots_.back();
+    freeSlots_.pop_back();
+    Slot& sl = slots_[s];
+    sl.active = true;
+    sl.finished = false;
+    sl.request = r.id;
+    sl.poolBase = base;
+    sl.count = n;
+    sl.firstPath = r.nextPath;
+    // pinned staging: each slot's entries are written once per step and the
+    // step ends with a stream synchronisation
+    hostPrm_[s] = r.prm;
+    hostLiveInit_[s] = n;
+    hip_check(hipMemcpyAsync(prmTable_.as<wrk_render_params>() + s, hostPrm_ + s, sizeof(wrk_render_params),
+                             hipMemcpyHostToDevice, stream_),
+              "hipMemcpyAsync");
+    hip_check(hipMemcpyAsync(live_.as<uint32_t>() + s, hostLiveInit_ + s, sizeof(uint32_t), hipMemcpyHostToDevice,
+                             stream_),
+              "hipMemcpyAsync");
+    const uint32_t at = queueBase + *added;
+    timed(T_GENERATE, [&] {
+        wrk_check(wrk_generate(&r.cam, &r.prm, &paths_, r.nextPath, n, base, s, qB_.as<uint32_t>() + at,
+                               cat_.as<uint8_t>() + at, 1 + 3 * r.sceneIdx, stream_),
+                  "wrk_generate");
+    });
+    *added += n;
+    r.nextPath += n;
+    r.chunksInFlight++;
+    r.chunks++;
+    stats_.batches++;
+    return true;
 }
 
 bool Engine::step() {
@@ -282,162 +370,125 @@ bool Engine::step() {
     eventNext_ = 0;
     marks_.clear();
 
-    // --- one iteration over the merged queue, scene by scene
+    // --- one iteration over the merged queue, scene by scene. A scene's
+    // part is [trace | trace + shadow | shadow]: the closest-hit launch takes
+    // the first two pieces, the any-hit launch (shadow rays in savedOrg /
+    // savedDir) the last two.
     uint32_t total = 0;
+    uint32_t* const qlive = qa + deadCount_;  // qa = [finished last iteration | live parts]
     for (uint32_t s = 0; s < S; ++s) {
-        const uint32_t nC = segC_[s], nS = segS_[s];
-        const uint32_t a = total;
-        total += nC + nS;
-        if (nC + nS == 0) continue;
+        const uint32_t nT = segT_[s], nTS = segTS_[s], nS = segS_[s];
+        const uint32_t a = total, n = nT + nTS + nS;
+        total += n;
+        if (n == 0) continue;
         const wrk_bvh& bvh = scenes_[s]->getBVH();
         const wrk_scene_view& view = scenes_[s]->getDeviceView();
+        const uint32_t nC = nT + nTS, nA = nTS + nS;
         if (nC) {
             timed(T_CLOSEST, [&] {
                 if (cfg_.countTraversal)
-                    wrk_check(wrk_trace_closest_counted(&bvh, org_.ptr, dir_.ptr, qa + a, nC, nullptr,
+                    wrk_check(wrk_trace_closest_counted(&bvh, org_.ptr, dir_.ptr, qlive + a, nC, nullptr,
                                                         hit_.as<uint32_t>(), fc, stream_),
                               "wrk_trace_closest_counted");
                 else
-                    wrk_check(wrk_trace_closest(&bvh, org_.ptr, dir_.ptr, qa + a, nC, nullptr, hit_.as<uint32_t>(),
+                    wrk_check(wrk_trace_closest(&bvh, org_.ptr, dir_.ptr, qlive + a, nC, nullptr, hit_.as<uint32_t>(),
                                                 stream_),
                               "wrk_trace_closest");
             });
             stats_.launchesTraceClosest++;
         }
-        if (nS) {
+        if (nA) {
             timed(T_ANY, [&] {
                 if (cfg_.countTraversal)
-                    wrk_check(wrk_trace_any_counted(&bvh, org_.ptr, dir_.ptr, qa + a + nC, nS, nullptr,
+                    wrk_check(wrk_trace_any_counted(&bvh, savedOrg_.ptr, savedDir_.ptr, qlive + a + nT, nA, nullptr,
                                                     occl_.as<uint8_t>(), fc + 2, stream_),
                               "wrk_trace_any_counted");
                 else
-                    wrk_check(wrk_trace_any(&bvh, org_.ptr, dir_.ptr, qa + a + nC, nS, nullptr, occl_.as<uint8_t>(),
-                                            stream_),
+                    wrk_check(wrk_trace_any(&bvh, savedOrg_.ptr, savedDir_.ptr, qlive + a + nT, nA, nullptr,
+                                            occl_.as<uint8_t>(), stream_),
                               "wrk_trace_any");
             });
             stats_.launchesTraceAny++;
         }
         stats_.closestRays += nC;
-        stats_.shadowRays += nS;
+        stats_.shadowRays += nA;
         timed(T_CLASSIFY, [&] {
-            wrk_check(wrk_classify(&view, prmTable, &paths_, qa + a, nC, nS, cat + a, stream_), "wrk_classify");
+            wrk_check(wrk_classify(&view, prmTable, &paths_, qlive + a, nT, nTS, nS, cat + a, stream_),
+                      "wrk_classify");
         });
         timed(T_PARTITION, [&] {
-            wrk_check(wrk_partition(cat + a, qa + a, nC + nS, 5, qb + a, off + 8 * s, scratch_.ptr, scratch_.bytes,
+            // five material queues + the paths classify finished
+            wrk_check(wrk_partition(cat + a, qlive + a, n, 6, qb + a, off + 8 * s, scratch_.ptr, scratch_.bytes,
                                     stream_),
                       "wrk_partition");
         });
         timed(T_SHADE, [&] {
-            wrk_check(wrk_shade(&view, prmTable, &paths_, qb + a, off + 8 * s, nC + nS, 2 * s, cat + a, stream_),
+            wrk_check(wrk_shade(&view, prmTable, &paths_, qb + a, off + 8 * s, n, 1 + 3 * s, 0, cat + a, stream_),
                       "wrk_shade");
         });
     }
     if (total) stats_.iterations++;
 
-    // --- admission: new chunks join behind the shaded paths
+    // --- admission: new chunks join behind the shaded paths. Pending
+    // requests take one chunk each per round, in arrival order, until slots
+    // or pool ranges run out; a request that does not fit the memory budget
+    // stops admission of the requests behind it.
     uint32_t added = 0;
-    while (!pending_.empty() && !freeSlots_.empty()) {
-        RequestState& r = *requests_.at(pending_.front());
-        if (!r.admitted) {
-            if (cfg_.memoryBudgetBytes && footprintUsed_ + r.footprint > cfg_.memoryBudgetBytes &&
-                footprintUsed_ != 0)
-                break;  // back-pressure: wait for memory to be released
-        }
-        uint64_t want = r.endPath - r.nextPath;
-        uint64_t chunkCap = cfg_.maxChunkPaths ? cfg_.maxChunkPaths : capacity_;
-        if (capacity_ == 0) break;
-        uint32_t n = (uint32_t)std::min<uint64_t>(want, std::min<uint64_t>(chunkCap, capacity_));
-        uint32_t base = 0;
-        if (!allocRange(n, &base)) {
-            if (numActiveSlots() != 0) break;  // wait for paths to drain
-            n = largestFree();
-            if (n == 0 || !allocRange(n, &base)) break;
-        }
-        if (!r.admitted) {
-            if (r.mode == Mode::Accumulate) {
-                r.accum.alloc((size_t)r.npix * 3 * sizeof(float));
-                r.image.alloc((size_t)r.npix * 3 * sizeof(float));
-                hip_check(hipMemsetAsync(r.accum.ptr, 0, (size_t)r.npix * 3 * sizeof(float), stream_),
-                          "hipMemsetAsync");
+    for (bool progress = true; progress && !pending_.empty();) {
+        progress = false;
+        for (auto it = pending_.begin(); it != pending_.end();) {
+            RequestState& r = *requests_.at(*it);
+            if (!admitChunk(r, &added, total)) {
+                if (!r.admitted) break;
+                ++it;
+                continue;
             }
-            r.admitted = true;
-            r.tAdmit = now();
-            footprintUsed_ += r.footprint;
+            progress = true;
+            if (r.nextPath == r.endPath) it = pending_.erase(it);
+            else ++it;
         }
-        uint32_t s = freeSlots_.back();
-        freeSlots_.pop_back();
-        Slot& sl = slots_[s];
-        sl.active = true;
-        sl.request = r.id;
-        sl.poolBase = base;
-        sl.count = n;
-        sl.firstPath = r.nextPath;
-        // pinned staging: each slot's entry is written once per step and the
-        // step ends with a stream synchronisation
-        hostPrm_[s] = r.prm;
-        hip_check(hipMemcpyAsync(prmTable_.as<wrk_render_params>() + s, hostPrm_ + s, sizeof(wrk_render_params),
-                                 hipMemcpyHostToDevice, stream_),
-                  "hipMemcpyAsync");
-        rangesDirty_ = true;
-        timed(T_GENERATE, [&] {
-            wrk_check(wrk_generate(&r.cam, &r.prm, &paths_, r.nextPath, n, base, s, qb + total + added,
-                                   cat + total + added, 2 * r.sceneIdx, stream_),
-                      "wrk_generate");
-        });
-        added += n;
-        r.nextPath += n;
-        r.inFlight = true;
-        pending_.pop_front();
+    }
+    if (numActiveSlots() == 0 && !pending_.empty()) {
+        // an idle pool admits any request unless the memory budget holds it
+        // back until the caller fetches finished images
+        const RequestState& r = *requests_.at(pending_.front());
+        const bool memoryBound = cfg_.memoryBudgetBytes && footprintUsed_ != 0 &&
+                                 footprintUsed_ + r.footprint > cfg_.memoryBudgetBytes;
+        if (!memoryBound) throw std::runtime_error("wavefront admission made no progress on an idle pool");
     }
 
-    // --- rebuild the queue: [scene0 closest | scene0 shadow | scene1 ...]
+    // --- rebuild the queue: [finished | per scene: trace, trace + shadow, shadow]
     const uint32_t S2 = (uint32_t)scenes_.size();
+    const uint32_t K = 1 + 3 * S2;
     const uint32_t items = total + added;
     if (items) {
         timed(T_PARTITION, [&] {
-            wrk_check(wrk_partition(cat, qb, items, 2 * S2, qa, off + 48, scratch_.ptr, scratch_.bytes, stream_),
+            wrk_check(wrk_partition(cat, qb, items, K, qa, off + 48, scratch_.ptr, scratch_.bytes, stream_),
                       "wrk_partition");
         });
-        hip_check(hipMemcpyAsync(hostOffs_, off + 48, (2 * S2 + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+        hip_check(hipMemcpyAsync(hostOffs_, off + 48, (K + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
                   "hipMemcpyAsync");
-    }
-    // live paths per slot = queue entries inside its pool range
-    const uint32_t numRanges = (uint32_t)numActiveSlots();
-    if (numRanges) {
-        if (rangesDirty_) {
-            uint32_t k = 0;
-            std::vector<std::pair<uint32_t, uint32_t>> act;
-            for (uint32_t s = 0; s < WRK_MAX_SLOTS; ++s)
-                if (slots_[s].active) act.push_back({slots_[s].poolBase, s});
-            std::sort(act.begin(), act.end());
-            for (auto& e : act) {
-                hostRanges_[k] = e.first;
-                hostRanges_[numRanges + k] = e.second;
-                ++k;
-            }
-            hip_check(hipMemcpyAsync(ranges_.ptr, hostRanges_, 2 * numRanges * sizeof(uint32_t),
-                                     hipMemcpyHostToDevice, stream_),
-                      "hipMemcpyAsync");
-            rangesDirty_ = false;
-        }
+        // live paths per slot: subtract the paths that ended
         timed(T_PARTITION, [&] {
-            wrk_check(wrk_slot_count(qa, items, items ? off + 48 + 2 * S2 : nullptr, ranges_.as<uint32_t>(),
-                                     numRanges, live, stream_),
-                      "wrk_slot_count");
+            wrk_check(wrk_slot_finish(qa, items, off + 48 + 1, &paths_, live, stream_), "wrk_slot_finish");
         });
+    }
+    if (numActiveSlots())
         hip_check(hipMemcpyAsync(hostLive_, live, WRK_MAX_SLOTS * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
                   "hipMemcpyAsync");
-    }
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    deadCount_ = items ? hostOffs_[1] : 0;
     for (uint32_t s = 0; s < S2; ++s) {
-        segC_[s] = items ? hostOffs_[2 * s + 1] - hostOffs_[2 * s] : 0;
-        segS_[s] = items ? hostOffs_[2 * s + 2] - hostOffs_[2 * s + 1] : 0;
+        const uint32_t* o = hostOffs_ + 1 + 3 * s;
+        segT_[s] = items ? o[1] - o[0] : 0;
+        segTS_[s] = items ? o[2] - o[1] : 0;
+        segS_[s] = items ? o[3] - o[2] : 0;
     }
 
     // --- completions
     bool any = false;
     for (uint32_t s = 0; s < WRK_MAX_SLOTS; ++s) {
-        if (slots_[s].active && hostLive_[s] == 0) {
+        if (slots_[s].active && !slots_[s].finished && hostLive_[s] == 0) {
             complete(s);
             any = true;
         }

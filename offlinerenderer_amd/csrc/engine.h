@@ -2,7 +2,10 @@
 // wr::RenderServer. One path pool holds paths of several requests (up to
 // WRK_MAX_SLOTS admitted chunks, up to 4 resident scenes); every iteration
 // traces, classifies and shades each scene's share of the merged queue and
-// rebuilds the queue with one ordered partition. Replaces the host loop of
+// rebuilds the queue with one ordered partition. Each admitted chunk holds a
+// contiguous pool range (queue-ordered state accesses stay coalesced) until
+// its last path ends; new chunks fill freed ranges while the deep paths of
+// other chunks finish. Replaces the host loop of
 // libWR/src/internal/renderer.cpp:190-284.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -48,6 +51,7 @@ wrk_camera camera_record(const Camera& c);
 struct EngineConfig {
     uint32_t capacity = 1u << 26;       // paths in flight
     uint32_t maxChunkPaths = 0;          // 0 = capacity
+    uint32_t minChunkPaths = 0;          // 0 = max(capacity / 32, 65536)
     uint64_t memoryBudgetBytes = 0;      // 0 = unlimited (beyond the pool itself)
     bool timeKernels = false;
     bool countTraversal = false;
@@ -67,7 +71,14 @@ public:
         uint32_t spp = 1, npix = 0;
         float* pathsOut = nullptr;  // host, Mode::Paths
         DevBuf accum, image;
-        bool inFlight = false, done = false, admitted = false;
+        // chunks finish in any order; their results enter the accumulation
+        // buffer in sample order: accumNext is the first path not yet added,
+        // held maps the first path of each finished chunk waiting for its
+        // predecessors to its slot
+        uint64_t accumNext = 0, finishedPaths = 0;
+        std::map<uint64_t, uint32_t> held;
+        uint32_t chunksInFlight = 0, chunks = 0;
+        bool done = false, admitted = false;
         double tSubmit = 0, tAdmit = 0, tDone = 0;
         uint64_t footprint = 0;
     };
@@ -102,8 +113,9 @@ public:
 private:
     struct Slot {
         bool active = false;
+        bool finished = false;  // all paths done, results held for ordered accumulation
         uint64_t request = 0;
-        uint32_t poolBase = 0, count = 0;
+        uint32_t poolBase = 0, count = 0;  // pool range
         uint64_t firstPath = 0;
     };
     void reserve(uint32_t cap);
@@ -112,6 +124,8 @@ private:
     void freeRange(uint32_t base, uint32_t n);
     uint32_t largestFree() const;
     void complete(uint32_t slot);
+    void retire(uint32_t slot);
+    bool admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase);
     template <typename F> void timed(int category, F&& launch);
 
     EngineConfig cfg_;
@@ -119,16 +133,17 @@ private:
     hipStream_t stream_ = nullptr;
     uint32_t capacity_ = 0;
     DevBuf org_, dir_, thr_, rad_, st_, contrib_, savedOrg_, savedDir_, hit_, occl_, result_;
-    DevBuf qA_, qB_, cat_, scratch_, offs_, prmTable_, live_, ranges_, fetch_;
+    DevBuf qA_, qB_, cat_, scratch_, offs_, prmTable_, live_, fetch_;
     wrk_paths paths_{};
+    uint32_t deadCount_ = 0;              // entries ended last iteration: qA_[0, deadCount_)
     uint32_t* hostOffs_ = nullptr;        // pinned
-    uint32_t* hostLive_ = nullptr;        // pinned
+    uint32_t* hostLive_ = nullptr;        // pinned: live paths per slot, read back each step
+    uint32_t* hostLiveInit_ = nullptr;    // pinned staging: initial live count per slot
     wrk_render_params* hostPrm_ = nullptr;  // pinned staging for the slot table
-    uint32_t* hostRanges_ = nullptr;      // pinned: [first pool positions | slots], ascending
-    bool rangesDirty_ = true;
     std::vector<Scene*> scenes_;
-    std::vector<uint32_t> segC_, segS_;   // per-scene closest/shadow queue sizes
-    std::map<uint32_t, uint32_t> free_;   // pool ranges: base -> length
+    // per-scene queue parts: trace only, trace + shadow, shadow only
+    std::vector<uint32_t> segT_, segTS_, segS_;
+    std::map<uint32_t, uint32_t> free_;   // free pool ranges: base -> length
     Slot slots_[WRK_MAX_SLOTS];
     std::vector<uint32_t> freeSlots_;
     std::deque<uint64_t> pending_;

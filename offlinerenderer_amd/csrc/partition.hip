@@ -19,7 +19,7 @@ namespace wrk {
 
 constexpr uint32_t kTile = 1024;       // items per tile (256 lanes x 4 rounds)
 constexpr uint32_t kRounds = kTile / 256;
-constexpr uint32_t kMaxCat = 8;
+constexpr uint32_t kMaxCat = 16;
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
     uint32_t lane = threadIdx.x & 63;
@@ -230,7 +230,15 @@ extern "C" int wrk_partition(const uint8_t* category, const uint32_t* in, uint32
         case 6: return partition_impl<6>(category, in, n, out, offsetsOut, scratch, scratchBytes, s);
         case 7: return partition_impl<7>(category, in, n, out, offsetsOut, scratch, scratchBytes, s);
         case 8: return partition_impl<8>(category, in, n, out, offsetsOut, scratch, scratchBytes, s);
-        default: set_error("wrk_partition: numCategories must be in [1, 8]"); return WRK_ERR_INVALID;
+        case 9: return partition_impl<9>(category, in, n, out, offsetsOut, scratch, scratchBytes, s);
+        case 10: return partition_impl<10>(category, in, n, out, offsetsOut, scratch, scratchBytes, s);
+        case 11: return partition_impl<11>(category, in, n, out, offsetsOut, scratch, scratchBytes, s);
+        case 12: return partition_impl<12>(category, in, n, out, offsetsOut, scratch, scratchBytes, s);
+        case 13: return partition_impl<13>(category, in, n, out, offsetsOut, scratch, scratchBytes, s);
+        case 14: return partition_impl<14>(category, in, n, out, offsetsOut, scratch, scratchBytes, s);
+        case 15: return partition_impl<15>(category, in, n, out, offsetsOut, scratch, scratchBytes, s);
+        case 16: return partition_impl<16>(category, in, n, out, offsetsOut, scratch, scratchBytes, s);
+        default: set_error("wrk_partition: numCategories must be in [1, 16]"); return WRK_ERR_INVALID;
     }
 }
 

@@ -12,7 +12,12 @@
 
 namespace wrk {
 
-enum : uint32_t { CAT_DONE = 5, NEXT_TRACE = 0, NEXT_SHADOW = 1, NEXT_DONE = 2 };
+// Shade categories 0..4 are the material kinds; CAT_DONE ends the path.
+// After shading a path continues as one of three queue kinds: trace the
+// continuation ray, trace it together with a shadow ray, or trace a shadow
+// ray alone (its continuation either ended or, under an iteration cap,
+// waits for the shadow result as in libWR's stage machine).
+enum : uint32_t { CAT_DONE = 5, NEXT_TRACE = 0, NEXT_TRACE_SHADOW = 1, NEXT_SHADOW = 2, NEXT_DONE = 3 };
 
 struct Pool {
     float4* org;
@@ -114,20 +119,31 @@ __global__ __launch_bounds__(256) void k_generate(wrk_camera cam, wrk_render_par
 }
 
 // ------------------------------------------------------------- classify
-// Closest-hit results (queue[0, nC)): miss -> environment + flush
+// Queue of one scene: [trace nT | trace + shadow nTS | shadow nS].
+// Shadow results first (trace.cu:27-51): add the NEE contribution when
+// unoccluded. Then closest-hit results: miss -> environment + flush
 // (trace.cu:143-148); hit -> category = material kind of the primitive
 // (unknown kinds join the emissive queue, whose shader flushes them like
 // the default case of shade.cu:79-81; an out-of-range material id ends the
-// path without a flush, shade.cu:35-38). Shadow results (queue[nC, nC+nS)):
-// add the NEE contribution when unoccluded, restore the direction, and
-// resume in the Lambertian shader (trace.cu:27-51).
+// path without a flush, shade.cu:35-38). A shadow-only entry either resumes
+// its deferred continuation in the Lambertian shader (neeDone set, iteration
+// cap mode) or ends: its continuation terminated while the shadow ray was in
+// flight, so the radiance is flushed now that the contribution is in.
 __global__ __launch_bounds__(256) void k_classify(SceneDev S, const wrk_render_params* __restrict__ prmTable, Pool P,
-                                                  const uint32_t* queue, uint32_t nC, uint32_t nS, uint8_t* cat) {
+                                                  const uint32_t* queue, uint32_t nT, uint32_t nTS, uint32_t nS,
+                                                  uint8_t* cat) {
     uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nC + nS) return;
+    if (i >= nT + nTS + nS) return;
     uint32_t p = queue[i];
     uint32_t c;
-    if (i < nC) {
+    if (i >= nT) {
+        if (!P.occl[p]) {
+            float4 L = P.rad[p];
+            float4 C = P.contrib[p];
+            P.rad[p] = make_float4(L.x + C.x, L.y + C.y, L.z + C.z, L.w);
+        }
+    }
+    if (i < nT + nTS) {
         uint2 h = P.hit[p];
         if (h.y == 0xFFFFFFFFu) {
             float4 T = P.thr[p], L = P.rad[p];
@@ -146,17 +162,13 @@ __global__ __launch_bounds__(256) void k_classify(SceneDev S, const wrk_render_p
                 c = type <= WRK_MAT_GGX_TRANSMISSION ? type : (uint32_t)WRK_MAT_EMISSIVE;
             }
         }
-    } else {
-        float4 L = P.rad[p];
-        if (!P.occl[p]) {
-            float4 C = P.contrib[p];
-            L = make_float4(L.x + C.x, L.y + C.y, L.z + C.z, L.w);
-            P.rad[p] = L;
-        }
-        float4 sd = P.savedDir[p];
-        float4 d = P.dir[p];
-        P.dir[p] = make_float4(sd.x, sd.y, sd.z, d.w);
+    } else if (P.st[p].z & FLAG_NEE_DONE) {
         c = WRK_MAT_LAMBERTIAN;
+    } else {
+        const uint32_t slot = P.st[p].z >> FLAG_SLOT_SHIFT;
+        float4 L = P.rad[p];
+        P.result[p] = f4(clamp_fireflies(mk(L.x, L.y, L.z), prmTable[slot].fireflyClamp), 0.0f);
+        c = CAT_DONE;
     }
     cat[i] = (uint8_t)c;
 }
@@ -178,7 +190,8 @@ struct PathState {
     uint32_t pixel;
     bool flushed, done, toShadow;
     f3 flushL;
-    f3 contrib, savedOrg, savedDir;
+    f3 contrib, shOrg, shDir;
+    float shTMax;
 };
 
 __device__ __forceinline__ f3 vert(const SceneDev& S, uint32_t i) {
@@ -279,8 +292,13 @@ __device__ __forceinline__ bool valid_contrib(f3 c) {
 }
 
 // materials/lambertian.cuh:8-147
+// The shadow ray goes to shOrg/shDir. Under an iteration cap the shader
+// returns here and resumes after the shadow test (libWR's stage machine);
+// otherwise it samples the continuation at once and both rays are traced in
+// the next iteration (same random dimensions, same arithmetic, and the
+// contribution still enters L before anything the continuation adds).
 __device__ void shade_lambertian(const SceneDev& S, const wrk_render_params& prm, PathState& r, const Hit& h,
-                                 const wrk_material& mat) {
+                                 const wrk_material& mat, bool deferContinuation) {
     if (prm.useNEE && S.numEmissive > 0 && !r.neeDone) {
         LightSample ls;
         if (sample_light(S, r, h, ls)) {
@@ -292,15 +310,14 @@ __device__ void shade_lambertian(const SceneDev& S, const wrk_render_params& prm
                       r.T.z * bsdf.z * ls.Le.z * w / ls.lightPdf);
             if (valid_contrib(c)) {
                 r.contrib = c;
-                r.savedDir = r.d;
-                r.savedOrg = r.o;
-                r.d = ls.toLight;
-                r.o = offset_origin(h.position, h.normal, ls.toLight);
-                r.tMin = 1e-5f;
-                r.tMax = ls.dist - 1e-5f;
-                r.neeDone = true;
+                r.shDir = ls.toLight;
+                r.shOrg = offset_origin(h.position, h.normal, ls.toLight);
+                r.shTMax = ls.dist - 1e-5f;
                 r.toShadow = true;
-                return;
+                if (deferContinuation) {
+                    r.neeDone = true;
+                    return;
+                }
             }
         }
     }
@@ -464,7 +481,8 @@ __device__ __forceinline__ f3 effective_normal(const SceneDev& S, const wrk_mate
 // as finished.
 __global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_params* __restrict__ prmTable, Pool P,
                                                const uint32_t* queue, const uint32_t* __restrict__ offsets,
-                                               uint32_t maxItems, uint32_t catBase, uint8_t* nextStage) {
+                                               uint32_t maxItems, uint32_t catBase, uint32_t catFinished,
+                                               uint8_t* nextStage) {
     uint32_t off[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) off[k] = offsets[k];
@@ -479,7 +497,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_para
     }
     if (seg < 0) {
         uint32_t j = off[5] + b * 256 + threadIdx.x;
-        if (j < maxItems) nextStage[j] = 0xFFu;
+        if (j < maxItems) nextStage[j] = (uint8_t)catFinished;
         return;
     }
     uint32_t j = off[seg] + b * 256 + threadIdx.x;
@@ -497,7 +515,8 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_para
     r.prevDelta = (st.z & FLAG_PREV_DELTA) != 0;
     r.neeDone = (st.z & FLAG_NEE_DONE) != 0;
     r.pixel = st.w;
-    float4 o4 = r.neeDone ? P.savedOrg[p] : P.org[p];
+    const bool defer = prm.iterationCap != 0;
+    float4 o4 = P.org[p];
     float4 d4 = P.dir[p];
     float4 T4 = P.thr[p], L4 = P.rad[p];
     r.o = xyz(o4); r.tMin = o4.w; r.d = xyz(d4); r.tMax = d4.w;
@@ -530,24 +549,28 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_para
     } else {
         Hit eh = h;
         if (mat.normalTextureId > 0) eh.normal = effective_normal(S, mat, h);
-        if (seg == WRK_MAT_LAMBERTIAN) shade_lambertian(S, prm, r, eh, mat);
+        if (seg == WRK_MAT_LAMBERTIAN) shade_lambertian(S, prm, r, eh, mat, defer);
         else if (seg == WRK_MAT_GGX_REFLECTION) shade_ggx(S, prm, r, eh, mat);
         else shade_glass(S, prm, r, eh, mat);
     }
 
     uint32_t next;
-    if (r.done) {
+    if (r.done && r.toShadow) {
+        next = NEXT_SHADOW;  // flushed once the shadow result is in
+    } else if (r.done) {
         next = NEXT_DONE;
         if (r.flushed) P.result[p] = f4(r.flushL, 0.0f);
-    } else if (prm.iterationCap != 0 && iter >= prm.iterationCap) {
+    } else if (defer && iter >= prm.iterationCap) {
         next = NEXT_DONE;  // renderer.cpp:213 cap: the path is dropped unflushed
     } else if (r.toShadow) {
-        next = NEXT_SHADOW;
-        P.contrib[p] = f4(r.contrib, 0.0f);
-        P.savedOrg[p] = f4(r.savedOrg, r.tMin);
-        P.savedDir[p] = f4(r.savedDir, 0.0f);
+        next = defer ? NEXT_SHADOW : NEXT_TRACE_SHADOW;
     } else {
         next = NEXT_TRACE;
+    }
+    if (r.toShadow && next != NEXT_DONE) {
+        P.contrib[p] = f4(r.contrib, 0.0f);
+        P.savedOrg[p] = f4(r.shOrg, 1e-5f);
+        P.savedDir[p] = f4(r.shDir, r.shTMax);
     }
     if (next != NEXT_DONE) {
         P.org[p] = f4(r.o, r.tMin);
@@ -559,24 +582,20 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_para
                          (slot << FLAG_SLOT_SHIFT);
         P.st[p] = make_uint4(r.seed, r.dim, flags, r.pixel);
     }
-    nextStage[j] = next == NEXT_DONE ? (uint8_t)0xFFu : (uint8_t)(catBase + next);
+    nextStage[j] = next == NEXT_DONE ? (uint8_t)catFinished : (uint8_t)(catBase + next);
 }
 
-// ------------------------------------------------------------ slot count
-// Live paths per slot = queue entries inside the slot's pool range. A block
-// counts 256 * perThread entries: each lane merges runs of one range before
-// adding them to an LDS histogram, and the block adds its nonzero bins to
-// counts[], so the device atomics per launch stay near one per block.
-__global__ __launch_bounds__(256) void k_clear_counts(uint32_t* counts) { counts[threadIdx.x] = 0u; }
-
-__global__ __launch_bounds__(256) void k_slot_count(const uint32_t* __restrict__ queue, uint32_t maxItems,
-                                                    const uint32_t* __restrict__ countDev,
-                                                    const uint32_t* __restrict__ ranges, uint32_t numRanges,
-                                                    uint32_t perThread, uint32_t* counts) {
-    __shared__ uint32_t base[kMaxSlots];
+// ----------------------------------------------------------- slot finish
+// live[slot] -= paths of that slot among the entries that ended this
+// iteration. A block takes 256 * perThread entries; each lane merges runs of
+// one slot before adding them to an LDS histogram, and the block subtracts
+// its nonzero bins from live[], so device atomics stay near one per block.
+__global__ __launch_bounds__(256) void k_slot_finish(const uint32_t* __restrict__ finished, uint32_t maxItems,
+                                                     const uint32_t* __restrict__ countDev,
+                                                     const uint4* __restrict__ st, uint32_t perThread,
+                                                     uint32_t* live) {
     __shared__ uint32_t hist[kMaxSlots];
     const uint32_t t = threadIdx.x;
-    base[t] = t < numRanges ? ranges[t] : 0xFFFFFFFFu;
     hist[t] = 0u;
     __syncthreads();
     uint32_t n = maxItems;
@@ -589,28 +608,25 @@ __global__ __launch_bounds__(256) void k_slot_count(const uint32_t* __restrict__
 #pragma unroll
         for (uint32_t u = 0; u < kBatch; ++u) {
             uint64_t i = start + (uint64_t)(k0 + u) * 256u + t;
-            p[u] = (k0 + u < perThread && i < n) ? queue[i] : 0xFFFFFFFFu;
+            p[u] = (k0 + u < perThread && i < n) ? finished[i] : 0xFFFFFFFFu;
         }
+        uint32_t sl[kBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) sl[u] = p[u] != 0xFFFFFFFFu ? (st[p[u]].z >> FLAG_SLOT_SHIFT) : 0xFFFFFFFFu;
 #pragma unroll
         for (uint32_t u = 0; u < kBatch; ++u) {
-            if (p[u] == 0xFFFFFFFFu) continue;
-            // last range whose first position is <= p
-            uint32_t lo = 0u, hi = numRanges;
-            while (hi - lo > 1u) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (base[mid] <= p[u]) lo = mid; else hi = mid;
-            }
-            if (lo != cur && run) {
+            if (sl[u] == 0xFFFFFFFFu) continue;
+            if (sl[u] != cur && run) {
                 atomicAdd(&hist[cur], run);
                 run = 0u;
             }
-            cur = lo;
+            cur = sl[u];
             ++run;
         }
     }
     if (run) atomicAdd(&hist[cur], run);
     __syncthreads();
-    if (t < numRanges && hist[t]) atomicAdd(&counts[ranges[numRanges + t]], hist[t]);
+    if (hist[t]) atomicSub(&live[t], hist[t]);
 }
 
 // accum[pixel] += result[path] over the batch in sample order.
@@ -667,49 +683,48 @@ extern "C" int wrk_generate(const wrk_camera* cam, const wrk_render_params* para
 }
 
 extern "C" int wrk_classify(const wrk_scene_view* scene, const wrk_render_params* paramsTable, const wrk_paths* paths,
-                            const uint32_t* queue, uint32_t numClosest, uint32_t numShadow, uint8_t* category,
-                            void* stream) {
+                            const uint32_t* queue, uint32_t numTrace, uint32_t numTraceShadow, uint32_t numShadow,
+                            uint8_t* category, void* stream) {
     WRK_REQUIRE(scene && paramsTable && paths && queue && category, "wrk_classify: null argument");
-    uint32_t n = numClosest + numShadow;
+    const uint64_t n = (uint64_t)numTrace + numTraceShadow + numShadow;
+    WRK_REQUIRE(n <= paths->capacity, "wrk_classify: queue longer than the path pool");
     if (n == 0) return WRK_OK;
     hipLaunchKernelGGL(k_classify, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), scene_of(scene),
-                       paramsTable, pool_of(paths), queue, numClosest, numShadow, category);
+                       paramsTable, pool_of(paths), queue, numTrace, numTraceShadow, numShadow, category);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }
 
 extern "C" int wrk_shade(const wrk_scene_view* scene, const wrk_render_params* paramsTable, const wrk_paths* paths,
                          const uint32_t* queue, const uint32_t* offsets, uint32_t maxItems, uint32_t categoryBase,
-                         uint8_t* nextCategory, void* stream) {
+                         uint32_t finishedCategory, uint8_t* nextCategory, void* stream) {
     WRK_REQUIRE(scene && paramsTable && paths && queue && offsets && nextCategory, "wrk_shade: null argument");
-    WRK_REQUIRE(categoryBase + 1u < 0xFFu, "wrk_shade: categoryBase out of range");
+    WRK_REQUIRE(categoryBase + 2u <= 0xFFu && finishedCategory <= 0xFFu, "wrk_shade: category out of range");
     if (maxItems == 0) return WRK_OK;
     hipLaunchKernelGGL(k_shade, dim3(ceil_div(maxItems, 256) + 6), dim3(256), 0, as_stream(stream), scene_of(scene),
-                       paramsTable, pool_of(paths), queue, offsets, maxItems, categoryBase, nextCategory);
+                       paramsTable, pool_of(paths), queue, offsets, maxItems, categoryBase, finishedCategory,
+                       nextCategory);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }
 
-extern "C" int wrk_slot_count(const uint32_t* queue, uint32_t maxItems, const uint32_t* countDev,
-                              const uint32_t* ranges, uint32_t numRanges, uint32_t* counts, void* stream) {
-    WRK_REQUIRE(counts, "wrk_slot_count: null counts");
-    WRK_REQUIRE(numRanges <= kMaxSlots, "wrk_slot_count: more ranges than slots");
-    hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(kMaxSlots), 0, as_stream(stream), counts);
-    WRK_CHECK_LAUNCH();
-    if (maxItems == 0 || numRanges == 0) return WRK_OK;
-    WRK_REQUIRE(queue && ranges, "wrk_slot_count: null argument");
-    // about 2048 blocks for large queues, one entry per lane for small ones
-    uint32_t perThread = (uint32_t)ceil_div(maxItems, 256ull * 2048ull);
+extern "C" int wrk_slot_finish(const uint32_t* finished, uint32_t maxItems, const uint32_t* countDev,
+                               const wrk_paths* paths, uint32_t* live, void* stream) {
+    WRK_REQUIRE(paths && live, "wrk_slot_finish: null argument");
+    if (maxItems == 0) return WRK_OK;
+    WRK_REQUIRE(finished, "wrk_slot_finish: null argument");
+    // about 2048 blocks for long lists, one entry per lane for short ones
+    uint32_t perThread = ceil_div(maxItems, 256ull * 2048ull);
     perThread = perThread < 1u ? 1u : (perThread > 256u ? 256u : perThread);
     uint32_t blocks = ceil_div(maxItems, 256ull * perThread);
-    hipLaunchKernelGGL(k_slot_count, dim3(blocks), dim3(256), 0, as_stream(stream), queue, maxItems, countDev,
-                       ranges, numRanges, perThread, counts);
+    hipLaunchKernelGGL(k_slot_finish, dim3(blocks), dim3(256), 0, as_stream(stream), finished, maxItems, countDev,
+                       (const uint4*)paths->state, perThread, live);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }
 
-extern "C" int wrk_accumulate(const wrk_paths* paths, uint32_t poolBase, uint64_t first, uint32_t n, uint32_t npix,
-                              float* accum, void* stream) {
+extern "C" int wrk_accumulate(const wrk_paths* paths, uint32_t poolBase, uint64_t first, uint32_t n,
+                              uint32_t npix, float* accum, void* stream) {
     WRK_REQUIRE(paths && accum && npix > 0, "wrk_accumulate: bad argument");
     WRK_REQUIRE((uint64_t)poolBase + n <= paths->capacity, "wrk_accumulate: range exceeds the path pool");
     if (n == 0) return WRK_OK;

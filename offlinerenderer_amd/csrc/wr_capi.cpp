@@ -139,11 +139,12 @@ void* wr_c_renderer_create(void) {
     return out;
 }
 void wr_c_renderer_destroy(void* r) { delete R(r); }
-int wr_c_renderer_set_wavefront(void* r, uint32_t cap, uint32_t maxPaths, uint32_t flags) {
+int wr_c_renderer_set_wavefront(void* r, uint32_t cap, uint32_t maxPaths, uint32_t maxChunk, uint32_t flags) {
     return guard([&] {
         wr::WavefrontConfig c;
         c.iterationCap = cap;
         c.maxPathsInFlight = maxPaths;
+        c.maxChunkPaths = maxChunk;
         c.timeKernels = (flags & 1u) != 0;
         c.countTraversal = (flags & 2u) != 0;
         R(r)->setWavefrontConfig(c);

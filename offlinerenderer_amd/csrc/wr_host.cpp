@@ -362,7 +362,8 @@ struct RendererImpl {
     detail::Engine& engine(uint64_t paths) {
         detail::EngineConfig c;
         c.capacity = (uint32_t)std::max<uint64_t>(1024, std::min<uint64_t>(paths, std::max(wf.maxPathsInFlight, 1024u)));
-        c.maxChunkPaths = std::max(wf.maxPathsInFlight, 1024u);
+        c.maxChunkPaths = std::max(wf.maxChunkPaths ? std::min(wf.maxChunkPaths, wf.maxPathsInFlight) : wf.maxPathsInFlight,
+                                   1024u);
         c.timeKernels = wf.timeKernels;
         c.countTraversal = wf.countTraversal;
         if (!eng) eng = std::make_unique<detail::Engine>(c);
@@ -394,8 +395,6 @@ struct RendererImpl {
         stats = e.stats();
         stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         stats.paths = total;
-        const uint64_t chunk = std::min<uint64_t>(e.config().capacity, e.config().maxChunkPaths);
-        stats.batches = (uint32_t)((total + chunk - 1) / chunk);
     }
     void render_paths(Scene* scene, const Camera& camera, const RenderParams& rp, uint64_t firstPath, uint32_t count,
                       float* out) {

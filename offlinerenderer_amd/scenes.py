@@ -321,3 +321,41 @@ def large_scene(width: int = 3840, height: int = 2160, spp: int = 16, seed: int 
                     fovY=math.radians(55.0), aspect=width / height)
     d.numTris = ntris
     return d
+
+
+# ------------------------------------------------------------ serving load
+SERVING_RESOLUTIONS = ((256, 256), (512, 512), (1024, 768), (1920, 1080))
+SERVING_SPP = (1, 4, 16, 64)
+SERVING_SCENES = ("cornell", "material", "large")
+
+
+def serving_requests(n: int, seed: int = 7, depth: int = 8) -> list[dict]:
+    """Seeded request stream over the three resident scenes: per request a
+    scene, a random camera around it, a resolution from
+    SERVING_RESOLUTIONS and a sample count from SERVING_SPP, depth 8."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        scene = SERVING_SCENES[int(rng.integers(0, len(SERVING_SCENES)))]
+        w, h = SERVING_RESOLUTIONS[int(rng.integers(0, len(SERVING_RESOLUTIONS)))]
+        spp = SERVING_SPP[int(rng.integers(0, len(SERVING_SPP)))]
+        if scene == "cornell":
+            pos = (rng.uniform(-0.6, 0.6), rng.uniform(-0.6, 0.6), rng.uniform(2.2, 3.2))
+            tgt = (rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), 0.0)
+            fov = 45.0
+        elif scene == "material":
+            th = rng.uniform(0.0, 2.0 * math.pi)
+            rad = rng.uniform(10.0, 18.0)
+            pos = (rad * math.sin(th), rng.uniform(3.0, 9.0), rad * math.cos(th))
+            tgt = (rng.uniform(-2.0, 2.0), 0.8, rng.uniform(-2.0, 2.0))
+            fov = 40.0
+        else:
+            v = rng.normal(size=3)
+            v /= np.linalg.norm(v)
+            pos = tuple(50.0 + v * rng.uniform(60.0, 90.0))
+            tgt = tuple(50.0 + rng.normal(size=3) * 10.0)
+            fov = 55.0
+        out.append(dict(scene=scene, width=w, height=h, spp=spp, maxBounces=depth, rrDepth=3.0,
+                        camera=dict(position=tuple(float(x) for x in pos), target=tuple(float(x) for x in tgt),
+                                    up=(0.0, 1.0, 0.0), fovY=math.radians(fov), aspect=w / h)))
+    return out

@@ -56,22 +56,28 @@ def test_partition_is_stable(gpu, K, n):
 
 
 @pytest.mark.parametrize("n", [0, 1, 1000, 3000017])
-def test_slot_count_matches_histogram(gpu, n):
+def test_slot_finish_matches_histogram(gpu, n):
     rng = np.random.default_rng(1234 + n)
     cap = max(2 * n, 64)
-    nr = 37
-    firsts = np.concatenate([[0], np.sort(rng.choice(np.arange(1, cap), nr - 1, replace=False))]).astype(np.uint32)
-    slots = rng.permutation(256)[:nr].astype(np.uint32)
-    queue = rng.permutation(cap)[:n].astype(np.uint32)
-    queue[: n // 2] = np.sort(queue[: n // 2])  # long runs of one range, then scattered entries
+    # path states: slot in bits 24..31 of state.z; runs of one slot as the
+    # partition leaves them, then scattered slots
+    slot_of = rng.integers(0, 256, size=cap).astype(np.uint32)
+    slot_of[: cap // 2] = np.sort(slot_of[: cap // 2])
+    state = np.zeros((cap, 4), np.uint32)
+    state[:, 2] = (slot_of << 24) | rng.integers(0, 1 << 24, size=cap).astype(np.uint32)
+    finished = rng.permutation(cap)[:n].astype(np.uint32)
+    finished[: n // 2] = np.sort(finished[: n // 2])
     m = max(n - 5, 0)  # device-side count shorter than the launch size
-    q = G.dev(queue if n else np.zeros(1, np.uint32))
-    ranges = G.dev(np.concatenate([firsts, slots]))
+    live0 = rng.integers(1 << 20, 1 << 24, size=256).astype(np.uint32)
+    st = G.dev(state)
+    q = G.dev(finished if n else np.zeros(1, np.uint32))
     cnt = G.dev(np.array([m], np.uint32))
-    counts = G.dev(np.full(256, 7, np.uint32))
-    G.check(wr.lib().wrk_slot_count(G.vp(q), n, G.vp(cnt), G.vp(ranges), nr, G.vp(counts), G.stream()))
+    live = G.dev(live0)
+    paths = wr.wrk_paths()
+    paths.state = st.data_ptr()
+    paths.capacity = cap
+    G.check(wr.lib().wrk_slot_finish(G.vp(q), n, G.vp(cnt), C.byref(paths), G.vp(live), G.stream()))
     torch.cuda.synchronize()
-    got = counts.cpu().numpy().view(np.uint32)
-    r = np.searchsorted(firsts, queue[:m], side="right") - 1
-    want = np.bincount(slots[r], minlength=256).astype(np.uint32)
+    got = live.cpu().numpy().view(np.uint32)
+    want = live0 - np.bincount(slot_of[finished[:m]], minlength=256).astype(np.uint32)
     assert np.array_equal(got, want)

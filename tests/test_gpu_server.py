@@ -112,3 +112,18 @@ def test_server_rejects_bad_requests(world):
     with pytest.raises(RuntimeError):
         srv.submit(sc, wr.Camera(**dc.camera), wr.RenderParams(width=0, height=4, spp=1))
     assert not srv.step()
+
+
+def test_renderer_small_chunks_match_one_batch(world):
+    dm, sm = world["material"]
+    cam, prm = wr.Camera(**dm.camera), G.params(dm)
+    one = wr.Renderer()
+    one.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=1 << 20))
+    a = one.render(sm, cam, prm)
+    # several chunks in flight at once; a chunk that finishes before its
+    # predecessor waits so the per-pixel sum keeps the sample order
+    many = wr.Renderer()
+    many.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=6000, maxChunkPaths=1500))
+    b = many.render(sm, cam, prm)
+    assert _same(a, b)
+    assert many.getStats()["batches"] >= (dm.width * dm.height * dm.spp) // 1500
