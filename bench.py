@@ -67,6 +67,8 @@ def main():
                     "8M for serving)")
     ap.add_argument("--requests", type=int, default=256, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrival rate, requests/s")
+    ap.add_argument("--policy", choices=["fifo", "shortest"], default="shortest",
+                    help="serving: admission order of pending requests")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target wall time of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -276,7 +278,8 @@ def serving(args):
     serial_lat = finish - arrivals
 
     def run_server(arrive):
-        srv = wr.RenderServer(wr.ServerConfig(maxPathsInFlight=pool, maxChunkPaths=chunk))
+        srv = wr.RenderServer(wr.ServerConfig(maxPathsInFlight=pool, maxChunkPaths=chunk,
+                                              shortestFirst=args.policy == "shortest"))
         ids, done_t, equal = [None] * len(jobs), np.zeros(len(jobs)), 0
         open_ids = {}
         t0 = srv.clock()
@@ -292,12 +295,10 @@ def serving(args):
                 continue
             srv.step()
             steps += 1
-            for rid, i in list(open_ids.items()):
-                st = srv.status(rid)
-                if st["done"]:
-                    done_t[i] = st["doneTime"] - t0
-                    equal += digest(srv.fetch(rid)) == ref[i]
-                    del open_ids[rid]
+            for rid in srv.takeCompleted():
+                i = open_ids.pop(rid)
+                done_t[i] = srv.status(rid)["doneTime"] - t0
+                equal += digest(srv.fetch(rid)) == ref[i]
         stats = srv.getStats()
         return done_t, equal, steps, stats
 
@@ -334,7 +335,7 @@ def serving(args):
             "workload": f"serving_{len(jobs)}req_poisson{args.rate:g}",
             "requests": len(jobs), "total_paths": int(total_paths), "resolutions": list(scenes.SERVING_RESOLUTIONS),
             "spp": list(scenes.SERVING_SPP), "max_depth": 8, "arrival_rate_per_s": args.rate,
-            "paths_in_flight": pool, "chunk_paths": chunk,
+            "paths_in_flight": pool, "chunk_paths": chunk, "admission": args.policy,
         },
         "serial_mpaths_per_s": round(serial_mpaths, 3),
         "speedup_vs_serial": round(server_mpaths / serial_mpaths, 3),

@@ -3,6 +3,7 @@
 // Additive: WavefrontConfig (iteration cap, paths in flight) and
 // Renderer::renderPaths / getStats for tests and benchmarks.
 #include <cstdint>
+#include <vector>
 
 #include "wr/scene.h"
 #include "wr/types.h"
@@ -103,6 +104,9 @@ struct ServerConfig {
     uint64_t memoryBudgetBytes = 0;        // accumulation + image buffers of admitted requests; 0 = unlimited
     uint32_t iterationCap = 0;             // as WavefrontConfig::iterationCap
     bool timeKernels = false;
+    // admission order of pending requests: false = arrival order, true =
+    // fewest remaining paths first (short requests pass long ones)
+    bool shortestFirst = false;
 };
 
 struct RequestStatus {
@@ -124,6 +128,9 @@ public:
     bool step();
     bool isDone(uint64_t id) const;
     RequestStatus status(uint64_t id) const;
+    // Up to maxIds ids of requests that finished since they were last taken,
+    // in completion order; returns how many were written.
+    uint32_t takeCompleted(uint64_t* ids, uint32_t maxIds);
     // Averaged image (width*height Vec3) of a finished request; releases it.
     void fetch(uint64_t id, Vec3* out);
     uint32_t numPending() const;

@@ -77,7 +77,7 @@ int wr_c_renderer_stats(void* renderer, wr_c_stats* out);
 typedef struct wr_c_server_config {
     uint32_t maxPathsInFlight, maxChunkPaths;
     uint64_t memoryBudgetBytes;
-    uint32_t iterationCap, flags; /* flags bit0: time kernels */
+    uint32_t iterationCap, flags; /* flags bit0: time kernels, bit1: shortest request first */
 } wr_c_server_config;
 
 typedef struct wr_c_request_status {
@@ -93,6 +93,8 @@ uint64_t wr_c_server_submit(void* server, void* scene, const wr_c_camera* cam, c
 /* 1 = work remains, 0 = idle, -1 = error */
 int wr_c_server_step(void* server);
 int wr_c_server_status(void* server, uint64_t id, wr_c_request_status* out);
+/* RenderServer::takeCompleted: writes up to maxIds finished request ids, returns the count (-1 on error) */
+int wr_c_server_take_completed(void* server, uint64_t* ids, uint32_t maxIds);
 int wr_c_server_fetch(void* server, uint64_t id, float* out);      /* width*height*3 floats */
 int wr_c_server_stats(void* server, wr_c_stats* out);
 uint32_t wr_c_server_pending(void* server);

@@ -122,6 +122,7 @@ _SIGS = {
     "wr_c_server_step": (C.c_int, [_P]),
     "wr_c_server_status": (C.c_int, [_P, C.c_uint64, C.POINTER(wr_c_request_status)]),
     "wr_c_server_fetch": (C.c_int, [_P, C.c_uint64, _P]),
+    "wr_c_server_take_completed": (C.c_int, [_P, _P, C.c_uint32]),
     "wr_c_server_stats": (C.c_int, [_P, C.POINTER(wr_c_stats)]),
     "wr_c_server_pending": (C.c_uint32, [_P]),
     "wr_c_server_in_flight": (C.c_uint32, [_P]),
@@ -424,6 +425,7 @@ class ServerConfig:
     memoryBudgetBytes: int = 0
     iterationCap: int = 0
     timeKernels: bool = False
+    shortestFirst: bool = False
 
 
 class RenderServer:
@@ -434,7 +436,7 @@ class RenderServer:
     def __init__(self, cfg: ServerConfig | None = None):
         c = cfg or ServerConfig()
         cc = wr_c_server_config(int(c.maxPathsInFlight), int(c.maxChunkPaths), int(c.memoryBudgetBytes),
-                                int(c.iterationCap), 1 if c.timeKernels else 0)
+                                int(c.iterationCap), (1 if c.timeKernels else 0) | (2 if c.shortestFirst else 0))
         self._h = lib().wr_c_server_create(C.byref(cc))
         if not self._h:
             raise RuntimeError(f"RenderServer: {_err()}")
@@ -464,6 +466,14 @@ class RenderServer:
         st = wr_c_request_status()
         _check(lib().wr_c_server_status(self._h, rid, C.byref(st)), "status")
         return {name: getattr(st, name) for name, _ in wr_c_request_status._fields_}
+
+    def takeCompleted(self, max_ids: int = 1024) -> list[int]:
+        """Ids of requests finished since they were last taken, in completion order."""
+        buf = (C.c_uint64 * max_ids)()
+        n = lib().wr_c_server_take_completed(self._h, buf, max_ids)
+        if n < 0:
+            raise RuntimeError(f"takeCompleted: {_err()}")
+        return list(buf[:n])
 
     def fetch(self, rid: int) -> np.ndarray:
         h, w = self._shapes.pop(rid)

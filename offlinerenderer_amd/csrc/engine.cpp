@@ -62,6 +62,11 @@ enum TimedCat { T_CLOSEST, T_ANY, T_CLASSIFY, T_PARTITION, T_SHADE, T_GENERATE, 
 
 Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    for (uint32_t s = 0; s < kMaxScenes; ++s) {
+        hip_check(hipStreamCreateWithFlags(&sceneStream_[s], hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&sceneDone_[s], hipEventDisableTiming), "hipEventCreate");
+    }
+    hip_check(hipEventCreateWithFlags(&stepStart_, hipEventDisableTiming), "hipEventCreate");
     hip_check(hipHostMalloc((void**)&hostOffs_, 64 * sizeof(uint32_t), hipHostMallocDefault), "hipHostMalloc");
     hip_check(hipHostMalloc((void**)&hostLive_, WRK_MAX_SLOTS * sizeof(uint32_t), hipHostMallocDefault),
               "hipHostMalloc");
@@ -84,6 +89,11 @@ Engine::~Engine() {
     if (hostLive_) (void)hipHostFree(hostLive_);
     if (hostPrm_) (void)hipHostFree(hostPrm_);
     if (hostLiveInit_) (void)hipHostFree(hostLiveInit_);
+    for (uint32_t s = 0; s < kMaxScenes; ++s) {
+        if (sceneDone_[s]) (void)hipEventDestroy(sceneDone_[s]);
+        if (sceneStream_[s]) (void)hipStreamDestroy(sceneStream_[s]);
+    }
+    if (stepStart_) (void)hipEventDestroy(stepStart_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -104,6 +114,7 @@ void Engine::reserve(uint32_t cap) {
     occl_.alloc(c); result_.alloc(c * 16);
     qA_.alloc(c * 4); qB_.alloc(c * 4); cat_.alloc(c);
     scratch_.alloc(wrk_partition_scratch_bytes(cap, kFinalCategories));
+    for (uint32_t s = 0; s < kMaxScenes; ++s) sceneScratch_[s].alloc(wrk_partition_scratch_bytes(cap, 6));
     capacity_ = cap;
     paths_.org = org_.addr(); paths_.dir = dir_.addr(); paths_.thr = thr_.addr(); paths_.rad = rad_.addr();
     paths_.state = st_.addr(); paths_.contrib = contrib_.addr(); paths_.savedOrg = savedOrg_.addr();
@@ -228,7 +239,7 @@ void Engine::fetch(uint64_t id, float* out) {
     release(id);
 }
 
-template <typename F> void Engine::timed(int category, F&& launch) {
+template <typename F> void Engine::timed(int category, hipStream_t stream, F&& launch) {
     if (!cfg_.timeKernels) {
         launch();
         return;
@@ -239,9 +250,9 @@ template <typename F> void Engine::timed(int category, F&& launch) {
         eventPool_.push_back(e);
     }
     hipEvent_t a = eventPool_[eventNext_++], b = eventPool_[eventNext_++];
-    hip_check(hipEventRecord(a, stream_), "hipEventRecord");
+    hip_check(hipEventRecord(a, stream), "hipEventRecord");
     launch();
-    hip_check(hipEventRecord(b, stream_), "hipEventRecord");
+    hip_check(hipEventRecord(b, stream), "hipEventRecord");
     marks_.push_back({category, {a, b}});
 }
 
@@ -274,7 +285,7 @@ void Engine::complete(uint32_t s) {
         while (!r.held.empty() && r.held.begin()->first == r.accumNext) {
             const uint32_t hs = r.held.begin()->second;
             Slot& h = slots_[hs];
-            timed(T_ACCUM, [&] {
+            timed(T_ACCUM, stream_, [&] {
                 wrk_check(wrk_accumulate(&paths_, h.poolBase, h.firstPath, h.count, r.npix, r.accum.as<float>(),
                                          stream_),
                           "wrk_accumulate");
@@ -289,6 +300,7 @@ void Engine::complete(uint32_t s) {
             wrk_check(wrk_average(r.accum.as<float>(), r.npix, r.spp, r.image.as<float>(), stream_), "wrk_average");
         r.done = true;
         r.tDone = -1;  // stamped after the step's synchronisation
+        completed_.push_back(r.id);
     }
 }
 
@@ -345,7 +357,7 @@ bool Engine::admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase) {
                              stream_),
               "hipMemcpyAsync");
     const uint32_t at = queueBase + *added;
-    timed(T_GENERATE, [&] {
+    timed(T_GENERATE, stream_, [&] {
         wrk_check(wrk_generate(&r.cam, &r.prm, &paths_, r.nextPath, n, base, s, qB_.as<uint32_t>() + at,
                                cat_.as<uint8_t>() + at, 1 + 3 * r.sceneIdx, stream_),
                   "wrk_generate");
@@ -374,6 +386,7 @@ bool Engine::step() {
     // part is [trace | trace + shadow | shadow]: the closest-hit launch takes
     // the first two pieces, the any-hit launch (shadow rays in savedOrg /
     // savedDir) the last two.
+    hip_check(hipEventRecord(stepStart_, stream_), "hipEventRecord");
     uint32_t total = 0;
     uint32_t* const qlive = qa + deadCount_;  // qa = [finished last iteration | live parts]
     for (uint32_t s = 0; s < S; ++s) {
@@ -381,51 +394,55 @@ bool Engine::step() {
         const uint32_t a = total, n = nT + nTS + nS;
         total += n;
         if (n == 0) continue;
+        hipStream_t ss = sceneStream_[s];
+        hip_check(hipStreamWaitEvent(ss, stepStart_, 0), "hipStreamWaitEvent");
         const wrk_bvh& bvh = scenes_[s]->getBVH();
         const wrk_scene_view& view = scenes_[s]->getDeviceView();
         const uint32_t nC = nT + nTS, nA = nTS + nS;
         if (nC) {
-            timed(T_CLOSEST, [&] {
+            timed(T_CLOSEST, ss, [&] {
                 if (cfg_.countTraversal)
                     wrk_check(wrk_trace_closest_counted(&bvh, org_.ptr, dir_.ptr, qlive + a, nC, nullptr,
-                                                        hit_.as<uint32_t>(), fc, stream_),
+                                                        hit_.as<uint32_t>(), fc, ss),
                               "wrk_trace_closest_counted");
                 else
                     wrk_check(wrk_trace_closest(&bvh, org_.ptr, dir_.ptr, qlive + a, nC, nullptr, hit_.as<uint32_t>(),
-                                                stream_),
+                                                ss),
                               "wrk_trace_closest");
             });
             stats_.launchesTraceClosest++;
         }
         if (nA) {
-            timed(T_ANY, [&] {
+            timed(T_ANY, ss, [&] {
                 if (cfg_.countTraversal)
                     wrk_check(wrk_trace_any_counted(&bvh, savedOrg_.ptr, savedDir_.ptr, qlive + a + nT, nA, nullptr,
-                                                    occl_.as<uint8_t>(), fc + 2, stream_),
+                                                    occl_.as<uint8_t>(), fc + 2, ss),
                               "wrk_trace_any_counted");
                 else
                     wrk_check(wrk_trace_any(&bvh, savedOrg_.ptr, savedDir_.ptr, qlive + a + nT, nA, nullptr,
-                                            occl_.as<uint8_t>(), stream_),
+                                            occl_.as<uint8_t>(), ss),
                               "wrk_trace_any");
             });
             stats_.launchesTraceAny++;
         }
         stats_.closestRays += nC;
         stats_.shadowRays += nA;
-        timed(T_CLASSIFY, [&] {
-            wrk_check(wrk_classify(&view, prmTable, &paths_, qlive + a, nT, nTS, nS, cat + a, stream_),
+        timed(T_CLASSIFY, ss, [&] {
+            wrk_check(wrk_classify(&view, prmTable, &paths_, qlive + a, nT, nTS, nS, cat + a, ss),
                       "wrk_classify");
         });
-        timed(T_PARTITION, [&] {
+        timed(T_PARTITION, ss, [&] {
             // five material queues + the paths classify finished
-            wrk_check(wrk_partition(cat + a, qlive + a, n, 6, qb + a, off + 8 * s, scratch_.ptr, scratch_.bytes,
-                                    stream_),
+            wrk_check(wrk_partition(cat + a, qlive + a, n, 6, qb + a, off + 8 * s, sceneScratch_[s].ptr,
+                                    sceneScratch_[s].bytes, ss),
                       "wrk_partition");
         });
-        timed(T_SHADE, [&] {
-            wrk_check(wrk_shade(&view, prmTable, &paths_, qb + a, off + 8 * s, n, 1 + 3 * s, 0, cat + a, stream_),
+        timed(T_SHADE, ss, [&] {
+            wrk_check(wrk_shade(&view, prmTable, &paths_, qb + a, off + 8 * s, n, 1 + 3 * s, 0, cat + a, ss),
                       "wrk_shade");
         });
+        hip_check(hipEventRecord(sceneDone_[s], ss), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(stream_, sceneDone_[s], 0), "hipStreamWaitEvent");
     }
     if (total) stats_.iterations++;
 
@@ -434,6 +451,13 @@ bool Engine::step() {
     // or pool ranges run out; a request that does not fit the memory budget
     // stops admission of the requests behind it.
     uint32_t added = 0;
+    if (cfg_.shortestFirst && pending_.size() > 1) {
+        std::stable_sort(pending_.begin(), pending_.end(), [&](uint64_t x, uint64_t y) {
+            const RequestState& a = *requests_.at(x);
+            const RequestState& b = *requests_.at(y);
+            return a.endPath - a.nextPath < b.endPath - b.nextPath;
+        });
+    }
     for (bool progress = true; progress && !pending_.empty();) {
         progress = false;
         for (auto it = pending_.begin(); it != pending_.end();) {
@@ -462,14 +486,14 @@ bool Engine::step() {
     const uint32_t K = 1 + 3 * S2;
     const uint32_t items = total + added;
     if (items) {
-        timed(T_PARTITION, [&] {
+        timed(T_PARTITION, stream_, [&] {
             wrk_check(wrk_partition(cat, qb, items, K, qa, off + 48, scratch_.ptr, scratch_.bytes, stream_),
                       "wrk_partition");
         });
         hip_check(hipMemcpyAsync(hostOffs_, off + 48, (K + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
                   "hipMemcpyAsync");
         // live paths per slot: subtract the paths that ended
-        timed(T_PARTITION, [&] {
+        timed(T_PARTITION, stream_, [&] {
             wrk_check(wrk_slot_finish(qa, items, off + 48 + 1, &paths_, live, stream_), "wrk_slot_finish");
         });
     }

@@ -10,11 +10,13 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdint>
 #include <deque>
 #include <map>
 #include <memory>
+#include <utility>
 #include <vector>
 
 #include "../../include/wr/wr.h"
@@ -55,6 +57,7 @@ struct EngineConfig {
     uint64_t memoryBudgetBytes = 0;      // 0 = unlimited (beyond the pool itself)
     bool timeKernels = false;
     bool countTraversal = false;
+    bool shortestFirst = false;          // admission order: fewest remaining paths first
 };
 
 class Engine {
@@ -101,6 +104,12 @@ public:
     bool step();
     bool isDone(uint64_t id) const;
     const RequestState& request(uint64_t id) const;
+    size_t takeCompleted(uint64_t* ids, size_t maxIds) {
+        const size_t n = std::min(maxIds, completed_.size());
+        std::copy(completed_.begin(), completed_.begin() + n, ids);
+        completed_.erase(completed_.begin(), completed_.begin() + n);
+        return n;
+    }
     // Copies the averaged image (W*H*3 floats) of a finished Accumulate
     // request to host memory and releases the request.
     void fetch(uint64_t id, float* out);
@@ -126,11 +135,16 @@ private:
     void complete(uint32_t slot);
     void retire(uint32_t slot);
     bool admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase);
-    template <typename F> void timed(int category, F&& launch);
+    template <typename F> void timed(int category, hipStream_t stream, F&& launch);
 
     EngineConfig cfg_;
     RenderStats stats_;
     hipStream_t stream_ = nullptr;
+    // one stream per resident scene: the scenes' trace/classify/shade chains
+    // of an iteration touch disjoint queue parts and run concurrently
+    hipStream_t sceneStream_[4] = {};
+    hipEvent_t stepStart_ = nullptr, sceneDone_[4] = {};
+    DevBuf sceneScratch_[4];
     uint32_t capacity_ = 0;
     DevBuf org_, dir_, thr_, rad_, st_, contrib_, savedOrg_, savedDir_, hit_, occl_, result_;
     DevBuf qA_, qB_, cat_, scratch_, offs_, prmTable_, live_, fetch_;
@@ -147,6 +161,7 @@ private:
     Slot slots_[WRK_MAX_SLOTS];
     std::vector<uint32_t> freeSlots_;
     std::deque<uint64_t> pending_;
+    std::deque<uint64_t> completed_;      // finished, not yet taken by takeCompleted()
     std::map<uint64_t, std::unique_ptr<RequestState>> requests_;
     uint64_t nextId_ = 1;
     uint64_t footprintUsed_ = 0;

@@ -170,6 +170,7 @@ void* wr_c_server_create(const wr_c_server_config* cfg) {
         c.memoryBudgetBytes = cfg->memoryBudgetBytes;
         c.iterationCap = cfg->iterationCap;
         c.timeKernels = (cfg->flags & 1u) != 0;
+        c.shortestFirst = (cfg->flags & 2u) != 0;
         out = new wr::RenderServer(c);
     });
     return out;
@@ -195,6 +196,11 @@ int wr_c_server_status(void* sv, uint64_t id, wr_c_request_status* out) {
         out->doneTime = st.doneTime;
         out->paths = st.paths;
     });
+}
+int wr_c_server_take_completed(void* sv, uint64_t* ids, uint32_t maxIds) {
+    int n = 0;
+    if (guard([&] { n = (int)SV(sv)->takeCompleted(ids, maxIds); }) != 0) return -1;
+    return n;
 }
 int wr_c_server_fetch(void* sv, uint64_t id, float* out) {
     return guard([&] { SV(sv)->fetch(id, reinterpret_cast<wr::Vec3*>(out)); });

@@ -462,6 +462,7 @@ struct ServerImpl {
         e.maxChunkPaths = c.maxChunkPaths;
         e.memoryBudgetBytes = c.memoryBudgetBytes;
         e.timeKernels = c.timeKernels;
+        e.shortestFirst = c.shortestFirst;
         eng = std::make_unique<detail::Engine>(e);
         eng->ensureCapacity(e.capacity);
     }
@@ -483,6 +484,9 @@ uint64_t RenderServer::submit(Scene* scene, const Camera& camera, const RenderPa
 }
 bool RenderServer::step() { return m_impl->eng->step(); }
 bool RenderServer::isDone(uint64_t id) const { return m_impl->eng->isDone(id); }
+uint32_t RenderServer::takeCompleted(uint64_t* ids, uint32_t maxIds) {
+    return (uint32_t)m_impl->eng->takeCompleted(ids, maxIds);
+}
 RequestStatus RenderServer::status(uint64_t id) const {
     const auto& r = m_impl->eng->request(id);
     RequestStatus s;

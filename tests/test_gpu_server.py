@@ -57,9 +57,10 @@ def _serve(reqs, cfg, stagger):
         pending = pending[take:]
         srv.step()
         steps += 1
-        for rid in ids:
-            if rid not in out and srv.status(rid)["done"]:
-                out[rid] = srv.fetch(rid)
+        done = srv.takeCompleted()
+        assert all(srv.status(rid)["done"] for rid in done)
+        for rid in done:
+            out[rid] = srv.fetch(rid)
         assert steps < 10000
     return [out[rid] for rid in ids], srv
 
@@ -68,10 +69,11 @@ def _same(a, b):
     return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-def test_mixed_requests_match_alone(world):
+@pytest.mark.parametrize("shortest_first", [False, True])
+def test_mixed_requests_match_alone(world, shortest_first):
     reqs = _requests(world)
     ref = _alone(reqs, cap=0)
-    got, srv = _serve(reqs, wr.ServerConfig(maxPathsInFlight=1 << 20), stagger=True)
+    got, srv = _serve(reqs, wr.ServerConfig(maxPathsInFlight=1 << 20, shortestFirst=shortest_first), stagger=True)
     for i, (a, b) in enumerate(zip(got, ref)):
         assert _same(a, b), f"request {i} differs from its stand-alone render"
     assert srv.numPending() == 0 and srv.numInFlight() == 0
