@@ -85,6 +85,11 @@ public:
     // out holds 4 floats per path (rgb, unused).
     void renderPaths(Scene* scene, const Camera& camera, const RenderParams& params, uint64_t firstPath,
                      uint32_t count, float* out);
+    // The averaged image plus the denoiser guide layers of libWR
+    // (trace.cu:267-286): first-hit albedo and ray-facing geometric normal,
+    // averaged over spp; each buffer holds width*height Vec3.
+    void renderWithGuides(Scene* scene, const Camera& camera, const RenderParams& params, Vec3* color,
+                          Vec3* albedo, Vec3* normal);
     const RenderStats& getStats() const;
 
 private:

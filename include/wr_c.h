@@ -72,6 +72,9 @@ int wr_c_render(void* renderer, void* scene, const wr_c_camera* cam, const wr_c_
 int wr_c_render_paths(void* renderer, void* scene, const wr_c_camera* cam, const wr_c_render_params* params,
                       uint64_t firstPath, uint32_t count, float* out);
 int wr_c_renderer_stats(void* renderer, wr_c_stats* out);
+/* Renderer::renderWithGuides; each output width*height*3 floats */
+int wr_c_render_guides(void* renderer, void* scene, const wr_c_camera* cam, const wr_c_render_params* params,
+                       float* color, float* albedo, float* normal);
 
 /* --- continuous-batching render server (wr::RenderServer) --- */
 typedef struct wr_c_server_config {

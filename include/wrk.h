@@ -138,6 +138,7 @@ typedef struct wrk_paths {
     uint64_t hit;      /* uint2: t (float bits), primitive id (0xFFFFFFFF = miss) */
     uint64_t occluded; /* uint8 */
     uint64_t result;   /* float4: value added to the accumulation buffer */
+    uint64_t guides;   /* optional (0 = none), 2 x float4 per path: first-hit albedo.xyz, normal.xyz */
     uint32_t capacity;
     uint32_t _pad;
 } wrk_paths;
@@ -244,9 +245,13 @@ int wrk_shade(const wrk_scene_view* scene, const wrk_render_params* paramsTable,
 int wrk_slot_finish(const uint32_t* finished, uint32_t maxItems, const uint32_t* countDev, const wrk_paths* paths,
                     uint32_t* live, void* stream);
 /* accum[pixel] += result[poolBase + k] for the chunk's paths in sample order
- * (renderer.cpp:190 serial samples; per-pixel accumulation order is fixed). */
+ * (renderer.cpp:190 serial samples; per-pixel accumulation order is fixed).
+ * With paths->guides and non-NULL accumAlbedo/accumNormal the first-hit
+ * guides are summed the same way (trace.cu:267-286: first-hit albedo and
+ * ray-facing geometric normal, summed over samples; paths that miss or end
+ * before their first shade add zero). */
 int wrk_accumulate(const wrk_paths* paths, uint32_t poolBase, uint64_t first, uint32_t n, uint32_t npix,
-                   float* accum, void* stream);
+                   float* accum, float* accumAlbedo, float* accumNormal, void* stream);
 /* out = accum * (1/spp) (renderer.cpp:308-313). */
 int wrk_average(const float* accum, uint32_t npix, uint32_t spp, float* out, void* stream);
 /* Exposure 2.2, ACES (Narkowicz), gamma 1/2.2, truncation to uint8

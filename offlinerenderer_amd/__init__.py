@@ -71,7 +71,7 @@ class wr_c_stats(C.Structure):
 class wrk_paths(C.Structure):
     """include/wrk.h wrk_paths: device addresses of the path pool arrays."""
     _fields_ = [(name, C.c_uint64) for name in ("org", "dir", "thr", "rad", "state", "contrib", "savedOrg",
-                                                 "savedDir", "hit", "occluded", "result")] + \
+                                                 "savedDir", "hit", "occluded", "result", "guides")] + \
                [("capacity", C.c_uint32), ("_pad", C.c_uint32)]
 
 
@@ -116,6 +116,7 @@ _SIGS = {
     "wr_c_render_paths": (C.c_int, [_P, _P, C.POINTER(wr_c_camera), C.POINTER(wr_c_render_params), C.c_uint64,
                                     C.c_uint32, _P]),
     "wr_c_renderer_stats": (C.c_int, [_P, C.POINTER(wr_c_stats)]),
+    "wr_c_render_guides": (C.c_int, [_P, _P, C.POINTER(wr_c_camera), C.POINTER(wr_c_render_params), _P, _P, _P]),
     "wr_c_server_create": (_P, [C.POINTER(wr_c_server_config)]),
     "wr_c_server_destroy": (None, [_P]),
     "wr_c_server_submit": (C.c_uint64, [_P, _P, C.POINTER(wr_c_camera), C.POINTER(wr_c_render_params)]),
@@ -146,7 +147,7 @@ _SIGS = {
     "wrk_classify": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_shade": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_slot_finish": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
-    "wrk_accumulate": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P]),
+    "wrk_accumulate": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P, _P, _P]),
     "wrk_average": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_tonemap_aces": (C.c_int, [_P, C.c_uint32, _P, _P]),
 }
@@ -401,6 +402,15 @@ class Renderer:
         cam, prm = camera.to_c(), params.to_c()
         _check(lib().wr_c_render(self._h, scene._h, C.byref(cam), C.byref(prm), _ptr(out)), "render")
         return out
+
+    def renderWithGuides(self, scene: Scene, camera: Camera, params: RenderParams):
+        """(color, albedo, normal), each (height, width, 3) float32 averaged over spp."""
+        shape = (params.height, params.width, 3)
+        col, alb, nrm = (np.zeros(shape, dtype=np.float32) for _ in range(3))
+        cam, prm = camera.to_c(), params.to_c()
+        _check(lib().wr_c_render_guides(self._h, scene._h, C.byref(cam), C.byref(prm), _ptr(col), _ptr(alb),
+                                        _ptr(nrm)), "renderWithGuides")
+        return col, alb, nrm
 
     def renderPaths(self, scene: Scene, camera: Camera, params: RenderParams, firstPath: int, count: int) -> np.ndarray:
         """Per-path values added to the accumulation buffer, shape (count, 3)."""

@@ -120,6 +120,10 @@ void Engine::reserve(uint32_t cap) {
     paths_.state = st_.addr(); paths_.contrib = contrib_.addr(); paths_.savedOrg = savedOrg_.addr();
     paths_.savedDir = savedDir_.addr(); paths_.hit = hit_.addr(); paths_.occluded = occl_.addr();
     paths_.result = result_.addr(); paths_.capacity = cap;
+    if (guides_.ptr) {
+        guides_.alloc(c * 32);
+        paths_.guides = guides_.addr();
+    }
     deadCount_ = 0;
     free_.clear();
     free_[0] = cap;
@@ -183,7 +187,7 @@ uint32_t Engine::largestFree() const {
 }
 
 uint64_t Engine::submit(Scene* scene, const wrk_camera& cam, const wrk_render_params& prm, uint32_t spp,
-                        uint64_t firstPath, uint64_t numPaths, Mode mode, float* pathsOut) {
+                        uint64_t firstPath, uint64_t numPaths, Mode mode, float* pathsOut, bool guides) {
     if (!scene) throw std::runtime_error("render: null scene");
     if (prm.width == 0 || prm.height == 0) throw std::runtime_error("render: empty image");
     if (numPaths == 0) throw std::runtime_error("render: no paths requested");
@@ -203,7 +207,8 @@ uint64_t Engine::submit(Scene* scene, const wrk_camera& cam, const wrk_render_pa
     r->npix = prm.width * prm.height;
     r->pathsOut = pathsOut;
     r->tSubmit = now();
-    r->footprint = mode == Mode::Accumulate ? (uint64_t)r->npix * 24u : 0u;
+    r->guides = guides && mode == Mode::Accumulate;
+    r->footprint = mode == Mode::Accumulate ? (uint64_t)r->npix * (r->guides ? 72u : 24u) : 0u;
     uint64_t id = r->id;
     requests_[id] = std::move(r);
     pending_.push_back(id);
@@ -230,13 +235,32 @@ void Engine::release(uint64_t id) {
     requests_.erase(it);
 }
 
-void Engine::fetch(uint64_t id, float* out) {
+void Engine::fetch(uint64_t id, float* out, float* albedo, float* normal) {
     RequestState& r = *requests_.at(id);
     if (!r.done) throw std::runtime_error("request still in flight");
     if (r.mode != Mode::Accumulate) throw std::runtime_error("request has no image");
-    hip_check(hipMemcpy(out, r.image.ptr, (size_t)r.npix * 3 * sizeof(float), hipMemcpyDeviceToHost),
-              "hipMemcpy D2H");
+    if ((albedo || normal) && !r.guides) throw std::runtime_error("request was submitted without guides");
+    const size_t bytes = (size_t)r.npix * 3 * sizeof(float);
+    if (out) hip_check(hipMemcpy(out, r.image.ptr, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    if (albedo) hip_check(hipMemcpy(albedo, r.imgAlb.ptr, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    if (normal) hip_check(hipMemcpy(normal, r.imgNrm.ptr, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
     release(id);
+}
+
+const float* Engine::deviceImage(uint64_t id) const {
+    const RequestState& r = request(id);
+    if (!r.done || r.mode != Mode::Accumulate) throw std::runtime_error("request has no finished image");
+    return r.image.as<float>();
+}
+const float* Engine::deviceAlbedo(uint64_t id) const {
+    const RequestState& r = request(id);
+    if (!r.done || !r.guides) throw std::runtime_error("request has no finished guides");
+    return r.imgAlb.as<float>();
+}
+const float* Engine::deviceNormal(uint64_t id) const {
+    const RequestState& r = request(id);
+    if (!r.done || !r.guides) throw std::runtime_error("request has no finished guides");
+    return r.imgNrm.as<float>();
 }
 
 template <typename F> void Engine::timed(int category, hipStream_t stream, F&& launch) {
@@ -287,7 +311,8 @@ void Engine::complete(uint32_t s) {
             Slot& h = slots_[hs];
             timed(T_ACCUM, stream_, [&] {
                 wrk_check(wrk_accumulate(&paths_, h.poolBase, h.firstPath, h.count, r.npix, r.accum.as<float>(),
-                                         stream_),
+                                         r.guides ? r.accAlb.as<float>() : nullptr,
+                                         r.guides ? r.accNrm.as<float>() : nullptr, stream_),
                           "wrk_accumulate");
             });
             r.accumNext += h.count;
@@ -298,6 +323,10 @@ void Engine::complete(uint32_t s) {
     if (r.finishedPaths == r.endPath - r.firstPath) {
         if (r.mode == Mode::Accumulate)
             wrk_check(wrk_average(r.accum.as<float>(), r.npix, r.spp, r.image.as<float>(), stream_), "wrk_average");
+        if (r.guides) {
+            wrk_check(wrk_average(r.accAlb.as<float>(), r.npix, r.spp, r.imgAlb.as<float>(), stream_), "wrk_average");
+            wrk_check(wrk_average(r.accNrm.as<float>(), r.npix, r.spp, r.imgNrm.as<float>(), stream_), "wrk_average");
+        }
         r.done = true;
         r.tDone = -1;  // stamped after the step's synchronisation
         completed_.push_back(r.id);
@@ -332,6 +361,17 @@ bool Engine::admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase) {
             r.accum.alloc((size_t)r.npix * 3 * sizeof(float));
             r.image.alloc((size_t)r.npix * 3 * sizeof(float));
             hip_check(hipMemsetAsync(r.accum.ptr, 0, (size_t)r.npix * 3 * sizeof(float), stream_), "hipMemsetAsync");
+        }
+        if (r.guides) {
+            const size_t bytes = (size_t)r.npix * 3 * sizeof(float);
+            r.accAlb.alloc(bytes); r.accNrm.alloc(bytes); r.imgAlb.alloc(bytes); r.imgNrm.alloc(bytes);
+            hip_check(hipMemsetAsync(r.accAlb.ptr, 0, bytes, stream_), "hipMemsetAsync");
+            hip_check(hipMemsetAsync(r.accNrm.ptr, 0, bytes, stream_), "hipMemsetAsync");
+            if (!guides_.ptr) {
+                // first request with guides: per-path guide records from now on
+                guides_.alloc((size_t)capacity_ * 32);
+                paths_.guides = guides_.addr();
+            }
         }
         r.admitted = true;
         r.tAdmit = now();

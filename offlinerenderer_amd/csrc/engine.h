@@ -74,6 +74,8 @@ public:
         uint32_t spp = 1, npix = 0;
         float* pathsOut = nullptr;  // host, Mode::Paths
         DevBuf accum, image;
+        bool guides = false;        // first-hit albedo / normal sums (trace.cu:267-286)
+        DevBuf accAlb, accNrm, imgAlb, imgNrm;
         // chunks finish in any order; their results enter the accumulation
         // buffer in sample order: accumNext is the first path not yet added,
         // held maps the first path of each finished chunk waiting for its
@@ -98,7 +100,7 @@ public:
     // Queues a request for paths [firstPath, firstPath + numPaths) of the
     // image described by params (numPaths = W*H*spp for a whole render).
     uint64_t submit(Scene* scene, const wrk_camera& cam, const wrk_render_params& prm, uint32_t spp,
-                    uint64_t firstPath, uint64_t numPaths, Mode mode, float* pathsOut);
+                    uint64_t firstPath, uint64_t numPaths, Mode mode, float* pathsOut, bool guides = false);
     // Admission + one wavefront iteration + completions. Returns true while
     // any request is pending or in flight.
     bool step();
@@ -112,7 +114,15 @@ public:
     }
     // Copies the averaged image (W*H*3 floats) of a finished Accumulate
     // request to host memory and releases the request.
-    void fetch(uint64_t id, float* out);
+    // albedo / normal (W*H*3 floats each, averaged over spp) when the
+    // request was submitted with guides; NULL skips them.
+    void fetch(uint64_t id, float* out, float* albedo = nullptr, float* normal = nullptr);
+    // The averaged images of a finished request, on the device (valid until
+    // fetch or release).
+    const float* deviceImage(uint64_t id) const;
+    const float* deviceAlbedo(uint64_t id) const;
+    const float* deviceNormal(uint64_t id) const;
+    hipStream_t stream() const { return stream_; }
     void release(uint64_t id);
     size_t numPending() const { return pending_.size(); }
     size_t numActiveSlots() const;
@@ -146,7 +156,7 @@ private:
     hipEvent_t stepStart_ = nullptr, sceneDone_[4] = {};
     DevBuf sceneScratch_[4];
     uint32_t capacity_ = 0;
-    DevBuf org_, dir_, thr_, rad_, st_, contrib_, savedOrg_, savedDir_, hit_, occl_, result_;
+    DevBuf org_, dir_, thr_, rad_, st_, contrib_, savedOrg_, savedDir_, hit_, occl_, result_, guides_;
     DevBuf qA_, qB_, cat_, scratch_, offs_, prmTable_, live_, fetch_;
     wrk_paths paths_{};
     uint32_t deadCount_ = 0;              // entries ended last iteration: qA_[0, deadCount_)

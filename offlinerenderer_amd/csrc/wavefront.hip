@@ -31,6 +31,7 @@ struct Pool {
     uint2* hit;
     uint8_t* occl;
     float4* result;
+    float4* guides;  // optional: [2p] albedo, [2p + 1] normal of the first hit
 };
 static Pool pool_of(const wrk_paths* p) {
     Pool q;
@@ -38,6 +39,7 @@ static Pool pool_of(const wrk_paths* p) {
     q.st = (uint4*)p->state; q.contrib = (float4*)p->contrib; q.savedOrg = (float4*)p->savedOrg;
     q.savedDir = (float4*)p->savedDir; q.hit = (uint2*)p->hit; q.occl = (uint8_t*)p->occluded;
     q.result = (float4*)p->result;
+    q.guides = (float4*)p->guides;
     return q;
 }
 
@@ -114,6 +116,10 @@ __global__ __launch_bounds__(256) void k_generate(wrk_camera cam, wrk_render_par
     P.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.st[i] = make_uint4(seed, 0u, FLAG_FIRST_HIT | FLAG_PREV_DELTA | (slot << FLAG_SLOT_SHIFT), pixel);
     P.result[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (P.guides) {
+        P.guides[2 * (size_t)i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.guides[2 * (size_t)i + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
     queue[k] = i;
     if (cat) cat[k] = (uint8_t)catValue;
 }
@@ -526,8 +532,14 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_para
     float t = __uint_as_float(hr.x);
     f3 e1, e2;
     Hit h = reconstruct(S, r.o, r.d, t, hr.y, &e1, &e2);
-    r.firstHit = false;  // first-hit guides (trace.cu:267-286) are not produced here
     const wrk_material& mat = S.materials[h.materialId];
+    if (r.firstHit && P.guides) {
+        // trace.cu:267-286: albedo (texture or constant) and the ray-facing
+        // geometric normal of the first hit
+        P.guides[2 * (size_t)p] = f4(material_albedo(S, mat, h.u, h.v), 0.0f);
+        P.guides[2 * (size_t)p + 1] = f4(h.geometricNormal, 0.0f);
+    }
+    r.firstHit = false;
 
     if (seg == WRK_MAT_EMISSIVE) {
         // trace.cu:289-316: emission with MIS, then shadeEmissive / default
@@ -629,9 +641,12 @@ __global__ __launch_bounds__(256) void k_slot_finish(const uint32_t* __restrict_
     if (hist[t]) atomicSub(&live[t], hist[t]);
 }
 
-// accum[pixel] += result[path] over the batch in sample order.
-__global__ __launch_bounds__(256) void k_accumulate(const float4* __restrict__ result, uint32_t poolBase,
-                                                    uint64_t first, uint32_t n, uint32_t npix, float* accum) {
+// accum[pixel] += result[path] over the batch in sample order; the
+// first-hit guides likewise when requested.
+__global__ __launch_bounds__(256) void k_accumulate(const float4* __restrict__ result,
+                                                    const float4* __restrict__ guides, uint32_t poolBase,
+                                                    uint64_t first, uint32_t n, uint32_t npix, float* accum,
+                                                    float* accAlb, float* accNrm) {
     uint32_t k = blockIdx.x * 256 + threadIdx.x;
     uint32_t cover = n < npix ? n : npix;
     if (k >= cover) return;
@@ -643,6 +658,17 @@ __global__ __launch_bounds__(256) void k_accumulate(const float4* __restrict__ r
         a0 = a0 + v.x; a1 = a1 + v.y; a2 = a2 + v.z;
     }
     accum[pixel * 3 + 0] = a0; accum[pixel * 3 + 1] = a1; accum[pixel * 3 + 2] = a2;
+    if (guides && accAlb && accNrm) {
+        float b0 = accAlb[pixel * 3 + 0], b1 = accAlb[pixel * 3 + 1], b2 = accAlb[pixel * 3 + 2];
+        float c0 = accNrm[pixel * 3 + 0], c1 = accNrm[pixel * 3 + 1], c2 = accNrm[pixel * 3 + 2];
+        for (uint64_t i = k; i < n; i += npix) {
+            float4 al = guides[2 * (poolBase + i)], nr = guides[2 * (poolBase + i) + 1];
+            b0 = b0 + al.x; b1 = b1 + al.y; b2 = b2 + al.z;
+            c0 = c0 + nr.x; c1 = c1 + nr.y; c2 = c2 + nr.z;
+        }
+        accAlb[pixel * 3 + 0] = b0; accAlb[pixel * 3 + 1] = b1; accAlb[pixel * 3 + 2] = b2;
+        accNrm[pixel * 3 + 0] = c0; accNrm[pixel * 3 + 1] = c1; accNrm[pixel * 3 + 2] = c2;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_average(const float* __restrict__ accum, uint32_t n3, float invSpp,
@@ -724,13 +750,14 @@ extern "C" int wrk_slot_finish(const uint32_t* finished, uint32_t maxItems, cons
 }
 
 extern "C" int wrk_accumulate(const wrk_paths* paths, uint32_t poolBase, uint64_t first, uint32_t n,
-                              uint32_t npix, float* accum, void* stream) {
+                              uint32_t npix, float* accum, float* accumAlbedo, float* accumNormal, void* stream) {
     WRK_REQUIRE(paths && accum && npix > 0, "wrk_accumulate: bad argument");
     WRK_REQUIRE((uint64_t)poolBase + n <= paths->capacity, "wrk_accumulate: range exceeds the path pool");
     if (n == 0) return WRK_OK;
     uint32_t cover = n < npix ? n : npix;
     hipLaunchKernelGGL(k_accumulate, dim3(ceil_div(cover, 256)), dim3(256), 0, as_stream(stream),
-                       (const float4*)paths->result, poolBase, first, n, npix, accum);
+                       (const float4*)paths->result, (const float4*)paths->guides, poolBase, first, n, npix, accum,
+                       accumAlbedo, accumNormal);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }

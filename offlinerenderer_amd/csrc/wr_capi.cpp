@@ -157,6 +157,13 @@ int wr_c_render_paths(void* r, void* s, const wr_c_camera* cam, const wr_c_rende
                       uint32_t count, float* out) {
     return guard([&] { R(r)->renderPaths(S(s), to_camera(cam), to_params(p), first, count, out); });
 }
+int wr_c_render_guides(void* r, void* s, const wr_c_camera* cam, const wr_c_render_params* p, float* color,
+                       float* albedo, float* normal) {
+    return guard([&] {
+        R(r)->renderWithGuides(S(s), to_camera(cam), to_params(p), reinterpret_cast<wr::Vec3*>(color),
+                               reinterpret_cast<wr::Vec3*>(albedo), reinterpret_cast<wr::Vec3*>(normal));
+    });
+}
 int wr_c_renderer_stats(void* r, wr_c_stats* out) {
     return guard([&] { fill_stats(R(r)->getStats(), out); });
 }

@@ -375,14 +375,16 @@ struct RendererImpl {
     }
     // renderer.cpp:132-316 — the whole image as one request, in chunks of
     // at most maxPathsInFlight paths, accumulated per pixel in sample order.
-    void render(Scene* scene, const Camera& camera, const RenderParams& rp, float* outHost) {
+    void render(Scene* scene, const Camera& camera, const RenderParams& rp, float* outHost, float* albedo = nullptr,
+                float* normal = nullptr) {
         if (!scene) throw std::runtime_error("render: null scene");
         if (rp.width == 0 || rp.height == 0 || rp.spp == 0) throw std::runtime_error("render: empty image");
         const uint64_t total = (uint64_t)rp.width * rp.height * rp.spp;
         detail::Engine& e = engine(total);
         auto t0 = std::chrono::steady_clock::now();
+        const bool guides = albedo || normal;
         uint64_t id = e.submit(scene, detail::camera_record(camera), params_of(rp), rp.spp, 0, total,
-                               detail::Engine::Mode::Accumulate, nullptr);
+                               detail::Engine::Mode::Accumulate, nullptr, guides);
         while (e.step()) {
         }
         std::vector<float> tmp;
@@ -391,7 +393,7 @@ struct RendererImpl {
             tmp.resize((size_t)rp.width * rp.height * 3);
             dst = tmp.data();
         }
-        e.fetch(id, dst);
+        e.fetch(id, dst, albedo, normal);
         stats = e.stats();
         stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         stats.paths = total;
@@ -450,6 +452,12 @@ void Renderer::renderPaths(Scene* scene, const Camera& camera, const RenderParam
     m_impl->render_paths(scene, camera, params, firstPath, count, out);
 }
 const RenderStats& Renderer::getStats() const { return m_impl->stats; }
+void Renderer::renderWithGuides(Scene* scene, const Camera& camera, const RenderParams& params, Vec3* color,
+                                Vec3* albedo, Vec3* normal) {
+    if (!albedo || !normal) throw std::runtime_error("renderWithGuides: null guide buffer");
+    m_impl->render(scene, camera, params, reinterpret_cast<float*>(color), reinterpret_cast<float*>(albedo),
+                   reinterpret_cast<float*>(normal));
+}
 
 // ----------------------------------------------------------- RenderServer
 struct ServerImpl {

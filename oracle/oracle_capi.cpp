@@ -201,6 +201,31 @@ void orc_render_paths(void* h, const orc_camera_in* ci, const orc_params_in* pi,
 // --- whole image: accum[p] = sum over samples [s0, s0+spp) in sample order
 // (renderer.cpp:190 serial samples). fp64 != 0 runs the FP64 instantiation
 // and accumulates in double (accum64 must then be non-null).
+// Guide layers averaged over samples 0..spp-1: per pixel summed in sample
+// order, then scaled by 1/spp in FP32 (trace.cu:267-286, renderer.cpp:308-313).
+void orc_render_guides(void* h, const orc_camera_in* ci, const orc_params_in* pi, uint32_t spp, int useBvh,
+                       int threads, float* albedo, float* normal) {
+    const Scene& s = static_cast<OScene*>(h)->s;
+    Camera cam = make_camera(ci->position, ci->target, ci->up, ci->fovY, ci->aspect, ci->focalDistance, ci->lensRadius);
+    Params p{pi->width, pi->height, pi->useNEE, pi->maxBounces, pi->rrStartDepth, pi->fireflyClamp, pi->iterationCap};
+    const int64_t npix = (int64_t)p.width * p.height;
+    const float inv = 1.0f / (float)spp;
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t px = 0; px < npix; ++px) {
+        float a[3] = {0, 0, 0}, n[3] = {0, 0, 0};
+        for (uint32_t k = 0; k < spp; ++k) {
+            float ga[3], gn[3];
+            first_hit_guides(s, cam, p, (uint32_t)px, k, useBvh != 0, ga, gn);
+            for (int c = 0; c < 3; ++c) { a[c] = a[c] + ga[c]; n[c] = n[c] + gn[c]; }
+        }
+        for (int c = 0; c < 3; ++c) {
+            albedo[px * 3 + c] = a[c] * inv;
+            normal[px * 3 + c] = n[c] * inv;
+        }
+    }
+}
+
 void orc_render_image(void* h, const orc_camera_in* ci, const orc_params_in* pi, uint32_t s0, uint32_t spp,
                       int useBvh, int threads, int fp64, float* accum, double* accum64) {
     const Scene& s = static_cast<OScene*>(h)->s;

@@ -73,6 +73,8 @@ def lib():
             "orc_camera_basis": (None, [C.POINTER(orc_camera_in), _P]),
             "orc_render_paths": (None, [_P, C.POINTER(orc_camera_in), C.POINTER(orc_params_in), _P, _P, C.c_uint32,
                                         C.c_int, C.c_int, _P, _P]),
+            "orc_render_guides": (None, [_P, C.POINTER(orc_camera_in), C.POINTER(orc_params_in), C.c_uint32,
+                                         C.c_int, C.c_int, _P, _P]),
             "orc_render_image": (None, [_P, C.POINTER(orc_camera_in), C.POINTER(orc_params_in), C.c_uint32,
                                         C.c_uint32, C.c_int, C.c_int, C.c_int, _P, _P]),
             "orc_compact": (C.c_uint32, [_P, _P, C.c_uint32, _P]),
@@ -247,6 +249,17 @@ def render_image(scene: OracleScene, cam: dict, prm: orc_params_in, spp: int, s0
         lib().orc_render_image(scene.h, C.byref(ci), C.byref(prm), s0, spp, 1 if use_bvh else 0, threads, 0,
                                _p(acc), None)
     return acc.reshape(prm.height, prm.width, 3)
+
+
+def render_guides(scene: OracleScene, cam: dict, prm: orc_params_in, spp: int, use_bvh=True, threads=0):
+    """First-hit albedo and normal guide layers averaged over samples 0..spp-1,
+    each (height*width, 3) float32."""
+    n = prm.width * prm.height
+    alb = np.zeros((n, 3), dtype=np.float32)
+    nrm = np.zeros((n, 3), dtype=np.float32)
+    ci = camera_in(cam)
+    lib().orc_render_guides(scene.h, C.byref(ci), C.byref(prm), spp, 1 if use_bvh else 0, threads, _p(alb), _p(nrm))
+    return alb, nrm
 
 
 def compact(keep, items=None):

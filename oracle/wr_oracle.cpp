@@ -896,6 +896,25 @@ void trace_path(const Scene& s, const Camera& cam, const Params& p, uint32_t pix
 }
 
 template void trace_path<float>(const Scene&, const Camera&, const Params&, uint32_t, uint32_t, bool, PathOut*, double*);
+
+void first_hit_guides(const Scene& s, const Camera& cam, const Params& p, uint32_t pixel, uint32_t sample,
+                      bool useBvh, float albedo[3], float normal[3]) {
+    Tracer<float> tr(s, cam, p, useBvh, sample);
+    Ray<float> ray;
+    std::memset(&ray, 0, sizeof(ray));
+    HitInfo<float> hit;
+    std::memset(&hit, 0, sizeof(hit));
+    albedo[0] = albedo[1] = albedo[2] = 0.0f;
+    normal[0] = normal[1] = normal[2] = 0.0f;
+    tr.cameraRay(ray, pixel);
+    Hit h = tr.traceClosest(ray);
+    if (h.prim == 0xFFFFFFFFu) return;
+    tr.closestHit(ray, hit, h.t, h.prim);
+    if (hit.materialId >= s.materials.size()) return;
+    V3<float> a = tr.materialAlbedo(s.materials[hit.materialId], hit.u, hit.v);
+    albedo[0] = a.x; albedo[1] = a.y; albedo[2] = a.z;
+    normal[0] = hit.geometricNormal.x; normal[1] = hit.geometricNormal.y; normal[2] = hit.geometricNormal.z;
+}
 template void trace_path<double>(const Scene&, const Camera&, const Params&, uint32_t, uint32_t, bool, PathOut*, double*);
 
 }  // namespace orc

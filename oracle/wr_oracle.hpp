@@ -259,4 +259,10 @@ template <typename T>
 void trace_path(const Scene& s, const Camera& cam, const Params& p, uint32_t pixel,
                 uint32_t sample, bool useBvh, PathOut* out, double* outL64 = nullptr);
 
+// Denoiser guide values of one (pixel, sample) (trace.cu:267-286): albedo
+// (texture or constant) and ray-facing geometric normal at the camera ray's
+// first hit; zero for a miss or an out-of-range material id.
+void first_hit_guides(const Scene& s, const Camera& cam, const Params& p, uint32_t pixel, uint32_t sample,
+                      bool useBvh, float albedo[3], float normal[3]);
+
 }  // namespace orc

@@ -66,3 +66,21 @@ def test_large_scene_product_mode_paths(gpu, oracle):
     d = scenes.large_scene(64, 36, 1, num_spheres=200, num_emitters=16)
     exact, bad, L_o = _paths_parity(gpu, oracle, d, cap=0, use_bvh=True, spp=1)
     assert bad.sum() == 0
+
+
+@pytest.mark.parametrize("which", ["cornell", "material"])
+def test_guides_match_oracle(gpu, oracle, which):
+    d = scenes.cornell(40, 32, 3) if which == "cornell" else scenes.material_scene(64, 36, 3, tex_size=64)
+    ps = G.product_scene(d)
+    os_ = oracle.build_oracle_scene(d, bvh=True)
+    r = wr.Renderer()
+    r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=1 << 20))
+    col, alb, nrm = r.renderWithGuides(ps, G.camera(d), G.params(d))
+    # the colour is the plain render's, bit for bit
+    plain = r.render(ps, G.camera(d), G.params(d))
+    assert np.array_equal(col.view(np.uint32), plain.view(np.uint32))
+    prm = oracle.params_in(d.width, d.height, d.maxBounces, d.rrDepth, iterationCap=0)
+    oa, on = oracle.render_guides(os_, d.camera, prm, d.spp, use_bvh=True)
+    assert np.array_equal(alb.reshape(-1, 3).view(np.uint32), oa.view(np.uint32))
+    assert np.array_equal(nrm.reshape(-1, 3).view(np.uint32), on.view(np.uint32))
+    assert np.count_nonzero(alb) > alb.size // 2
