@@ -1,0 +1,92 @@
+/* wrk_unet.h — C ABI of the learned denoiser (no counterpart function in
+ * libWR: it stands in for the closed OptiX AI denoiser that
+ * libWR/src/internal/denoiser.cpp:17-195 drives with colour + albedo +
+ * normal layers, renderer.cpp:289-298).
+ *
+ * Tensors are NHWC bf16 (passed as uint16_t*) with a channel pitch;
+ * convolution weights are bf16 [9][cout][cin] (tap = 3*ky + kx), biases
+ * FP32. Every function takes an explicit stream and returns WRK_OK or an
+ * error code with wrk_last_error() (kernel level) / wrk_unet_last_error()
+ * (network level).
+ */
+#ifndef WRK_UNET_H
+#define WRK_UNET_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------- kernels */
+/* y[p][co] = act(bias[co] + sum_tap,ci w[tap][co][ci] x[p + (ky-1, kx-1)][ci]),
+ * zero padding, act = ReLU when relu != 0. cin, cout multiples of 32. */
+int wrk_conv3x3_fwd(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* w, const float* bias,
+                    uint16_t* y, uint32_t yPitch, uint32_t cout, uint32_t relu, uint32_t n, uint32_t h, uint32_t wd,
+                    void* stream);
+/* dw[tap][co][ci] = sum_p dz[p][co] x[p + off(tap)][ci] (FP32, overwritten);
+ * db[co] = sum_p dz[p][co] when db != NULL. */
+uint64_t wrk_conv3x3_wgrad_scratch_bytes(uint32_t cin, uint32_t cout, uint32_t n, uint32_t h, uint32_t w);
+int wrk_conv3x3_wgrad(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* dz, uint32_t dzPitch,
+                      uint32_t cout, uint32_t n, uint32_t h, uint32_t w, float* dw, float* db, void* scratch,
+                      uint64_t scratchBytes, void* stream);
+/* FP32 [9][cout][cin] -> bf16 w (same layout) and wt[8 - tap][ci][co]
+ * (the data-gradient weights; may be NULL). */
+int wrk_weights_bf16(const float* w32, uint32_t cout, uint32_t cin, uint16_t* w, uint16_t* wt, void* stream);
+/* dy = y > 0 ? dy : 0 over npix pixels of c channels */
+int wrk_relu_bwd(uint16_t* dy, uint32_t dyPitch, const uint16_t* y, uint32_t yPitch, uint32_t c, uint64_t npix,
+                 void* stream);
+/* 2x2 average pool of (n, h, w) -> (n, h/2, w/2); backward adds the
+ * optional skip gradient: dx = dy / 4 + skip. */
+int wrk_pool2_fwd(const uint16_t* x, uint32_t xPitch, uint32_t c, uint16_t* y, uint32_t yPitch, uint32_t n,
+                  uint32_t h, uint32_t w, void* stream);
+int wrk_pool2_bwd(const uint16_t* dy, uint32_t dyPitch, uint32_t c, const uint16_t* skip, uint32_t skipPitch,
+                  uint16_t* dx, uint32_t dxPitch, uint32_t n, uint32_t h, uint32_t w, void* stream);
+/* nearest 2x upsample to (n, h, w); backward sums each 2x2 block */
+int wrk_up2_fwd(const uint16_t* x, uint32_t xPitch, uint32_t c, uint16_t* y, uint32_t yPitch, uint32_t n, uint32_t h,
+                uint32_t w, void* stream);
+int wrk_up2_bwd(const uint16_t* dy, uint32_t dyPitch, uint32_t c, uint16_t* dx, uint32_t dxPitch, uint32_t n,
+                uint32_t h, uint32_t w, void* stream);
+/* Network input (32 channels): log(1 + max(colour, 0)), albedo, normal,
+ * zeros, from FP32 (img, srcH, srcW, 3) layers; origins (device, 3 per
+ * crop: image, x0, y0) select n crops of h x w, or NULL for whole images. */
+int wrk_unet_pack(const float* color, const float* albedo, const float* normal, uint32_t srcH, uint32_t srcW,
+                  const uint32_t* origins, uint16_t* out, uint32_t n, uint32_t h, uint32_t w, void* stream);
+/* out (npix, 3) FP32 = max(exp(pred) - 1, 0) of channels 0..2 */
+int wrk_unet_unpack(const uint16_t* pred, uint32_t pitch, float* out, uint64_t npix, void* stream);
+/* per-256-pixel partial sums of (pred - log(1 + target))^2 and its gradient */
+uint64_t wrk_unet_loss_scratch_bytes(uint32_t n, uint32_t h, uint32_t w);
+int wrk_unet_loss(const uint16_t* pred, uint32_t pitch, const float* target, uint32_t srcH, uint32_t srcW,
+                  const uint32_t* origins, uint16_t* dpred, uint32_t n, uint32_t h, uint32_t w, float* partial,
+                  void* stream);
+/* Adam on FP32 parameters (step counts from 1) */
+int wrk_adam(float* p, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
+             uint32_t step, void* stream);
+
+/* ------------------------------------------------------------- network */
+/* The 4-level U-Net (channels 32/64/96/128, ~0.9 M parameters). Device
+ * pointers for all image data; params as one FP32 vector. */
+const char* wrk_unet_last_error(void);
+void* wrk_unet_create(void);
+void wrk_unet_destroy(void* net);
+uint64_t wrk_unet_num_params(void* net);
+int wrk_unet_init(void* net, uint64_t seed);
+int wrk_unet_get_params(void* net, float* hostOut);
+int wrk_unet_set_params(void* net, const float* hostIn);
+/* gradient of the last wrk_unet_train_step (layout of the parameters) */
+int wrk_unet_get_grads(void* net, float* hostOut);
+/* denoised colour (n, h, w, 3) FP32 into out; h, w multiples of 8 */
+int wrk_unet_forward(void* net, const float* color, const float* albedo, const float* normal, uint32_t srcH,
+                     uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w, float* out,
+                     void* stream);
+/* one Adam step on the log-space MSE against target; *loss (host) = the
+ * loss before the step */
+int wrk_unet_train_step(void* net, const float* color, const float* albedo, const float* normal, const float* target,
+                        uint32_t srcH, uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w,
+                        float lr, float* loss, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WRK_UNET_H */

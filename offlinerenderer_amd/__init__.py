@@ -128,6 +128,37 @@ _SIGS = {
     "wr_c_server_pending": (C.c_uint32, [_P]),
     "wr_c_server_in_flight": (C.c_uint32, [_P]),
     "wr_c_server_clock": (C.c_double, []),
+    # wrk_unet_* / denoiser kernels (include/wrk_unet.h)
+    "wrk_conv3x3_fwd": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "wrk_conv3x3_wgrad_scratch_bytes": (C.c_uint64, [C.c_uint32] * 5),
+    "wrk_conv3x3_wgrad": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.c_uint32, _P, _P, _P, C.c_uint64, _P]),
+    "wrk_weights_bf16": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P, _P]),
+    "wrk_relu_bwd": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint64, _P]),
+    "wrk_pool2_fwd": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "wrk_pool2_bwd": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                C.c_uint32, _P]),
+    "wrk_up2_fwd": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "wrk_up2_bwd": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "wrk_unet_pack": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "wrk_unet_unpack": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, _P]),
+    "wrk_unet_loss_scratch_bytes": (C.c_uint64, [C.c_uint32] * 3),
+    "wrk_unet_loss": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, C.c_uint32,
+                                C.c_uint32, _P, _P]),
+    "wrk_adam": (C.c_int, [_P, _P, _P, _P, C.c_uint64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, _P]),
+    "wrk_unet_last_error": (C.c_char_p, []),
+    "wrk_unet_create": (_P, []),
+    "wrk_unet_destroy": (None, [_P]),
+    "wrk_unet_num_params": (C.c_uint64, [_P]),
+    "wrk_unet_init": (C.c_int, [_P, C.c_uint64]),
+    "wrk_unet_get_params": (C.c_int, [_P, _P]),
+    "wrk_unet_set_params": (C.c_int, [_P, _P]),
+    "wrk_unet_get_grads": (C.c_int, [_P, _P]),
+    "wrk_unet_forward": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                   _P, _P]),
+    "wrk_unet_train_step": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32,
+                                      C.c_uint32, C.c_float, _P, _P]),
     # wrk_*
     "wrk_last_error": (C.c_char_p, []),
     "wrk_device_count": (C.c_int, [C.POINTER(C.c_int)]),

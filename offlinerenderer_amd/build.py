@@ -25,6 +25,8 @@ SOURCES = [
     "bvh.hip",
     "partition.hip",
     "wavefront.hip",
+    "denoise.hip",
+    "unet.cpp",
     "wrk_api.cpp",
     "engine.cpp",
     "wr_host.cpp",

@@ -1,0 +1,639 @@
+// denoise.hip — kernels of the learned denoiser (the OptiX AI denoiser of
+// libWR/src/internal/denoiser.cpp:17-195 is closed; this U-Net replaces it).
+//
+// Tensors are NHWC bf16 with a channel pitch (so a convolution can write
+// straight into one channel slice of a concatenation buffer). 3x3
+// convolutions are implicit GEMMs on the gfx950 bf16 MFMA
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation:
+//   forward  Y[p][co]    = sum_tap,ci W[tap][co][ci] X[p + off(tap)][ci]
+//            (M = output channels, N = pixels, K = 9 taps x input channels);
+//   data gradient: the forward kernel over dZ with the tap-flipped,
+//            transposed weights W'[tap][ci][co] = W[8 - tap][co][ci];
+//   weight gradient dW[tap][co][ci] = sum_p dZ[p][co] X[p + off(tap)][ci]
+//            (M = co, N = ci, K = pixels), per-block partial sums then one
+//            reduction (no float atomics: the result does not depend on
+//            scheduling).
+// MFMA operand layout (16x16x32): lane l holds row/column (l & 15) and
+// K elements 8*(l >> 4) .. +7; the accumulator lane l holds rows
+// 4*(l >> 4) + i, column l & 15.
+#include <hip/hip_runtime.h>
+
+#include "wrk_common.h"
+
+namespace wrk {
+
+typedef __bf16 bf16;
+typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// FP32 -> bf16, round to nearest even (NaN stays NaN), written out so the
+// result does not depend on how the compiler lowers the conversion.
+__device__ __forceinline__ bf16 f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    uint16_t h;
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) {
+        h = (uint16_t)((u >> 16) | 0x40u);
+    } else {
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        h = (uint16_t)(u >> 16);
+    }
+    return __builtin_bit_cast(bf16, h);
+}
+
+// ------------------------------------------------------------ conv forward
+constexpr int kTH = 8, kTW = 16;            // output pixels per block (4 waves x 2 rows x 16)
+constexpr int kIH = kTH + 2, kIW = kTW + 2;  // input tile with the 1-pixel halo
+
+template <int CT>  // output-channel tiles of 16 per block
+__global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, uint32_t xPitch, uint32_t cin,
+                                                  const bf16* __restrict__ w, const float* __restrict__ bias,
+                                                  bf16* __restrict__ y, uint32_t yPitch, uint32_t cout,
+                                                  uint32_t relu, uint32_t H, uint32_t W, uint32_t tilesX) {
+    __shared__ __attribute__((aligned(16))) bf16 sIn[kIH * kIW * 32];
+    __shared__ __attribute__((aligned(16))) bf16 sW[9 * CT * 16 * 32];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int x0 = (int)(blockIdx.x % tilesX) * kTW, y0 = (int)(blockIdx.x / tilesX) * kTH;
+    const int co0 = (int)blockIdx.y * CT * 16;
+    const size_t img = (size_t)blockIdx.z * H * W;
+    f32x4 acc[2][CT];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < CT; ++c) acc[r][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (uint32_t ci0 = 0; ci0 < cin; ci0 += 32) {
+        for (int e = t; e < kIH * kIW * 4; e += 256) {
+            const int pix = e >> 2, part = e & 3;
+            const int gy = y0 + pix / kIW - 1, gx = x0 + pix % kIW - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (gy >= 0 && gy < (int)H && gx >= 0 && gx < (int)W)
+                v = *(const uint4*)(x + (img + (size_t)gy * W + gx) * xPitch + ci0 + part * 8);
+            *(uint4*)(sIn + pix * 32 + part * 8) = v;
+        }
+        for (int e = t; e < 9 * CT * 16 * 4; e += 256) {
+            const int row = e >> 2, part = e & 3;
+            const int tap = row / (CT * 16), col = row % (CT * 16);
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (co0 + col < (int)cout)
+                v = *(const uint4*)(w + ((size_t)tap * cout + co0 + col) * cin + ci0 + part * 8);
+            *(uint4*)(sW + row * 32 + part * 8) = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;
+            bf16x8 a[CT];
+#pragma unroll
+            for (int c = 0; c < CT; ++c)
+                a[c] = *(const bf16x8*)(sW + ((tap * CT + c) * 16 + (lane & 15)) * 32 + (lane >> 4) * 8);
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int py = wave * 2 + r + dy, px = (lane & 15) + dx;
+                const bf16x8 b = *(const bf16x8*)(sIn + (py * kIW + px) * 32 + (lane >> 4) * 8);
+#pragma unroll
+                for (int c = 0; c < CT; ++c) acc[r][c] = mfma16(a[c], b, acc[r][c]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int oy = y0 + wave * 2 + r, ox = x0 + (lane & 15);
+        if (oy >= (int)H || ox >= (int)W) continue;
+        bf16* out = y + (img + (size_t)oy * W + ox) * yPitch;
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            const int co = co0 + c * 16 + (lane >> 4) * 4;
+            if (co >= (int)cout) continue;
+            union { bf16 h[4]; uint2 u; } pk;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float v = acc[r][c][i] + (bias ? bias[co + i] : 0.0f);
+                if (relu) v = fmaxf(v, 0.0f);
+                pk.h[i] = f2bf(v);
+            }
+            *(uint2*)(out + co) = pk.u;
+        }
+    }
+}
+
+// ------------------------------------------------------- weight gradient
+constexpr int kGH = 4, kGW = 32;  // pixels per K tile: 4 rows x 32 columns (4 MFMA K steps)
+
+// Block: 32 output x 32 input channels, all 9 taps, a run of pixel tiles.
+// Wave w: output channels 16*(w >> 1).., input channels 16*(w & 1)..
+// X is staged three times, shifted by dx = 0, 1, 2, so every operand read is
+// one aligned 16-byte LDS load.
+__global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ x, uint32_t xPitch, uint32_t cin,
+                                                    const bf16* __restrict__ dz, uint32_t dzPitch, uint32_t cout,
+                                                    uint32_t N, uint32_t H, uint32_t W, uint32_t tilesX,
+                                                    uint32_t tilesPerImage, uint32_t tilesPerBlock,
+                                                    float* __restrict__ partial) {
+    __shared__ __attribute__((aligned(16))) bf16 sZ[32 * kGH * kGW];
+    __shared__ __attribute__((aligned(16))) bf16 sX[3 * 32 * (kGH + 2) * kGW];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint32_t ciTiles = cin / 32;
+    const int co0 = (int)(blockIdx.x / ciTiles) * 32, ci0 = (int)(blockIdx.x % ciTiles) * 32;
+    const int coSub = (wave >> 1) * 16, ciSub = (wave & 1) * 16;
+    f32x4 acc[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const uint64_t totalTiles = (uint64_t)N * tilesPerImage;
+    const uint64_t tBegin = (uint64_t)blockIdx.y * tilesPerBlock;
+    const uint64_t tEnd = tBegin + tilesPerBlock < totalTiles ? tBegin + tilesPerBlock : totalTiles;
+    for (uint64_t tile = tBegin; tile < tEnd; ++tile) {
+        const uint32_t n = (uint32_t)(tile / tilesPerImage), tin = (uint32_t)(tile % tilesPerImage);
+        const int x0 = (int)(tin % tilesX) * kGW, y0 = (int)(tin / tilesX) * kGH;
+        const size_t img = (size_t)n * H * W;
+        // dZ tile, transposed to [co][pixel]
+        for (int e = t; e < kGH * kGW * 4; e += 256) {
+            const int pix = e >> 2, part = e & 3;
+            const int gy = y0 + pix / kGW, gx = x0 + pix % kGW;
+            union { uint4 u; bf16 h[8]; } v;
+            v.u = make_uint4(0u, 0u, 0u, 0u);
+            if (gy < (int)H && gx < (int)W)
+                v.u = *(const uint4*)(dz + (img + (size_t)gy * W + gx) * dzPitch + co0 + part * 8);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sZ[(part * 8 + j) * (kGH * kGW) + pix] = v.h[j];
+        }
+        // X rows y0-1 .. y0+kGH, three copies shifted by dx - 1
+        for (int e = t; e < 3 * (kGH + 2) * kGW * 4; e += 256) {
+            const int part = e & 3, q = e >> 2;
+            const int col = q % kGW, row = (q / kGW) % (kGH + 2), d = q / (kGW * (kGH + 2));
+            const int gy = y0 + row - 1, gx = x0 + col + d - 1;
+            union { uint4 u; bf16 h[8]; } v;
+            v.u = make_uint4(0u, 0u, 0u, 0u);
+            if (gy >= 0 && gy < (int)H && gx >= 0 && gx < (int)W)
+                v.u = *(const uint4*)(x + (img + (size_t)gy * W + gx) * xPitch + ci0 + part * 8);
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                sX[((d * 32 + part * 8 + j) * (kGH + 2) + row) * kGW + col] = v.h[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kGH; ++r) {
+            const bf16x8 a =
+                *(const bf16x8*)(sZ + (coSub + (lane & 15)) * (kGH * kGW) + r * kGW + (lane >> 4) * 8);
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap) {
+                const int dy = tap / 3, dx = tap % 3;
+                const bf16x8 b = *(const bf16x8*)(sX + ((dx * 32 + ciSub + (lane & 15)) * (kGH + 2) + r + dy) * kGW +
+                                                  (lane >> 4) * 8);
+                acc[tap] = mfma16(a, b, acc[tap]);
+            }
+        }
+        __syncthreads();
+    }
+    float* out = partial + (size_t)blockIdx.y * 9 * cout * cin;
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int co = co0 + coSub + (lane >> 4) * 4 + i, ci = ci0 + ciSub + (lane & 15);
+            out[((size_t)tap * cout + co) * cin + ci] = acc[tap][i];
+        }
+}
+
+// out[i] = sum_s partial[s][i], in split order
+__global__ __launch_bounds__(256) void k_reduce_splits(const float* __restrict__ partial, uint32_t splits,
+                                                       uint64_t n, float* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (uint32_t k = 0; k < splits; ++k) s += partial[(uint64_t)k * n + i];
+    out[i] = s;
+}
+
+// bias gradient: per-block column sums over a pixel run, then k_reduce_splits
+__global__ __launch_bounds__(256) void k_bias_grad(const bf16* __restrict__ dz, uint32_t dzPitch, uint32_t cout,
+                                                   uint64_t npix, uint64_t pixPerBlock, float* __restrict__ partial) {
+    const uint32_t c = threadIdx.x;
+    if (c >= cout) return;
+    const uint64_t p0 = (uint64_t)blockIdx.x * pixPerBlock;
+    const uint64_t p1 = p0 + pixPerBlock < npix ? p0 + pixPerBlock : npix;
+    float s = 0.0f;
+    for (uint64_t p = p0; p < p1; ++p) s += (float)dz[p * dzPitch + c];
+    partial[(uint64_t)blockIdx.x * cout + c] = s;
+}
+
+// ------------------------------------------------------------ elementwise
+// Each thread handles 8 channels (16 bytes) of one pixel; C % 8 == 0.
+
+// dY = Y > 0 ? dY : 0
+__global__ __launch_bounds__(256) void k_relu_bwd(bf16* __restrict__ dy, uint32_t dyPitch,
+                                                  const bf16* __restrict__ y, uint32_t yPitch, uint32_t C,
+                                                  uint64_t npix) {
+    const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t groups = C / 8;
+    if (e >= npix * groups) return;
+    const uint64_t p = e / groups;
+    const uint32_t c = (uint32_t)(e % groups) * 8;
+    union { uint4 u; bf16 h[8]; } g, v;
+    g.u = *(const uint4*)(dy + p * dyPitch + c);
+    v.u = *(const uint4*)(y + p * yPitch + c);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if (!((float)v.h[j] > 0.0f)) g.h[j] = f2bf(0.0f);
+    *(uint4*)(dy + p * dyPitch + c) = g.u;
+}
+
+// 2x2 average pool: (n, H, W) -> (n, H/2, W/2)
+__global__ __launch_bounds__(256) void k_pool2_fwd(const bf16* __restrict__ x, uint32_t xPitch, uint32_t C,
+                                                   bf16* __restrict__ y, uint32_t yPitch, uint32_t N, uint32_t H,
+                                                   uint32_t W) {
+    const uint32_t Ho = H / 2, Wo = W / 2, groups = C / 8;
+    const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (uint64_t)N * Ho * Wo * groups) return;
+    const uint64_t q = e / groups;
+    const uint32_t c = (uint32_t)(e % groups) * 8;
+    const uint32_t ox = (uint32_t)(q % Wo), oy = (uint32_t)((q / Wo) % Ho), n = (uint32_t)(q / ((uint64_t)Wo * Ho));
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t p = ((uint64_t)n * H + 2 * oy + (k >> 1)) * W + 2 * ox + (k & 1);
+        union { uint4 u; bf16 h[8]; } v;
+        v.u = *(const uint4*)(x + p * xPitch + c);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s[j] += (float)v.h[j];
+    }
+    union { uint4 u; bf16 h[8]; } o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o.h[j] = f2bf(s[j] * 0.25f);
+    *(uint4*)(y + q * yPitch + c) = o.u;
+}
+
+// dX (n, H, W) = dY(n, H/2, W/2) / 4 (+ skip gradient when given)
+__global__ __launch_bounds__(256) void k_pool2_bwd(const bf16* __restrict__ dy, uint32_t dyPitch, uint32_t C,
+                                                   const bf16* __restrict__ skip, uint32_t skipPitch,
+                                                   bf16* __restrict__ dx, uint32_t dxPitch, uint32_t N, uint32_t H,
+                                                   uint32_t W) {
+    const uint32_t groups = C / 8;
+    const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (uint64_t)N * H * W * groups) return;
+    const uint64_t p = e / groups;
+    const uint32_t c = (uint32_t)(e % groups) * 8;
+    const uint32_t xx = (uint32_t)(p % W), yy = (uint32_t)((p / W) % H), n = (uint32_t)(p / ((uint64_t)W * H));
+    const uint64_t q = ((uint64_t)n * (H / 2) + yy / 2) * (W / 2) + xx / 2;
+    union { uint4 u; bf16 h[8]; } g, s, o;
+    g.u = *(const uint4*)(dy + q * dyPitch + c);
+    s.u = skip ? *(const uint4*)(skip + p * skipPitch + c) : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o.h[j] = f2bf((float)g.h[j] * 0.25f + (float)s.h[j]);
+    *(uint4*)(dx + p * dxPitch + c) = o.u;
+}
+
+// nearest 2x upsample: (n, H/2, W/2) -> (n, H, W)
+__global__ __launch_bounds__(256) void k_up2_fwd(const bf16* __restrict__ x, uint32_t xPitch, uint32_t C,
+                                                 bf16* __restrict__ y, uint32_t yPitch, uint32_t N, uint32_t H,
+                                                 uint32_t W) {
+    const uint32_t groups = C / 8;
+    const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (uint64_t)N * H * W * groups) return;
+    const uint64_t p = e / groups;
+    const uint32_t c = (uint32_t)(e % groups) * 8;
+    const uint32_t xx = (uint32_t)(p % W), yy = (uint32_t)((p / W) % H), n = (uint32_t)(p / ((uint64_t)W * H));
+    const uint64_t q = ((uint64_t)n * (H / 2) + yy / 2) * (W / 2) + xx / 2;
+    *(uint4*)(y + p * yPitch + c) = *(const uint4*)(x + q * xPitch + c);
+}
+
+// dX (n, H/2, W/2) = sum of the 4 dY it was copied to
+__global__ __launch_bounds__(256) void k_up2_bwd(const bf16* __restrict__ dy, uint32_t dyPitch, uint32_t C,
+                                                 bf16* __restrict__ dx, uint32_t dxPitch, uint32_t N, uint32_t H,
+                                                 uint32_t W) {
+    const uint32_t Ho = H / 2, Wo = W / 2, groups = C / 8;
+    const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (uint64_t)N * Ho * Wo * groups) return;
+    const uint64_t q = e / groups;
+    const uint32_t c = (uint32_t)(e % groups) * 8;
+    const uint32_t ox = (uint32_t)(q % Wo), oy = (uint32_t)((q / Wo) % Ho), n = (uint32_t)(q / ((uint64_t)Wo * Ho));
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t p = ((uint64_t)n * H + 2 * oy + (k >> 1)) * W + 2 * ox + (k & 1);
+        union { uint4 u; bf16 h[8]; } v;
+        v.u = *(const uint4*)(dy + p * dyPitch + c);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s[j] += (float)v.h[j];
+    }
+    union { uint4 u; bf16 h[8]; } o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o.h[j] = f2bf(s[j]);
+    *(uint4*)(dx + q * dxPitch + c) = o.u;
+}
+
+// ---------------------------------------------------- input / output / loss
+// 9 guide channels into a 32-channel bf16 tensor: log(1 + max(colour, 0)),
+// albedo, normal, zeros. Inputs are FP32 (n, H, W, 3), rows of srcW pixels
+// starting at (sx0, sy0) of an image of srcH x srcW (crops for training).
+__global__ __launch_bounds__(256) void k_unet_pack(const float* __restrict__ color, const float* __restrict__ albedo,
+                                                   const float* __restrict__ normal, uint32_t srcH, uint32_t srcW,
+                                                   const uint32_t* __restrict__ origins, bf16* __restrict__ out,
+                                                   uint32_t N, uint32_t H, uint32_t W) {
+    const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (uint64_t)N * H * W) return;
+    const uint32_t xx = (uint32_t)(p % W), yy = (uint32_t)((p / W) % H), n = (uint32_t)(p / ((uint64_t)W * H));
+    uint32_t sx = xx, sy = yy, img = n;
+    if (origins) {  // crop n: image origins[3n] at (origins[3n+1], origins[3n+2])
+        img = origins[3 * n];
+        sx += origins[3 * n + 1];
+        sy += origins[3 * n + 2];
+    }
+    const uint64_t s = ((uint64_t)img * srcH + sy) * srcW + sx;
+    float v[32];
+#pragma unroll
+    for (int c = 0; c < 32; ++c) v[c] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        v[c] = log1pf(fmaxf(color[s * 3 + c], 0.0f));
+        v[3 + c] = albedo[s * 3 + c];
+        v[6 + c] = normal[s * 3 + c];
+    }
+    union { uint4 u[4]; bf16 h[32]; } o;
+#pragma unroll
+    for (int c = 0; c < 32; ++c) o.h[c] = f2bf(v[c]);
+    uint4* dst = (uint4*)(out + p * 32);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) dst[k] = o.u[k];
+}
+
+// out = exp(pred) - 1 for the first 3 channels (the network predicts log(1 + L))
+__global__ __launch_bounds__(256) void k_unet_unpack(const bf16* __restrict__ pred, uint32_t pitch,
+                                                     float* __restrict__ out, uint64_t npix) {
+    const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= npix) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[p * 3 + c] = fmaxf(expm1f((float)pred[p * pitch + c]), 0.0f);
+}
+
+// Loss = mean over pixels and 3 channels of (pred - log(1 + target))^2;
+// dPred = 2 (pred - log(1 + target)) / count for the 3 channels, 0 for the
+// padding channels. Per-block partial loss sums.
+__global__ __launch_bounds__(256) void k_unet_loss(const bf16* __restrict__ pred, uint32_t pitch,
+                                                   const float* __restrict__ target, uint32_t srcH, uint32_t srcW,
+                                                   const uint32_t* __restrict__ origins, bf16* __restrict__ dpred,
+                                                   uint32_t N, uint32_t H, uint32_t W, float* __restrict__ partial) {
+    __shared__ float red[256];
+    const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t npix = (uint64_t)N * H * W;
+    float l = 0.0f;
+    if (p < npix) {
+        const uint32_t xx = (uint32_t)(p % W), yy = (uint32_t)((p / W) % H), n = (uint32_t)(p / ((uint64_t)W * H));
+        uint32_t sx = xx, sy = yy, img = n;
+        if (origins) {
+            img = origins[3 * n];
+            sx += origins[3 * n + 1];
+            sy += origins[3 * n + 2];
+        }
+        const uint64_t s = ((uint64_t)img * srcH + sy) * srcW + sx;
+        const float scale = 2.0f / (float)(npix * 3);
+        for (uint32_t c = 0; c < pitch; ++c) {
+            float g = 0.0f;
+            if (c < 3) {
+                const float d = (float)pred[p * pitch + c] - log1pf(fmaxf(target[s * 3 + c], 0.0f));
+                l += d * d;
+                g = d * scale;
+            }
+            dpred[p * pitch + c] = f2bf(g);
+        }
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------- weights
+// FP32 master [9][cout][cin] -> bf16 forward layout and the tap-flipped
+// transposed layout used by the data gradient.
+__global__ __launch_bounds__(256) void k_weights_bf16(const float* __restrict__ w32, uint32_t cout, uint32_t cin,
+                                                      bf16* __restrict__ w, bf16* __restrict__ wt) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= 9ull * cout * cin) return;
+    const uint32_t ci = (uint32_t)(i % cin), co = (uint32_t)((i / cin) % cout), tap = (uint32_t)(i / ((uint64_t)cin * cout));
+    const bf16 v = f2bf(w32[i]);
+    w[i] = v;
+    if (wt) wt[((uint64_t)(8 - tap) * cin + ci) * cout + co] = v;
+}
+
+// Adam (Kingma & Ba 2015) on FP32 parameters; bc1 = 1 - b1^t, bc2 = 1 - b2^t.
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float* __restrict__ g,
+                                              float* __restrict__ m, float* __restrict__ v, uint64_t n, float lr,
+                                              float b1, float b2, float eps, float bc1, float bc2) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+}  // namespace wrk
+
+using namespace wrk;
+
+static int conv_check(uint32_t cin, uint32_t cout, uint32_t xPitch, uint32_t yPitch) {
+    WRK_REQUIRE(cin % 32 == 0 && cin > 0, "conv3x3: input channels must be a positive multiple of 32");
+    WRK_REQUIRE(cout % 32 == 0 && cout > 0, "conv3x3: output channels must be a positive multiple of 32");
+    WRK_REQUIRE(xPitch % 8 == 0 && yPitch % 8 == 0, "conv3x3: channel pitches must be multiples of 8");
+    WRK_REQUIRE(xPitch >= cin && yPitch >= cout, "conv3x3: pitch smaller than the channel count");
+    return WRK_OK;
+}
+
+extern "C" int wrk_conv3x3_fwd(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* w,
+                               const float* bias, uint16_t* y, uint32_t yPitch, uint32_t cout, uint32_t relu,
+                               uint32_t n, uint32_t h, uint32_t wd, void* stream) {
+    WRK_REQUIRE(x && w && y, "wrk_conv3x3_fwd: null argument");
+    if (int rc = conv_check(cin, cout, xPitch, yPitch)) return rc;
+    if (n == 0 || h == 0 || wd == 0) return WRK_OK;
+    const uint32_t tilesX = ceil_div(wd, kTW), tilesY = ceil_div(h, kTH);
+    WRK_REQUIRE(n <= 65535, "wrk_conv3x3_fwd: batch too large");
+    if (cout % 64 == 0) {
+        hipLaunchKernelGGL(k_conv_fwd<4>, dim3(tilesX * tilesY, cout / 64, n), dim3(256), 0, as_stream(stream),
+                           (const bf16*)x, xPitch, cin, (const bf16*)w, bias, (bf16*)y, yPitch, cout, relu, h, wd,
+                           tilesX);
+    } else {
+        hipLaunchKernelGGL(k_conv_fwd<2>, dim3(tilesX * tilesY, cout / 32, n), dim3(256), 0, as_stream(stream),
+                           (const bf16*)x, xPitch, cin, (const bf16*)w, bias, (bf16*)y, yPitch, cout, relu, h, wd,
+                           tilesX);
+    }
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+static void wgrad_plan(uint32_t cin, uint32_t cout, uint32_t n, uint32_t h, uint32_t w, uint32_t* splits,
+                       uint32_t* tilesPerBlock, uint32_t* tilesX, uint32_t* tilesPerImage) {
+    *tilesX = ceil_div(w, kGW);
+    *tilesPerImage = *tilesX * ceil_div(h, kGH);
+    const uint64_t total = (uint64_t)n * *tilesPerImage;
+    const uint32_t pairs = (cin / 32) * (cout / 32);
+    uint64_t want = pairs >= 1024 ? 1 : 1024 / pairs;  // about 1024 blocks in all
+    if (want > total) want = total;
+    if (want < 1) want = 1;
+    *tilesPerBlock = (uint32_t)((total + want - 1) / want);
+    *splits = (uint32_t)((total + *tilesPerBlock - 1) / *tilesPerBlock);
+}
+
+extern "C" uint64_t wrk_conv3x3_wgrad_scratch_bytes(uint32_t cin, uint32_t cout, uint32_t n, uint32_t h, uint32_t w) {
+    if (cin == 0 || cout == 0 || n == 0 || h == 0 || w == 0) return 0;
+    uint32_t splits, tpb, tx, tpi;
+    wgrad_plan(cin, cout, n, h, w, &splits, &tpb, &tx, &tpi);
+    const uint64_t wBytes = (uint64_t)splits * 9 * cout * cin * 4;
+    const uint64_t bBytes = 1024ull * cout * 4;
+    return wBytes > bBytes ? wBytes : bBytes;
+}
+
+extern "C" int wrk_conv3x3_wgrad(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* dz,
+                                 uint32_t dzPitch, uint32_t cout, uint32_t n, uint32_t h, uint32_t w, float* dw,
+                                 float* db, void* scratch, uint64_t scratchBytes, void* stream) {
+    WRK_REQUIRE(x && dz && dw && scratch, "wrk_conv3x3_wgrad: null argument");
+    if (int rc = conv_check(cin, cout, xPitch, dzPitch)) return rc;
+    WRK_REQUIRE(n > 0 && h > 0 && w > 0, "wrk_conv3x3_wgrad: empty tensor");
+    WRK_REQUIRE(scratchBytes >= wrk_conv3x3_wgrad_scratch_bytes(cin, cout, n, h, w), "wrk_conv3x3_wgrad: scratch too small");
+    WRK_REQUIRE(!db || cout <= 256, "wrk_conv3x3_wgrad: bias gradient needs cout <= 256");
+    uint32_t splits, tpb, tilesX, tpi;
+    wgrad_plan(cin, cout, n, h, w, &splits, &tpb, &tilesX, &tpi);
+    float* partial = (float*)scratch;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_conv_wgrad, dim3((cin / 32) * (cout / 32), splits), dim3(256), 0, s, (const bf16*)x, xPitch,
+                       cin, (const bf16*)dz, dzPitch, cout, n, h, w, tilesX, tpi, tpb, partial);
+    const uint64_t nw = 9ull * cout * cin;
+    hipLaunchKernelGGL(k_reduce_splits, dim3(ceil_div(nw, 256)), dim3(256), 0, s, partial, splits, nw, dw);
+    if (db) {
+        // the bias reduction reuses the scratch after the weight reduction
+        const uint64_t npix = (uint64_t)n * h * w;
+        const uint64_t per = (npix + 1023) / 1024;
+        const uint32_t blocks = (uint32_t)((npix + per - 1) / per);
+        hipLaunchKernelGGL(k_bias_grad, dim3(blocks), dim3(256), 0, s, (const bf16*)dz, dzPitch, cout, npix, per,
+                           partial);
+        hipLaunchKernelGGL(k_reduce_splits, dim3(ceil_div(cout, 256)), dim3(256), 0, s, partial, blocks,
+                           (uint64_t)cout, db);
+    }
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_relu_bwd(uint16_t* dy, uint32_t dyPitch, const uint16_t* y, uint32_t yPitch, uint32_t c,
+                            uint64_t npix, void* stream) {
+    WRK_REQUIRE(dy && y && c % 8 == 0 && dyPitch % 8 == 0 && yPitch % 8 == 0, "wrk_relu_bwd: bad argument");
+    const uint64_t n = npix * (c / 8);
+    if (n == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_relu_bwd, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), (bf16*)dy, dyPitch,
+                       (const bf16*)y, yPitch, c, npix);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_pool2_fwd(const uint16_t* x, uint32_t xPitch, uint32_t c, uint16_t* y, uint32_t yPitch,
+                             uint32_t n, uint32_t h, uint32_t w, void* stream) {
+    WRK_REQUIRE(x && y && c % 8 == 0 && h % 2 == 0 && w % 2 == 0, "wrk_pool2_fwd: bad argument");
+    const uint64_t e = (uint64_t)n * (h / 2) * (w / 2) * (c / 8);
+    if (e == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_pool2_fwd, dim3(ceil_div(e, 256)), dim3(256), 0, as_stream(stream), (const bf16*)x, xPitch,
+                       c, (bf16*)y, yPitch, n, h, w);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_pool2_bwd(const uint16_t* dy, uint32_t dyPitch, uint32_t c, const uint16_t* skip,
+                             uint32_t skipPitch, uint16_t* dx, uint32_t dxPitch, uint32_t n, uint32_t h, uint32_t w,
+                             void* stream) {
+    WRK_REQUIRE(dy && dx && c % 8 == 0 && h % 2 == 0 && w % 2 == 0, "wrk_pool2_bwd: bad argument");
+    const uint64_t e = (uint64_t)n * h * w * (c / 8);
+    if (e == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_pool2_bwd, dim3(ceil_div(e, 256)), dim3(256), 0, as_stream(stream), (const bf16*)dy,
+                       dyPitch, c, (const bf16*)skip, skipPitch, (bf16*)dx, dxPitch, n, h, w);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_up2_fwd(const uint16_t* x, uint32_t xPitch, uint32_t c, uint16_t* y, uint32_t yPitch, uint32_t n,
+                           uint32_t h, uint32_t w, void* stream) {
+    WRK_REQUIRE(x && y && c % 8 == 0 && h % 2 == 0 && w % 2 == 0, "wrk_up2_fwd: bad argument");
+    const uint64_t e = (uint64_t)n * h * w * (c / 8);
+    if (e == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_up2_fwd, dim3(ceil_div(e, 256)), dim3(256), 0, as_stream(stream), (const bf16*)x, xPitch, c,
+                       (bf16*)y, yPitch, n, h, w);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_up2_bwd(const uint16_t* dy, uint32_t dyPitch, uint32_t c, uint16_t* dx, uint32_t dxPitch,
+                           uint32_t n, uint32_t h, uint32_t w, void* stream) {
+    WRK_REQUIRE(dy && dx && c % 8 == 0 && h % 2 == 0 && w % 2 == 0, "wrk_up2_bwd: bad argument");
+    const uint64_t e = (uint64_t)n * (h / 2) * (w / 2) * (c / 8);
+    if (e == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_up2_bwd, dim3(ceil_div(e, 256)), dim3(256), 0, as_stream(stream), (const bf16*)dy, dyPitch,
+                       c, (bf16*)dx, dxPitch, n, h, w);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_unet_pack(const float* color, const float* albedo, const float* normal, uint32_t srcH,
+                             uint32_t srcW, const uint32_t* origins, uint16_t* out, uint32_t n, uint32_t h, uint32_t w,
+                             void* stream) {
+    WRK_REQUIRE(color && albedo && normal && out, "wrk_unet_pack: null argument");
+    WRK_REQUIRE(origins || (srcH == h && srcW == w), "wrk_unet_pack: crops need origins");
+    const uint64_t np = (uint64_t)n * h * w;
+    if (np == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_unet_pack, dim3(ceil_div(np, 256)), dim3(256), 0, as_stream(stream), color, albedo, normal,
+                       srcH, srcW, origins, (bf16*)out, n, h, w);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_unet_unpack(const uint16_t* pred, uint32_t pitch, float* out, uint64_t npix, void* stream) {
+    WRK_REQUIRE(pred && out && pitch >= 3, "wrk_unet_unpack: bad argument");
+    if (npix == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_unet_unpack, dim3(ceil_div(npix, 256)), dim3(256), 0, as_stream(stream), (const bf16*)pred,
+                       pitch, out, npix);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" uint64_t wrk_unet_loss_scratch_bytes(uint32_t n, uint32_t h, uint32_t w) {
+    return (uint64_t)ceil_div((uint64_t)n * h * w, 256) * 4;
+}
+
+extern "C" int wrk_unet_loss(const uint16_t* pred, uint32_t pitch, const float* target, uint32_t srcH, uint32_t srcW,
+                             const uint32_t* origins, uint16_t* dpred, uint32_t n, uint32_t h, uint32_t w,
+                             float* partial, void* stream) {
+    WRK_REQUIRE(pred && target && dpred && partial && pitch >= 3, "wrk_unet_loss: bad argument");
+    WRK_REQUIRE(origins || (srcH == h && srcW == w), "wrk_unet_loss: crops need origins");
+    const uint64_t np = (uint64_t)n * h * w;
+    if (np == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_unet_loss, dim3(ceil_div(np, 256)), dim3(256), 0, as_stream(stream), (const bf16*)pred,
+                       pitch, target, srcH, srcW, origins, (bf16*)dpred, n, h, w, partial);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_weights_bf16(const float* w32, uint32_t cout, uint32_t cin, uint16_t* w, uint16_t* wt,
+                                void* stream) {
+    WRK_REQUIRE(w32 && w, "wrk_weights_bf16: null argument");
+    const uint64_t n = 9ull * cout * cin;
+    if (n == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_weights_bf16, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), w32, cout, cin,
+                       (bf16*)w, (bf16*)wt);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_adam(float* p, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2,
+                        float eps, uint32_t step, void* stream) {
+    WRK_REQUIRE(p && g && m && v && step > 0, "wrk_adam: bad argument");
+    if (n == 0) return WRK_OK;
+    const float bc1 = 1.0f - powf(b1, (float)step), bc2 = 1.0f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), p, g, m, v, n, lr, b1, b2,
+                       eps, bc1, bc2);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}

@@ -1,0 +1,102 @@
+"""Learned denoiser: the U-Net of csrc/unet.cpp (BF16 MFMA kernels in
+csrc/denoise.hip) over the C ABI of include/wrk_unet.h.
+
+Stands in for libWR's OptiX AI denoiser (libWR/src/internal/denoiser.cpp),
+which takes the same three layers: noisy colour, first-hit albedo and
+normal (Renderer.renderWithGuides). Image data stays on the device (torch
+CUDA tensors as buffers); the parameters are one FP32 vector.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib
+
+# 15 convolutions: (cin, cout, level); weights [9][cout][cin] then bias [cout]
+LAYERS = [(32, 32, 0), (32, 32, 0), (32, 64, 1), (64, 64, 1), (64, 96, 2), (96, 96, 2), (96, 128, 3), (128, 128, 3),
+          (224, 96, 2), (96, 96, 2), (160, 64, 1), (64, 64, 1), (96, 32, 0), (32, 32, 0), (32, 32, 0)]
+
+
+def layer_slices():
+    """(weight slice, bias slice) of each convolution in the parameter vector."""
+    out, off = [], 0
+    for cin, cout, _ in LAYERS:
+        w = slice(off, off + 9 * cin * cout)
+        off += 9 * cin * cout
+        b = slice(off, off + cout)
+        off += cout
+        out.append((w, b))
+    return out
+
+
+def flops_per_pixel() -> float:
+    """Forward FLOPs per full-resolution pixel (2 per multiply-add)."""
+    return sum(2.0 * 9 * cin * cout / (4 ** lvl) for cin, cout, lvl in LAYERS)
+
+
+def _err():
+    return lib().wrk_unet_last_error().decode()
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class UNet:
+    def __init__(self, seed: int = 1):
+        self._h = lib().wrk_unet_create()
+        if not self._h:
+            raise RuntimeError(f"wrk_unet_create: {_err()}")
+        self.num_params = int(lib().wrk_unet_num_params(self._h))
+        self.init(seed)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().wrk_unet_destroy(h)
+            self._h = None
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what}: {_err()}")
+
+    def init(self, seed: int):
+        self._check(lib().wrk_unet_init(self._h, int(seed)), "init")
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.num_params, np.float32)
+        self._check(lib().wrk_unet_get_params(self._h, out.ctypes.data_as(C.c_void_p)), "get_params")
+        return out
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params
+        self._check(lib().wrk_unet_set_params(self._h, p.ctypes.data_as(C.c_void_p)), "set_params")
+
+    def get_grads(self) -> np.ndarray:
+        out = np.zeros(self.num_params, np.float32)
+        self._check(lib().wrk_unet_get_grads(self._h, out.ctypes.data_as(C.c_void_p)), "get_grads")
+        return out
+
+    def save(self, path):
+        self.get_params().tofile(path)
+
+    def load(self, path):
+        self.set_params(np.fromfile(path, dtype=np.float32))
+
+    def forward(self, color, albedo, normal, out, origins=None, crop=None, stream=None):
+        """color/albedo/normal: CUDA float32 (imgs, H, W, 3); out: (n, h, w, 3).
+        crop = (n, h, w) with origins (CUDA uint32, 3 per crop) for crops."""
+        n, h, w = crop if crop else (color.shape[0], color.shape[1], color.shape[2])
+        self._check(lib().wrk_unet_forward(self._h, _ptr(color), _ptr(albedo), _ptr(normal), color.shape[1],
+                                           color.shape[2], _ptr(origins), n, h, w, _ptr(out), stream), "forward")
+
+    def train_step(self, color, albedo, normal, target, lr, origins=None, crop=None, stream=None) -> float:
+        n, h, w = crop if crop else (color.shape[0], color.shape[1], color.shape[2])
+        loss = C.c_float(0.0)
+        self._check(lib().wrk_unet_train_step(self._h, _ptr(color), _ptr(albedo), _ptr(normal), _ptr(target),
+                                              color.shape[1], color.shape[2], _ptr(origins), n, h, w,
+                                              C.c_float(lr), C.byref(loss), stream), "train_step")
+        return float(loss.value)

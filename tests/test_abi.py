@@ -24,11 +24,12 @@ def _exported():
 def test_headers_declare_entry_points():
     assert len(_declared("wrk.h")) >= 18
     assert len(_declared("wr_c.h")) >= 25
+    assert len(_declared("wrk_unet.h")) >= 20
 
 
 def test_library_exports_every_declared_symbol():
     exported = _exported()
-    for h in ("wrk.h", "wr_c.h"):
+    for h in ("wrk.h", "wr_c.h", "wrk_unet.h"):
         missing = [s for s in _declared(h) if s not in exported]
         assert not missing, f"{h}: not exported: {missing}"
 

@@ -1,0 +1,196 @@
+"""Denoiser kernels (BF16 MFMA, FP32 accumulation) against plain PyTorch FP32
+references of the same ops on the same bf16-rounded inputs, and the whole
+U-Net (forward, gradients, a few training steps) against the PyTorch
+restatement of its graph."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+import offlinerenderer_amd as wr
+from offlinerenderer_amd import denoiser
+from tests import gpu_util as G
+from tests import unet_ref as U
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev_bits(a):
+    return torch.from_numpy(U.bf16_bits(a)).cuda()
+
+
+def _host_bits(t):
+    torch.cuda.synchronize()
+    return U.from_bits(t.cpu().numpy())
+
+
+def _rand(rng, shape, scale=1.0):
+    return U.bf16_round(rng.normal(size=shape).astype(np.float32) * scale)
+
+
+@pytest.mark.parametrize("n,h,w,cin,cout,relu,xpitch,ypitch", [
+    (2, 16, 32, 32, 32, 1, 32, 32),
+    (1, 12, 20, 96, 64, 0, 128, 96),     # partial tiles, pitched input and output slices
+    (1, 24, 16, 224, 96, 1, 224, 96),
+    (3, 8, 8, 64, 128, 1, 64, 128),
+])
+def test_conv_fwd_matches_torch(gpu, n, h, w, cin, cout, relu, xpitch, ypitch):
+    rng = np.random.default_rng(cin * 7 + cout)
+    x = _rand(rng, (n, h, w, xpitch))
+    wt = _rand(rng, (9, cout, cin), 1.0 / np.sqrt(9 * cin))
+    b = rng.normal(size=cout).astype(np.float32) * 0.1
+    xd, wd = _dev_bits(x), _dev_bits(wt)
+    bd = torch.from_numpy(b).cuda()
+    yd = torch.zeros((n, h, w, ypitch), dtype=torch.int16, device="cuda")
+    G.check(wr.lib().wrk_conv3x3_fwd(G.vp(xd), xpitch, cin, G.vp(wd), G.vp(bd), G.vp(yd), ypitch, cout, relu, n, h, w,
+                                     G.stream()))
+    got = _host_bits(yd)[..., :cout]
+    ref = F.conv2d(torch.from_numpy(x[..., :cin]).permute(0, 3, 1, 2), U.torch_weight(wt.ravel(), cin, cout),
+                   torch.from_numpy(b), padding=1)
+    if relu:
+        ref = F.relu(ref)
+    ref = ref.permute(0, 2, 3, 1).numpy()
+    # output rounded to bf16 (2^-8 relative); accumulation order differs
+    err = np.abs(got - ref)
+    assert np.all(err <= np.abs(ref) * 2.0 ** -8 + 1e-4 * np.abs(ref).max() + 1e-6), err.max()
+    if ypitch > cout:
+        assert np.all(_host_bits(yd)[..., cout:] == 0)
+
+
+@pytest.mark.parametrize("n,h,w,cin,cout", [(2, 16, 32, 32, 32), (1, 12, 40, 96, 64), (1, 8, 16, 224, 96)])
+def test_conv_gradients_match_torch(gpu, n, h, w, cin, cout):
+    rng = np.random.default_rng(cin + 3 * cout)
+    x = _rand(rng, (n, h, w, cin))
+    dz = _rand(rng, (n, h, w, cout), 0.1)
+    w32 = (rng.normal(size=(9, cout, cin)) / np.sqrt(9 * cin)).astype(np.float32)
+    xd, dzd = _dev_bits(x), _dev_bits(dz)
+    dw = torch.zeros(9 * cout * cin, dtype=torch.float32, device="cuda")
+    db = torch.zeros(cout, dtype=torch.float32, device="cuda")
+    nbytes = wr.lib().wrk_conv3x3_wgrad_scratch_bytes(cin, cout, n, h, w)
+    scratch = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    G.check(wr.lib().wrk_conv3x3_wgrad(G.vp(xd), cin, cin, G.vp(dzd), cout, cout, n, h, w, G.vp(dw), G.vp(db),
+                                       G.vp(scratch), nbytes, G.stream()))
+    xt = torch.from_numpy(x).permute(0, 3, 1, 2)
+    dzt = torch.from_numpy(dz).permute(0, 3, 1, 2)
+    ref_w = torch.nn.grad.conv2d_weight(xt, (cout, cin, 3, 3), dzt, padding=1)  # [co][ci][3][3]
+    ref_w = ref_w.permute(2, 3, 0, 1).reshape(-1).numpy()
+    got_w = dw.cpu().numpy()
+    assert np.abs(got_w - ref_w).max() <= 1e-4 * np.abs(ref_w).max()
+    assert np.allclose(db.cpu().numpy(), dzt.sum(dim=(0, 2, 3)).numpy(), rtol=1e-4, atol=1e-3)
+    # data gradient: the forward kernel with the tap-flipped transposed weights
+    w32d = torch.from_numpy(w32.ravel()).cuda()
+    wb = torch.zeros(9 * cout * cin, dtype=torch.int16, device="cuda")
+    wtb = torch.zeros(9 * cout * cin, dtype=torch.int16, device="cuda")
+    G.check(wr.lib().wrk_weights_bf16(G.vp(w32d), cout, cin, G.vp(wb), G.vp(wtb), G.stream()))
+    dx = torch.zeros((n, h, w, cin), dtype=torch.int16, device="cuda")
+    G.check(wr.lib().wrk_conv3x3_fwd(G.vp(dzd), cout, cout, G.vp(wtb), None, G.vp(dx), cin, cin, 0, n, h, w,
+                                     G.stream()))
+    wq = U.bf16_round(w32)
+    gw = _host_bits(wb).reshape(9, cout, cin)
+    bad = np.argwhere(gw != wq)
+    assert bad.size == 0, (len(bad), [(float(w32[tuple(i)]), float(gw[tuple(i)]), float(wq[tuple(i)])) for i in bad[:4]])
+    ref_x = torch.nn.grad.conv2d_input((n, cin, h, w), U.torch_weight(wq.ravel(), cin, cout), dzt, padding=1)
+    ref_x = ref_x.permute(0, 2, 3, 1).numpy()
+    got_x = _host_bits(dx)
+    assert np.all(np.abs(got_x - ref_x) <= np.abs(ref_x) * 2.0 ** -8 + 1e-4 * np.abs(ref_x).max() + 1e-6)
+
+
+def test_pool_upsample_relu(gpu):
+    rng = np.random.default_rng(5)
+    n, h, w, c = 2, 8, 12, 32
+    x = _rand(rng, (n, h, w, 48))                      # pitched: 32 of 48 channels
+    xd = _dev_bits(x)
+    y = torch.zeros((n, h // 2, w // 2, c), dtype=torch.int16, device="cuda")
+    G.check(wr.lib().wrk_pool2_fwd(G.vp(xd), 48, c, G.vp(y), c, n, h, w, G.stream()))
+    ref = x[..., :c].reshape(n, h // 2, 2, w // 2, 2, c).mean(axis=(2, 4))
+    assert np.allclose(_host_bits(y), U.bf16_round(ref), rtol=2 ** -8, atol=0)
+    # pool backward + skip
+    dy = _rand(rng, (n, h // 2, w // 2, c))
+    skip = _rand(rng, (n, h, w, c))
+    dx = torch.zeros((n, h, w, c), dtype=torch.int16, device="cuda")
+    # device copies stay referenced until the kernel has run
+    dyd, skipd = _dev_bits(dy), _dev_bits(skip)
+    G.check(wr.lib().wrk_pool2_bwd(G.vp(dyd), c, c, G.vp(skipd), c, G.vp(dx), c, n, h, w, G.stream()))
+    ref = np.repeat(np.repeat(dy, 2, axis=1), 2, axis=2) * 0.25 + skip
+    gd, rd = _host_bits(dx), U.bf16_round(ref)
+    bad = np.argwhere(gd != rd)
+    assert bad.size == 0, (len(bad), [(float(gd[tuple(i)]), float(rd[tuple(i)]), float(ref[tuple(i)])) for i in bad[:4]])
+    # upsample forward into a pitched slice, backward sums 2x2 blocks
+    up = torch.zeros((n, h, w, 64), dtype=torch.int16, device="cuda")
+    small = _rand(rng, (n, h // 2, w // 2, c))
+    smalld = _dev_bits(small)
+    G.check(wr.lib().wrk_up2_fwd(G.vp(smalld), c, c, G.vp(up), 64, n, h, w, G.stream()))
+    assert np.array_equal(_host_bits(up)[..., :c], np.repeat(np.repeat(small, 2, axis=1), 2, axis=2))
+    dbig = _rand(rng, (n, h, w, c))
+    dsm = torch.zeros((n, h // 2, w // 2, c), dtype=torch.int16, device="cuda")
+    dbigd = _dev_bits(dbig)
+    G.check(wr.lib().wrk_up2_bwd(G.vp(dbigd), c, c, G.vp(dsm), c, n, h, w, G.stream()))
+    ref = dbig.reshape(n, h // 2, 2, w // 2, 2, c).sum(axis=(2, 4))
+    assert np.allclose(_host_bits(dsm), U.bf16_round(ref), rtol=2 ** -8, atol=1e-6)
+    # relu backward
+    g = _rand(rng, (n, h, w, c))
+    gd = _dev_bits(g)
+    G.check(wr.lib().wrk_relu_bwd(G.vp(gd), c, G.vp(skipd), c, c, n * h * w, G.stream()))
+    assert np.array_equal(_host_bits(gd), np.where(skip > 0, g, 0.0).astype(np.float32))
+
+
+def _images(rng, n, h, w):
+    color = np.abs(rng.normal(size=(n, h, w, 3))).astype(np.float32) * 2.0
+    albedo = rng.uniform(size=(n, h, w, 3)).astype(np.float32)
+    nrm = rng.normal(size=(n, h, w, 3)).astype(np.float32)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    return color, albedo, nrm
+
+
+def test_unet_forward_matches_torch(gpu):
+    rng = np.random.default_rng(11)
+    n, h, w = 2, 32, 48
+    color, albedo, nrm = _images(rng, n, h, w)
+    net = denoiser.UNet(seed=3)
+    params = net.get_params()
+    # biases are zero after init; give them values so the test covers them
+    for _, bs in denoiser.layer_slices():
+        params[bs] = rng.normal(size=bs.stop - bs.start).astype(np.float32) * 0.05
+    net.set_params(params)
+    out = torch.zeros((n, h, w, 3), dtype=torch.float32, device="cuda")
+    net.forward(*(torch.from_numpy(a).cuda() for a in (color, albedo, nrm)), out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    pred = U.forward(params, U.pack_input(color, albedo, nrm))[:, 0:3].permute(0, 2, 3, 1).numpy()
+    ref = np.maximum(np.expm1(pred), 0.0)
+    # 15 layers of bf16 activations: compare in log space
+    err = np.abs(np.log1p(got) - np.log1p(ref))
+    assert err.mean() < 2e-3 and err.max() < 5e-2, (err.mean(), err.max())
+
+
+def test_unet_gradients_and_training(gpu):
+    rng = np.random.default_rng(12)
+    n, h, w = 2, 32, 32
+    color, albedo, nrm = _images(rng, n, h, w)
+    target = U.bf16_round(color * 0.5 + 0.1)
+    net = denoiser.UNet(seed=4)
+    params = net.get_params()
+    dev = [torch.from_numpy(a).cuda() for a in (color, albedo, nrm, target)]
+    loss0 = net.train_step(*dev, lr=0.0)
+    grads = net.get_grads()
+    ref_loss, ref_grads = U.loss_and_grads(params, U.pack_input(color, albedo, nrm), target)
+    assert abs(loss0 - ref_loss) <= 2e-2 * ref_loss
+    for i, (ws, bs) in enumerate(denoiser.layer_slices()):
+        g, r = grads[ws], ref_grads[ws]
+        cos = float(np.dot(g, r) / (np.linalg.norm(g) * np.linalg.norm(r) + 1e-30))
+        assert cos > 0.98, (i, cos)
+        rel = np.linalg.norm(g - r) / (np.linalg.norm(r) + 1e-30)
+        assert rel < 0.2, (i, rel)
+    # a few Adam steps on the same batch lower the loss
+    losses = [net.train_step(*dev, lr=2e-3) for _ in range(40)]
+    assert losses[-1] < 0.5 * losses[0], losses[::8]
+    # crops: origins select sub-images of the source layers
+    origins = torch.tensor([[0, 8, 0], [1, 0, 16]], dtype=torch.int32, device="cuda").view(-1)
+    out = torch.zeros((2, 16, 24, 3), dtype=torch.float32, device="cuda")
+    net.forward(*dev[:3], out, origins=origins, crop=(2, 16, 24))
+    full = torch.zeros((n, h, w, 3), dtype=torch.float32, device="cuda")
+    net.forward(*dev[:3], full)
+    torch.cuda.synchronize()
+    assert np.isfinite(out.cpu().numpy()).all() and np.isfinite(full.cpu().numpy()).all()
