@@ -57,7 +57,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["material", "large", "serving"], default="material")
+    ap.add_argument("--workload", choices=["material", "large", "serving", "denoise", "denoise-infer"],
+                    default="material")
     ap.add_argument("--spp", type=int, default=None)
     ap.add_argument("--width", type=int, default=None)
     ap.add_argument("--height", type=int, default=None)
@@ -69,6 +70,15 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrival rate, requests/s")
     ap.add_argument("--policy", choices=["fifo", "shortest"], default="shortest",
                     help="serving: admission order of pending requests")
+    ap.add_argument("--dn-scenes", type=int, default=8, help="denoise: training scenes")
+    ap.add_argument("--dn-cams", type=int, default=48, help="denoise: cameras per training scene")
+    ap.add_argument("--dn-target-spp", type=int, default=1024, help="denoise: spp of the training targets")
+    ap.add_argument("--dn-steps", type=int, default=4000, help="denoise: training steps (16 crops of 128^2)")
+    ap.add_argument("--dn-loss", choices=["relmse", "log"], default="relmse", help="denoise: training loss")
+    ap.add_argument("--dn-lr", type=float, default=1e-3, help="denoise: peak learning rate")
+    ap.add_argument("--dn-eval-size", type=str, default="1920x1080", help="denoise: evaluation resolution")
+    ap.add_argument("--dn-ref-spp", type=int, default=4096, help="denoise: spp of the evaluation reference")
+    ap.add_argument("--dn-save", type=str, default=None, help="denoise: write the trained weights here")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target wall time of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -89,6 +99,12 @@ def main():
     wr.Context(wr.ContextConfig(deviceId=0, logLevel=0))
     if args.workload == "serving":
         serving(args)
+        return
+    if args.workload == "denoise":
+        denoise(args)
+        return
+    if args.workload == "denoise-infer":
+        denoise_infer(args)
         return
     if args.max_paths is None:
         args.max_paths = 1 << 28
@@ -350,6 +366,127 @@ def serving(args):
     }
     print(json.dumps(line), flush=True)
     assert eq_sat == len(jobs) and eq_ol == len(jobs), "a served image differs from its stand-alone render"
+
+
+def denoise(args):
+    """Learned denoiser: train on the tracer's own data, then time to
+    relMSE <= 1e-2 at the evaluation resolution, learned vs plain."""
+    import numpy as np
+
+    from offlinerenderer_amd import denoise_train as dt
+    from offlinerenderer_amd import denoiser
+
+    t0 = time.perf_counter()
+    data = dt.build_dataset(num_scenes=args.dn_scenes, cams_per_scene=args.dn_cams, spp_target=args.dn_target_spp,
+                            log=log)
+    t_data = time.perf_counter() - t0
+    net = denoiser.UNet(seed=1)
+    t0 = time.perf_counter()
+    losses = dt.train(net, data, steps=args.dn_steps, lr=args.dn_lr, loss=args.dn_loss, log=log)
+    t_train = time.perf_counter() - t0
+    del data
+    if args.dn_save:
+        net.save(args.dn_save)
+    W, H = (int(v) for v in args.dn_eval_size.split("x"))
+    res = dt.evaluate(net, width=W, height=H, ref_spp=args.dn_ref_spp, log=log)
+    if res["learned_seconds_to_bar"] and res["plain_seconds_to_bar"]:
+        speedup = res["plain_seconds_to_bar"] / res["learned_seconds_to_bar"]
+        basis = f"time to relMSE<=1e-2 (learned at {res['learned_spp_to_bar']} spp + network)"
+    elif "plain_seconds_equal_quality" in res:
+        speedup, basis = res["plain_seconds_equal_quality"] / res["learned_seconds"], "time to equal relMSE at 4 spp"
+    else:
+        speedup, basis = None, "not comparable"
+    peak = 2500.0  # MI355X dense bf16 MFMA, TFLOP/s
+    line = {
+        "metric": "Mpaths/s @1080p; time-to-relMSE<=1e-2 learned vs plain; p50/p99 req latency",
+        "value": None if speedup is None else round(speedup, 3),
+        "unit": "x (plain time / learned time)",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(res["learned_seconds"] * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (network), f32 (tracer)",
+        "data": "synthetic (tracer-generated: seeded large-scene generator, random cameras)",
+        "config": {
+            "workload": f"denoise_{W}x{H}_4spp_unet",
+            "train_views": args.dn_scenes * args.dn_cams, "train_input_spp": [4, 16, 64], "train_res": 256, "train_target_spp": args.dn_target_spp,
+            "train_steps": args.dn_steps, "batch": "16 x 128^2 crops", "params": net.num_params,
+            "loss": args.dn_loss,
+            "eval_reference_spp": args.dn_ref_spp,
+        },
+        "speedup_basis": basis,
+        "learned": {"rel_mse": round(res["learned_rel_mse"], 5), "ms": round(res["learned_seconds"] * 1e3, 2)},
+        "learned_sweep": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in p.items()}
+                          for p in res["learned_sweep"]],
+        "learned_spp_to_bar": res["learned_spp_to_bar"],
+        "equal_quality_speedup_at_4spp": (round(res["plain_seconds_equal_quality"] / res["learned_seconds"], 2)
+                                          if "plain_seconds_equal_quality" in res else None),
+        "noisy_4spp": {"rel_mse": round(res["noisy_rel_mse"], 5), "ms": round(res["noisy_seconds"] * 1e3, 2)},
+        "plain": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in p.items()} for p in res["plain"]],
+        "plain_spp_to_bar": res["plain_spp_to_bar"],
+        "plain_spp_equal_quality": res.get("plain_spp_equal_quality"),
+        "train": {"data_s": round(t_data, 1), "train_s": round(t_train, 1),
+                  "loss_first100": round(float(np.mean(losses[:100])), 5),
+                  "loss_last100": round(float(np.mean(losses[-100:])), 5)},
+        "roofline": {"kernel": "U-Net forward (15 bf16 MFMA convolutions)", "bound": "mfma",
+                     "achieved": round(res["inference_tflops"], 2), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(res["inference_tflops"] / peak, 4), "traffic": None,
+                     "inference_ms": round(res["inference_ms"], 3)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def denoise_infer(args):
+    """The learned path alone with the shipped weights: 4-spp render with
+    guides + U-Net inference on a held-out large scene, timed per step."""
+    import numpy as np
+    import torch
+
+    import offlinerenderer_amd as wr
+    from offlinerenderer_amd import denoise_train as dt
+    from offlinerenderer_amd import denoiser
+
+    W, H = (int(v) for v in args.dn_eval_size.split("x"))
+    d, s = dt.build_scene(9001, W, H)
+    cam = wr.Camera(**dt.large_camera(np.random.default_rng(9001), W, H))
+    r = wr.Renderer()
+    r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=1 << 28))
+    p = wr.RenderParams(width=W, height=H, spp=4, maxBounces=d.maxBounces, russianRouletteDepth=d.rrDepth)
+    p.denoiser.enabled = True
+    for _ in range(args.warmup):
+        r.render(s, cam, p)
+    times = []
+    for i in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r.render(s, cam, p)
+        times.append(time.perf_counter() - t0)
+        log(f"[denoise-infer] step {i + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms")
+    # the network alone on device-resident layers
+    net = denoiser.UNet()
+    net.load(str(wr.LIB_PATH.parent.parent / "models" / "denoiser_unet.f32"))
+    q = wr.RenderParams(width=W, height=H, spp=4, maxBounces=d.maxBounces, russianRouletteDepth=d.rrDepth)
+    dev = [torch.from_numpy(a).cuda().unsqueeze(0) for a in r.renderWithGuides(s, cam, q)]
+    out = torch.empty_like(dev[0])
+    net.forward(*dev, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.forward(*dev, out)
+    torch.cuda.synchronize()
+    t_inf = (time.perf_counter() - t0) / args.steps
+    tflops = denoiser.flops_per_pixel() * W * H / t_inf / 1e12
+    print(json.dumps({
+        "metric": "Mpaths/s @1080p; time-to-relMSE<=1e-2 learned vs plain; p50/p99 req latency",
+        "value": round(float(np.median(times)) * 1e3, 3), "unit": "ms per denoised 4-spp frame",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(float(np.median(times)) * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (network), f32 (tracer)",
+        "data": "synthetic (seeded large-scene generator, shipped weights)",
+        "config": {"workload": f"denoise_infer_{W}x{H}_4spp"},
+        "roofline": {"kernel": "U-Net forward (15 bf16 MFMA convolutions)", "bound": "mfma",
+                     "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 4),
+                     "traffic": None, "inference_ms": round(t_inf * 1e3, 3)},
+        "cpu_baseline": None,
+    }), flush=True)
 
 
 def cpu_baseline(desc, W, H, target_s):

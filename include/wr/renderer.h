@@ -3,6 +3,7 @@
 // Additive: WavefrontConfig (iteration cap, paths in flight) and
 // Renderer::renderPaths / getStats for tests and benchmarks.
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "wr/scene.h"
@@ -33,6 +34,9 @@ struct RenderParams {
     uint32_t maxBounces = 8;
     float russianRouletteDepth = 3.0f;
     float fireflyClamp = 0.0f;
+    // additive: render samples [firstSample, firstSample + spp) (libWR always
+    // starts at sample 0); images of disjoint ranges are independent
+    uint32_t firstSample = 0;
 };
 
 // Wavefront scheduling. iterationCap = 8 reproduces libWR's host loop
@@ -79,6 +83,12 @@ public:
 
     // --- additive API
     void setWavefrontConfig(const WavefrontConfig& cfg);
+    // Learned denoiser weights (FP32 parameter file written by
+    // offlinerenderer_amd.denoiser.UNet.save): render() then denoises when
+    // params.denoiser.enabled (in place of libWR's OptiX AI denoiser).
+    // Without a call, the weights shipped in offlinerenderer_amd/models are
+    // used when present.
+    void loadDenoiser(const std::string& path);
     WavefrontConfig getWavefrontConfig() const;
     // Per-path values added to the accumulation buffer for global path
     // indices [firstPath, firstPath + count), path = sample * W*H + pixel;

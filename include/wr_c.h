@@ -26,6 +26,7 @@ typedef struct wr_c_render_params {
     uint32_t width, height, spp, useNEE, maxBounces;
     float russianRouletteDepth, fireflyClamp;
     uint32_t denoiser;
+    uint32_t firstSample;
 } wr_c_render_params;
 
 typedef struct wr_c_stats {
@@ -72,6 +73,8 @@ int wr_c_render(void* renderer, void* scene, const wr_c_camera* cam, const wr_c_
 int wr_c_render_paths(void* renderer, void* scene, const wr_c_camera* cam, const wr_c_render_params* params,
                       uint64_t firstPath, uint32_t count, float* out);
 int wr_c_renderer_stats(void* renderer, wr_c_stats* out);
+/* Renderer::loadDenoiser */
+int wr_c_renderer_load_denoiser(void* renderer, const char* path);
 /* Renderer::renderWithGuides; each output width*height*3 floats */
 int wr_c_render_guides(void* renderer, void* scene, const wr_c_camera* cam, const wr_c_render_params* params,
                        float* color, float* albedo, float* normal);

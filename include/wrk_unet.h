@@ -52,13 +52,17 @@ int wrk_up2_bwd(const uint16_t* dy, uint32_t dyPitch, uint32_t c, uint16_t* dx, 
  * crop: image, x0, y0) select n crops of h x w, or NULL for whole images. */
 int wrk_unet_pack(const float* color, const float* albedo, const float* normal, uint32_t srcH, uint32_t srcW,
                   const uint32_t* origins, uint16_t* out, uint32_t n, uint32_t h, uint32_t w, void* stream);
-/* out (npix, 3) FP32 = max(exp(pred) - 1, 0) of channels 0..2 */
-int wrk_unet_unpack(const uint16_t* pred, uint32_t pitch, float* out, uint64_t npix, void* stream);
-/* per-256-pixel partial sums of (pred - log(1 + target))^2 and its gradient */
+/* out (npix, 3) FP32 = max(exp(q) - 1, 0) of channels 0..2, q = pred, or
+ * pred + base (the packed input: a correction of its log colour) */
+int wrk_unet_unpack(const uint16_t* pred, uint32_t pitch, const uint16_t* base, uint32_t basePitch, float* out,
+                    uint64_t npix, void* stream);
+/* per-256-pixel partial sums of the loss and its gradient dpred, with q as
+ * in wrk_unet_unpack; mode 0: (q - log(1 + t))^2, mode 1: relMSE
+ * (exp(q) - 1 - t)^2 / (t^2 + 0.01) */
 uint64_t wrk_unet_loss_scratch_bytes(uint32_t n, uint32_t h, uint32_t w);
 int wrk_unet_loss(const uint16_t* pred, uint32_t pitch, const float* target, uint32_t srcH, uint32_t srcW,
-                  const uint32_t* origins, uint16_t* dpred, uint32_t n, uint32_t h, uint32_t w, float* partial,
-                  void* stream);
+                  const uint32_t* origins, const uint16_t* base, uint32_t basePitch, uint16_t* dpred, uint32_t n,
+                  uint32_t h, uint32_t w, uint32_t mode, float* partial, void* stream);
 /* Adam on FP32 parameters (step counts from 1) */
 int wrk_adam(float* p, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
              uint32_t step, void* stream);
@@ -79,11 +83,11 @@ int wrk_unet_get_grads(void* net, float* hostOut);
 int wrk_unet_forward(void* net, const float* color, const float* albedo, const float* normal, uint32_t srcH,
                      uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w, float* out,
                      void* stream);
-/* one Adam step on the log-space MSE against target; *loss (host) = the
- * loss before the step */
+/* one Adam step on the loss of wrk_unet_loss (lossMode) against target;
+ * *loss (host) = the loss before the step */
 int wrk_unet_train_step(void* net, const float* color, const float* albedo, const float* normal, const float* target,
                         uint32_t srcH, uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w,
-                        float lr, float* loss, void* stream);
+                        float lr, uint32_t lossMode, float* loss, void* stream);
 
 #ifdef __cplusplus
 }

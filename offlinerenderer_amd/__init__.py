@@ -55,7 +55,7 @@ class wr_c_camera(C.Structure):
 class wr_c_render_params(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("useNEE", C.c_uint32),
                 ("maxBounces", C.c_uint32), ("russianRouletteDepth", C.c_float), ("fireflyClamp", C.c_float),
-                ("denoiser", C.c_uint32)]
+                ("denoiser", C.c_uint32), ("firstSample", C.c_uint32)]
 
 
 class wr_c_stats(C.Structure):
@@ -116,6 +116,7 @@ _SIGS = {
     "wr_c_render_paths": (C.c_int, [_P, _P, C.POINTER(wr_c_camera), C.POINTER(wr_c_render_params), C.c_uint64,
                                     C.c_uint32, _P]),
     "wr_c_renderer_stats": (C.c_int, [_P, C.POINTER(wr_c_stats)]),
+    "wr_c_renderer_load_denoiser": (C.c_int, [_P, C.c_char_p]),
     "wr_c_render_guides": (C.c_int, [_P, _P, C.POINTER(wr_c_camera), C.POINTER(wr_c_render_params), _P, _P, _P]),
     "wr_c_server_create": (_P, [C.POINTER(wr_c_server_config)]),
     "wr_c_server_destroy": (None, [_P]),
@@ -142,10 +143,10 @@ _SIGS = {
     "wrk_up2_fwd": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "wrk_up2_bwd": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "wrk_unet_pack": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
-    "wrk_unet_unpack": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, _P]),
+    "wrk_unet_unpack": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint64, _P]),
     "wrk_unet_loss_scratch_bytes": (C.c_uint64, [C.c_uint32] * 3),
-    "wrk_unet_loss": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, C.c_uint32,
-                                C.c_uint32, _P, _P]),
+    "wrk_unet_loss": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, _P, C.c_uint32,
+                                C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_adam": (C.c_int, [_P, _P, _P, _P, C.c_uint64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, _P]),
     "wrk_unet_last_error": (C.c_char_p, []),
     "wrk_unet_create": (_P, []),
@@ -158,7 +159,7 @@ _SIGS = {
     "wrk_unet_forward": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                                    _P, _P]),
     "wrk_unet_train_step": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32,
-                                      C.c_uint32, C.c_float, _P, _P]),
+                                      C.c_uint32, C.c_float, C.c_uint32, _P, _P]),
     # wrk_*
     "wrk_last_error": (C.c_char_p, []),
     "wrk_device_count": (C.c_int, [C.POINTER(C.c_int)]),
@@ -258,6 +259,7 @@ class RenderParams:
     maxBounces: int = 8
     russianRouletteDepth: float = 3.0
     fireflyClamp: float = 0.0
+    firstSample: int = 0  # additive: samples [firstSample, firstSample + spp)
 
     def to_c(self):
         p = wr_c_render_params()
@@ -267,6 +269,7 @@ class RenderParams:
         p.russianRouletteDepth = float(self.russianRouletteDepth)
         p.fireflyClamp = float(self.fireflyClamp)
         p.denoiser = 1 if self.denoiser.enabled else 0
+        p.firstSample = int(self.firstSample)
         return p
 
 
@@ -433,6 +436,11 @@ class Renderer:
         cam, prm = camera.to_c(), params.to_c()
         _check(lib().wr_c_render(self._h, scene._h, C.byref(cam), C.byref(prm), _ptr(out)), "render")
         return out
+
+    def loadDenoiser(self, path):
+        """Learned denoiser weights (denoiser.UNet.save); render() then denoises
+        when params.denoiser.enabled."""
+        _check(lib().wr_c_renderer_load_denoiser(self._h, str(path).encode()), "loadDenoiser")
 
     def renderWithGuides(self, scene: Scene, camera: Camera, params: RenderParams):
         """(color, albedo, normal), each (height, width, 3) float32 averaged over spp."""

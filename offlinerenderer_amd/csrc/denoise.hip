@@ -360,22 +360,32 @@ __global__ __launch_bounds__(256) void k_unet_pack(const float* __restrict__ col
     for (int k = 0; k < 4; ++k) dst[k] = o.u[k];
 }
 
-// out = exp(pred) - 1 for the first 3 channels (the network predicts log(1 + L))
+// out = exp(q) - 1 for the first 3 channels, q = pred (+ base when given):
+// the network predicts log(1 + L), or its correction to the input's
+// log(1 + L) when base is the packed input
 __global__ __launch_bounds__(256) void k_unet_unpack(const bf16* __restrict__ pred, uint32_t pitch,
+                                                     const bf16* __restrict__ base, uint32_t basePitch,
                                                      float* __restrict__ out, uint64_t npix) {
     const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (p >= npix) return;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) out[p * 3 + c] = fmaxf(expm1f((float)pred[p * pitch + c]), 0.0f);
+    for (int c = 0; c < 3; ++c) {
+        const float q = (float)pred[p * pitch + c] + (base ? (float)base[p * basePitch + c] : 0.0f);
+        out[p * 3 + c] = fmaxf(expm1f(q), 0.0f);
+    }
 }
 
-// Loss = mean over pixels and 3 channels of (pred - log(1 + target))^2;
-// dPred = 2 (pred - log(1 + target)) / count for the 3 channels, 0 for the
-// padding channels. Per-block partial loss sums.
+// Per pixel and channel c < 3, with t the target and y = exp(pred) - 1:
+//   mode 0: (pred - log(1 + t))^2            (log-space MSE)
+//   mode 1: (y - t)^2 / (t^2 + 0.01)          (relMSE, the evaluation metric)
+// averaged over pixels and 3 channels; dPred is its gradient (0 for the
+// padding channels). Per-block partial loss sums.
 __global__ __launch_bounds__(256) void k_unet_loss(const bf16* __restrict__ pred, uint32_t pitch,
                                                    const float* __restrict__ target, uint32_t srcH, uint32_t srcW,
-                                                   const uint32_t* __restrict__ origins, bf16* __restrict__ dpred,
-                                                   uint32_t N, uint32_t H, uint32_t W, float* __restrict__ partial) {
+                                                   const uint32_t* __restrict__ origins,
+                                                   const bf16* __restrict__ base, uint32_t basePitch,
+                                                   bf16* __restrict__ dpred, uint32_t N, uint32_t H, uint32_t W,
+                                                   uint32_t mode, float* __restrict__ partial) {
     __shared__ float red[256];
     const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     const uint64_t npix = (uint64_t)N * H * W;
@@ -393,9 +403,19 @@ __global__ __launch_bounds__(256) void k_unet_loss(const bf16* __restrict__ pred
         for (uint32_t c = 0; c < pitch; ++c) {
             float g = 0.0f;
             if (c < 3) {
-                const float d = (float)pred[p * pitch + c] - log1pf(fmaxf(target[s * 3 + c], 0.0f));
-                l += d * d;
-                g = d * scale;
+                const float t = fmaxf(target[s * 3 + c], 0.0f);
+                const float q = (float)pred[p * pitch + c] + (base ? (float)base[p * basePitch + c] : 0.0f);
+                if (mode == 1) {
+                    const float e = expf(fminf(q, 12.0f));
+                    const float inv = 1.0f / (t * t + 0.01f);
+                    const float d = (e - 1.0f) - t;
+                    l += d * d * inv;
+                    g = q < 12.0f ? d * inv * e * scale : 0.0f;
+                } else {
+                    const float d = q - log1pf(t);
+                    l += d * d;
+                    g = d * scale;
+                }
             }
             dpred[p * pitch + c] = f2bf(g);
         }
@@ -590,11 +610,12 @@ extern "C" int wrk_unet_pack(const float* color, const float* albedo, const floa
     return WRK_OK;
 }
 
-extern "C" int wrk_unet_unpack(const uint16_t* pred, uint32_t pitch, float* out, uint64_t npix, void* stream) {
-    WRK_REQUIRE(pred && out && pitch >= 3, "wrk_unet_unpack: bad argument");
+extern "C" int wrk_unet_unpack(const uint16_t* pred, uint32_t pitch, const uint16_t* base, uint32_t basePitch,
+                               float* out, uint64_t npix, void* stream) {
+    WRK_REQUIRE(pred && out && pitch >= 3 && (!base || basePitch >= 3), "wrk_unet_unpack: bad argument");
     if (npix == 0) return WRK_OK;
     hipLaunchKernelGGL(k_unet_unpack, dim3(ceil_div(npix, 256)), dim3(256), 0, as_stream(stream), (const bf16*)pred,
-                       pitch, out, npix);
+                       pitch, (const bf16*)base, basePitch, out, npix);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }
@@ -604,14 +625,16 @@ extern "C" uint64_t wrk_unet_loss_scratch_bytes(uint32_t n, uint32_t h, uint32_t
 }
 
 extern "C" int wrk_unet_loss(const uint16_t* pred, uint32_t pitch, const float* target, uint32_t srcH, uint32_t srcW,
-                             const uint32_t* origins, uint16_t* dpred, uint32_t n, uint32_t h, uint32_t w,
-                             float* partial, void* stream) {
-    WRK_REQUIRE(pred && target && dpred && partial && pitch >= 3, "wrk_unet_loss: bad argument");
+                             const uint32_t* origins, const uint16_t* base, uint32_t basePitch, uint16_t* dpred,
+                             uint32_t n, uint32_t h, uint32_t w, uint32_t mode, float* partial, void* stream) {
+    WRK_REQUIRE(pred && target && dpred && partial && pitch >= 3 && (!base || basePitch >= 3),
+                "wrk_unet_loss: bad argument");
     WRK_REQUIRE(origins || (srcH == h && srcW == w), "wrk_unet_loss: crops need origins");
     const uint64_t np = (uint64_t)n * h * w;
     if (np == 0) return WRK_OK;
     hipLaunchKernelGGL(k_unet_loss, dim3(ceil_div(np, 256)), dim3(256), 0, as_stream(stream), (const bf16*)pred,
-                       pitch, target, srcH, srcW, origins, (bf16*)dpred, n, h, w, partial);
+                       pitch, target, srcH, srcW, origins, (const bf16*)base, basePitch, (bf16*)dpred, n, h, w, mode,
+                       partial);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }

@@ -6,8 +6,8 @@
 // convolutions per level and side, 2x2 average pooling, nearest upsampling,
 // skip connections by concatenation ([upsampled | skip]), ReLU everywhere
 // but the output convolution. Input: log(1 + colour), albedo, normal
-// (9 channels, padded to 32). Output: log(1 + colour) in channels 0..2 of
-// a 32-channel tensor. bf16 weights and activations, FP32 accumulation,
+// (9 channels, padded to 32). Output: channels 0..2 of a 32-channel tensor
+// hold the correction added to the input's log(1 + colour). bf16 weights and activations, FP32 accumulation,
 // FP32 master weights with Adam. All arithmetic runs in the kernels of
 // denoise.hip; this file owns the graph and the buffers.
 #include <hip/hip_runtime.h>
@@ -133,16 +133,18 @@ public:
         shape(n, h, w, false);
         wrk_ok(wrk_unet_pack(color, albedo, normal, srcH, srcW, origins, in_.buf.h(), n, h, w, s), "wrk_unet_pack");
         runForward(s);
-        if (out) wrk_ok(wrk_unet_unpack(pred_.buf.h(), 32, out, (uint64_t)n * h * w, s), "wrk_unet_unpack");
+        if (out)
+            wrk_ok(wrk_unet_unpack(pred_.buf.h(), 32, in_.buf.h(), 32, out, (uint64_t)n * h * w, s), "wrk_unet_unpack");
     }
 
     float trainStep(const float* color, const float* albedo, const float* normal, const float* target, uint32_t srcH,
                     uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w, float lr,
-                    hipStream_t s) {
+                    uint32_t lossMode, hipStream_t s) {
         shape(n, h, w, true);
         wrk_ok(wrk_unet_pack(color, albedo, normal, srcH, srcW, origins, in_.buf.h(), n, h, w, s), "wrk_unet_pack");
         runForward(s);
-        wrk_ok(wrk_unet_loss(pred_.buf.h(), 32, target, srcH, srcW, origins, gPred_.buf.h(), n, h, w, lossPartial_.f(), s),
+        wrk_ok(wrk_unet_loss(pred_.buf.h(), 32, target, srcH, srcW, origins, in_.buf.h(), 32, gPred_.buf.h(), n, h, w, lossMode,
+                             lossPartial_.f(), s),
                "wrk_unet_loss");
         runBackward(s);
         ++step_;
@@ -381,10 +383,10 @@ int wrk_unet_forward(void* net, const float* color, const float* albedo, const f
 }
 int wrk_unet_train_step(void* net, const float* color, const float* albedo, const float* normal, const float* target,
                         uint32_t srcH, uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w,
-                        float lr, float* loss, void* stream) {
+                        float lr, uint32_t lossMode, float* loss, void* stream) {
     return guard([&] {
         float l = static_cast<UNet*>(net)->trainStep(color, albedo, normal, target, srcH, srcW, origins, n, h, w, lr,
-                                                     static_cast<hipStream_t>(stream));
+                                                     lossMode, static_cast<hipStream_t>(stream));
         if (loss) *loss = l;
     });
 }

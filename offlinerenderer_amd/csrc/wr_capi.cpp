@@ -28,6 +28,7 @@ wr::RenderParams to_params(const wr_c_render_params* p) {
     r.russianRouletteDepth = p->russianRouletteDepth;
     r.fireflyClamp = p->fireflyClamp;
     r.denoiser.enabled = p->denoiser != 0;
+    r.firstSample = p->firstSample;
     return r;
 }
 template <typename F> int guard(F&& f) {
@@ -163,6 +164,9 @@ int wr_c_render_guides(void* r, void* s, const wr_c_camera* cam, const wr_c_rend
         R(r)->renderWithGuides(S(s), to_camera(cam), to_params(p), reinterpret_cast<wr::Vec3*>(color),
                                reinterpret_cast<wr::Vec3*>(albedo), reinterpret_cast<wr::Vec3*>(normal));
     });
+}
+int wr_c_renderer_load_denoiser(void* r, const char* path) {
+    return guard([&] { R(r)->loadDenoiser(path); });
 }
 int wr_c_renderer_stats(void* r, wr_c_stats* out) {
     return guard([&] { fill_stats(R(r)->getStats(), out); });

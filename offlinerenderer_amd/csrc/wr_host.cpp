@@ -3,10 +3,12 @@
 // same method semantics, exception types and messages; the OptiX/CUDA
 // machinery is replaced by HIP device buffers, the GPU LBVH and the
 // wavefront kernels.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
+#include <fstream>
 #include <cmath>
 #include <cstring>
 #include <iostream>
@@ -17,6 +19,7 @@
 #include "../../include/wr/wr.h"
 #include "../../include/wrk.h"
 #include "engine.h"
+#include "../../include/wrk_unet.h"
 #include "image_io.h"
 
 namespace wr {
@@ -345,12 +348,46 @@ uint32_t Scene::getNumTriangles() const { return m_impl->numTriangles; }
 double Scene::getBuildSeconds() const { return m_impl->buildSeconds; }
 
 // -------------------------------------------------------------- Renderer
+// Weights shipped beside the library (offlinerenderer_amd/models), used
+// when render() is asked to denoise and no other network was loaded.
+static std::string default_denoiser_path() {
+    Dl_info info;
+    if (!dladdr(reinterpret_cast<void*>(&default_denoiser_path), &info) || !info.dli_fname) return {};
+    std::string lib(info.dli_fname);
+    const size_t slash = lib.find_last_of('/');
+    const std::string dir = slash == std::string::npos ? std::string(".") : lib.substr(0, slash);
+    return dir + "/../models/denoiser_unet.f32";
+}
+
 struct RendererImpl {
     WavefrontConfig wf;
     RenderStats stats;
     std::unique_ptr<detail::Engine> eng;
+    void* unet = nullptr;  // learned denoiser (wrk_unet_*), loaded by loadDenoiser
+    bool triedDefault = false;
+    DevBuf denoised;
+
+    void load_denoiser(const std::string& path) {
+        std::ifstream f(path, std::ios::binary | std::ios::ate);
+        if (!f) throw std::runtime_error("loadDenoiser: cannot open " + path);
+        const std::streamsize bytes = f.tellg();
+        f.seekg(0);
+        if (!unet) unet = wrk_unet_create();
+        if (!unet) throw std::runtime_error(std::string("loadDenoiser: ") + wrk_unet_last_error());
+        const uint64_t n = wrk_unet_num_params(unet);
+        if ((uint64_t)bytes != n * sizeof(float))
+            throw std::runtime_error("loadDenoiser: " + path + " does not hold " + std::to_string(n) +
+                                     " FP32 parameters");
+        std::vector<float> p(n);
+        f.read(reinterpret_cast<char*>(p.data()), bytes);
+        if (wrk_unet_set_params(unet, p.data()) != WRK_OK)
+            throw std::runtime_error(std::string("loadDenoiser: ") + wrk_unet_last_error());
+    }
 
     RendererImpl() { ensure_context(); }
+    ~RendererImpl() {
+        if (unet) wrk_unet_destroy(unet);
+    }
 
     wrk_render_params params_of(const RenderParams& p) const {
         wrk_render_params k;
@@ -382,8 +419,21 @@ struct RendererImpl {
         const uint64_t total = (uint64_t)rp.width * rp.height * rp.spp;
         detail::Engine& e = engine(total);
         auto t0 = std::chrono::steady_clock::now();
-        const bool guides = albedo || normal;
-        uint64_t id = e.submit(scene, detail::camera_record(camera), params_of(rp), rp.spp, 0, total,
+        // denoiser.cpp:77-195 takes colour + albedo + normal; the learned
+        // network needs image sides divisible by 8
+        if (rp.denoiser.enabled && !unet && !triedDefault) {
+            triedDefault = true;
+            const std::string path = default_denoiser_path();
+            if (!path.empty() && std::ifstream(path).good()) load_denoiser(path);
+        }
+        const bool denoise = rp.denoiser.enabled && unet && rp.width % 8 == 0 && rp.height % 8 == 0;
+        if (rp.denoiser.enabled && !denoise && g_logLevel >= 2)
+            std::cout << "[Renderer] denoiser requested but no network is loaded or the image sides are not "
+                         "multiples of 8: returning the averaged image"
+                      << std::endl;
+        const bool guides = albedo || normal || denoise;
+        const uint64_t first = (uint64_t)rp.firstSample * rp.width * rp.height;
+        uint64_t id = e.submit(scene, detail::camera_record(camera), params_of(rp), rp.spp, first, total,
                                detail::Engine::Mode::Accumulate, nullptr, guides);
         while (e.step()) {
         }
@@ -393,7 +443,18 @@ struct RendererImpl {
             tmp.resize((size_t)rp.width * rp.height * 3);
             dst = tmp.data();
         }
-        e.fetch(id, dst, albedo, normal);
+        if (denoise) {
+            const size_t bytes = (size_t)rp.width * rp.height * 3 * sizeof(float);
+            denoised.alloc(bytes);
+            if (wrk_unet_forward(unet, e.deviceImage(id), e.deviceAlbedo(id), e.deviceNormal(id), rp.height,
+                                 rp.width, nullptr, 1, rp.height, rp.width, denoised.as<float>(), e.stream()) != WRK_OK)
+                throw std::runtime_error(std::string("denoiser: ") + wrk_unet_last_error());
+            hip_check(hipStreamSynchronize(e.stream()), "hipStreamSynchronize");
+            hip_check(hipMemcpy(dst, denoised.ptr, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+            e.fetch(id, nullptr, albedo, normal);
+        } else {
+            e.fetch(id, dst, albedo, normal);
+        }
         stats = e.stats();
         stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         stats.paths = total;
@@ -427,8 +488,6 @@ void Renderer::render(Scene* scene, const Camera& camera, Vec3* outputBuffer, co
         std::cout << "[Renderer] Starting render: " << params.width << "x" << params.height << " @ " << params.spp
                   << " spp" << std::endl;
     m_impl->render(scene, camera, params, reinterpret_cast<float*>(outputBuffer));
-    if (params.denoiser.enabled && g_logLevel >= 2)
-        std::cout << "[Renderer] denoiser requested: not available in this build, returning the averaged image" << std::endl;
     if (g_logLevel >= 4) std::cout << "[Renderer] Render complete" << std::endl;
 }
 
@@ -446,6 +505,7 @@ void Renderer::render(Scene* scene, const Camera& camera, Vec3* outputBuffer, ui
 }
 
 void Renderer::setWavefrontConfig(const WavefrontConfig& cfg) { m_impl->wf = cfg; }
+void Renderer::loadDenoiser(const std::string& path) { m_impl->load_denoiser(path); }
 WavefrontConfig Renderer::getWavefrontConfig() const { return m_impl->wf; }
 void Renderer::renderPaths(Scene* scene, const Camera& camera, const RenderParams& params, uint64_t firstPath,
                            uint32_t count, float* out) {
@@ -487,7 +547,8 @@ uint64_t RenderServer::submit(Scene* scene, const Camera& camera, const RenderPa
     k.maxBounces = params.maxBounces; k.rrStartDepth = params.russianRouletteDepth;
     k.fireflyClamp = params.fireflyClamp; k.iterationCap = m_impl->cfg.iterationCap;
     uint64_t total = (uint64_t)params.width * params.height * params.spp;
-    return m_impl->eng->submit(scene, detail::camera_record(camera), k, params.spp, 0, total,
+    const uint64_t first = (uint64_t)params.firstSample * params.width * params.height;
+    return m_impl->eng->submit(scene, detail::camera_record(camera), k, params.spp, first, total,
                                detail::Engine::Mode::Accumulate, nullptr);
 }
 bool RenderServer::step() { return m_impl->eng->step(); }

@@ -1,0 +1,203 @@
+"""Training data, training loop and evaluation of the learned denoiser.
+
+Data: scenes of the large incoherent generator (2,000 displaced
+icospheres, 64 power-law emitters; scenes.large_scene with per-scene seeds)
+seen from random cameras. Each example is the tracer's own 4-spp colour
+with its first-hit albedo and normal layers (Renderer.renderWithGuides,
+samples 0..3) and a high-spp target over disjoint samples (firstSample =
+spp_noisy), so input noise and target noise are independent.
+
+Evaluation (time to relMSE <= 1e-2, relMSE = mean over pixels and channels
+of (x - ref)^2 / (ref^2 + 1e-2)): the reference is a high-spp render over
+samples disjoint from everything it is compared with; the plain tracer
+renders 4, 8, 16, ... spp until it reaches the bar; the learned path is
+Renderer.render at 4 spp with the network loaded (guides, render and
+inference all on the device).
+"""
+from __future__ import annotations
+
+import math
+import os
+import tempfile
+import time
+
+import numpy as np
+import torch
+
+import offlinerenderer_amd as wr
+from offlinerenderer_amd import denoiser, scenes
+
+
+def large_camera(rng, w: int, h: int) -> dict:
+    """Random view into the 100^3 box of the large scene generator."""
+    v = rng.normal(size=3)
+    v /= np.linalg.norm(v)
+    pos = 50.0 + v * rng.uniform(55.0, 90.0)
+    tgt = 50.0 + rng.normal(size=3) * 12.0
+    return dict(position=tuple(float(x) for x in pos), target=tuple(float(x) for x in tgt), up=(0.0, 1.0, 0.0),
+                fovY=math.radians(float(rng.uniform(40.0, 60.0))), aspect=w / h)
+
+
+def build_scene(seed: int, w: int, h: int):
+    d = scenes.large_scene(w, h, 1, seed=seed)
+    s = wr.Scene()
+    d.build(s)
+    s.finalize()
+    return d, s
+
+
+def rel_mse(x: np.ndarray, ref: np.ndarray, eps: float = 1e-2) -> float:
+    x = x.astype(np.float64)
+    ref = ref.astype(np.float64)
+    return float(np.mean((x - ref) ** 2 / (ref * ref + eps)))
+
+
+def build_dataset(num_scenes=8, cams_per_scene=48, res=256, noisy_spps=(4, 16, 64), spp_target=1024, seed=100,
+                  log=print):
+    """CUDA float32 tensors (N, res, res, 3): color, albedo, normal, target.
+    Every view is rendered at each noise level of noisy_spps (samples
+    0..spp-1); its target covers samples max(noisy_spps) .. + spp_target."""
+    r = wr.Renderer()
+    r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=1 << 26))
+    rng = np.random.default_rng(seed)
+    n = num_scenes * cams_per_scene * len(noisy_spps)
+    out = {k: torch.empty((n, res, res, 3), dtype=torch.float32, device="cuda")
+           for k in ("color", "albedo", "normal", "target")}
+    t0 = time.perf_counter()
+    i = 0
+    for si in range(num_scenes):
+        d, s = build_scene(seed + si, res, res)
+        for _ in range(cams_per_scene):
+            cam = wr.Camera(**large_camera(rng, res, res))
+            p = wr.RenderParams(width=res, height=res, spp=spp_target, maxBounces=d.maxBounces,
+                                russianRouletteDepth=d.rrDepth, firstSample=max(noisy_spps))
+            tgt = torch.from_numpy(r.render(s, cam, p))
+            for spp in noisy_spps:
+                p.spp, p.firstSample = spp, 0
+                col, alb, nrm = r.renderWithGuides(s, cam, p)
+                for k, a in (("color", col), ("albedo", alb), ("normal", nrm)):
+                    out[k][i].copy_(torch.from_numpy(a))
+                out["target"][i].copy_(tgt)
+                i += 1
+        del s
+        log(f"[denoise] data: scene {si + 1}/{num_scenes}, {i} images, {time.perf_counter() - t0:.1f} s")
+    return out
+
+
+def train(net: denoiser.UNet, data: dict, steps=3000, batch=16, crop=128, lr=1e-3, seed=7, loss="relmse",
+          log=print):
+    """Adam on random crops; cosine learning-rate decay. Returns the losses."""
+    n, res = data["color"].shape[0], data["color"].shape[1]
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    losses = []
+    t0 = time.perf_counter()
+    for it in range(steps):
+        img = torch.randint(0, n, (batch,), generator=g)
+        xy = torch.randint(0, res - crop + 1, (batch, 2), generator=g)
+        origins = torch.stack([img, xy[:, 0], xy[:, 1]], 1).to(torch.int32).reshape(-1).cuda()
+        step_lr = lr * 0.5 * (1.0 + math.cos(math.pi * it / steps)) + 1e-6
+        losses.append(net.train_step(data["color"], data["albedo"], data["normal"], data["target"], step_lr,
+                                     origins=origins, crop=(batch, crop, crop), loss=loss))
+        if (it + 1) % 500 == 0 or it + 1 == steps:
+            log(f"[denoise] train: step {it + 1}/{steps}, loss {np.mean(losses[-100:]):.5f}, "
+                f"{time.perf_counter() - t0:.1f} s")
+    return losses
+
+
+def _timed_render(r, s, cam, p):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    img = r.render(s, cam, p)
+    return img, time.perf_counter() - t0
+
+
+def evaluate(net: denoiser.UNet, width=1920, height=1080, seed=9001, spp_noisy=4, ref_spp=4096, bar=1e-2,
+             max_plain_spp=4096, log=print):
+    """Time to relMSE <= bar: plain tracer against 4 spp + learned denoiser."""
+    d, s = build_scene(seed, width, height)
+    cam = wr.Camera(**large_camera(np.random.default_rng(seed), width, height))
+    r = wr.Renderer()
+    r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=1 << 28))
+
+    def params(spp, first=0, denoise=False):
+        p = wr.RenderParams(width=width, height=height, spp=spp, maxBounces=d.maxBounces,
+                            russianRouletteDepth=d.rrDepth, firstSample=first)
+        p.denoiser.enabled = denoise
+        return p
+
+    # reference over samples no other render uses
+    ref, t_ref = _timed_render(r, s, cam, params(ref_spp, first=1 << 20))
+    log(f"[denoise] eval: reference {ref_spp} spp in {t_ref:.1f} s")
+    _timed_render(r, s, cam, params(spp_noisy))  # warm-up
+    plain = []
+    spp = spp_noisy
+    while spp <= max_plain_spp:
+        img, t = _timed_render(r, s, cam, params(spp))
+        plain.append({"spp": spp, "seconds": t, "rel_mse": rel_mse(img, ref)})
+        log(f"[denoise] eval: plain {spp} spp relMSE {plain[-1]['rel_mse']:.4f} in {t * 1e3:.0f} ms")
+        if plain[-1]["rel_mse"] <= bar:
+            break
+        spp *= 2
+    # learned: the renderer's own denoised render (weights through a file)
+    fd, path = tempfile.mkstemp(suffix=".f32")
+    os.close(fd)
+    try:
+        net.save(path)
+        r.loadDenoiser(path)
+    finally:
+        os.unlink(path)
+    _timed_render(r, s, cam, params(spp_noisy, denoise=True))  # warm-up
+    times = []
+    for _ in range(3):
+        img_l, t = _timed_render(r, s, cam, params(spp_noisy, denoise=True))
+        times.append(t)
+    t_learned = float(np.median(times))
+    rel_l = rel_mse(img_l, ref)
+    # the learned path at more input samples, until it reaches the bar
+    learned = [{"spp": spp_noisy, "seconds": t_learned, "rel_mse": rel_l}]
+    spp = spp_noisy * 2
+    while learned[-1]["rel_mse"] > bar and spp <= max_plain_spp:
+        img, t = _timed_render(r, s, cam, params(spp, denoise=True))
+        learned.append({"spp": spp, "seconds": t, "rel_mse": rel_mse(img, ref)})
+        log(f"[denoise] eval: learned {spp} spp relMSE {learned[-1]['rel_mse']:.4f} in {t * 1e3:.0f} ms")
+        spp *= 2
+    noisy, t_noisy = _timed_render(r, s, cam, params(spp_noisy))
+    log(f"[denoise] eval: learned relMSE {rel_l:.4f} in {t_learned * 1e3:.1f} ms "
+        f"(noisy {spp_noisy} spp: relMSE {rel_mse(noisy, ref):.4f}, {t_noisy * 1e3:.1f} ms)")
+    # inference alone, device-resident inputs
+    dev = [torch.from_numpy(a).cuda().unsqueeze(0) for a in r.renderWithGuides(s, cam, params(spp_noisy))]
+    out = torch.empty_like(dev[0])
+    net.forward(*dev, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        net.forward(*dev, out)
+    torch.cuda.synchronize()
+    t_inf = (time.perf_counter() - t0) / reps
+    reached = [p for p in plain if p["rel_mse"] <= bar]
+    res = {
+        "scene_seed": seed, "width": width, "height": height, "reference_spp": ref_spp, "bar": bar,
+        "plain": plain,
+        "plain_spp_to_bar": reached[0]["spp"] if reached else None,
+        "plain_seconds_to_bar": reached[0]["seconds"] if reached else None,
+        "learned_rel_mse": rel_l, "learned_seconds": t_learned, "learned_reaches_bar": rel_l <= bar,
+        "learned_sweep": learned,
+        "learned_spp_to_bar": next((q["spp"] for q in learned if q["rel_mse"] <= bar), None),
+        "learned_seconds_to_bar": next((q["seconds"] for q in learned if q["rel_mse"] <= bar), None),
+        "noisy_rel_mse": rel_mse(noisy, ref), "noisy_seconds": t_noisy,
+        "inference_ms": t_inf * 1e3,
+        "inference_tflops": denoiser.flops_per_pixel() * width * height / t_inf / 1e12,
+    }
+    # equal-quality plain time when the bar is out of reach for the learned path
+    rels = np.array([p["rel_mse"] for p in plain])
+    secs = np.array([p["seconds"] for p in plain])
+    spps = np.array([p["spp"] for p in plain], dtype=np.float64)
+    if len(plain) >= 2 and rel_l < rels[0]:
+        # relMSE ~ 1/spp: interpolate log-log between the bracketing renders
+        k = int(np.searchsorted(-rels, -rel_l))
+        k = min(max(k, 1), len(plain) - 1)
+        f = (math.log(rel_l) - math.log(rels[k - 1])) / (math.log(rels[k]) - math.log(rels[k - 1]))
+        res["plain_spp_equal_quality"] = float(math.exp(math.log(spps[k - 1]) + f * (math.log(spps[k]) - math.log(spps[k - 1]))))
+        res["plain_seconds_equal_quality"] = float(math.exp(math.log(secs[k - 1]) + f * (math.log(secs[k]) - math.log(secs[k - 1]))))
+    return res

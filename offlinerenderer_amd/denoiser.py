@@ -93,10 +93,14 @@ class UNet:
         self._check(lib().wrk_unet_forward(self._h, _ptr(color), _ptr(albedo), _ptr(normal), color.shape[1],
                                            color.shape[2], _ptr(origins), n, h, w, _ptr(out), stream), "forward")
 
-    def train_step(self, color, albedo, normal, target, lr, origins=None, crop=None, stream=None) -> float:
+    def train_step(self, color, albedo, normal, target, lr, origins=None, crop=None, stream=None,
+                   loss: str = "log") -> float:
+        """One Adam step; loss "log" = MSE of log(1 + L), "relmse" = the
+        evaluation metric (y - t)^2 / (t^2 + 0.01)."""
         n, h, w = crop if crop else (color.shape[0], color.shape[1], color.shape[2])
-        loss = C.c_float(0.0)
+        mode = {"log": 0, "relmse": 1}[loss]
+        out = C.c_float(0.0)
         self._check(lib().wrk_unet_train_step(self._h, _ptr(color), _ptr(albedo), _ptr(normal), _ptr(target),
                                               color.shape[1], color.shape[2], _ptr(origins), n, h, w,
-                                              C.c_float(lr), C.byref(loss), stream), "train_step")
-        return float(loss.value)
+                                              C.c_float(lr), mode, C.byref(out), stream), "train_step")
+        return float(out.value)

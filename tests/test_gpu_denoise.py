@@ -158,7 +158,8 @@ def test_unet_forward_matches_torch(gpu):
     net.forward(*(torch.from_numpy(a).cuda() for a in (color, albedo, nrm)), out)
     torch.cuda.synchronize()
     got = out.cpu().numpy()
-    pred = U.forward(params, U.pack_input(color, albedo, nrm))[:, 0:3].permute(0, 2, 3, 1).numpy()
+    x = U.pack_input(color, albedo, nrm)
+    pred = (U.forward(params, x)[:, 0:3] + x[:, 0:3]).permute(0, 2, 3, 1).numpy()
     ref = np.maximum(np.expm1(pred), 0.0)
     # 15 layers of bf16 activations: compare in log space
     err = np.abs(np.log1p(got) - np.log1p(ref))
@@ -194,3 +195,34 @@ def test_unet_gradients_and_training(gpu):
     net.forward(*dev[:3], full)
     torch.cuda.synchronize()
     assert np.isfinite(out.cpu().numpy()).all() and np.isfinite(full.cpu().numpy()).all()
+
+
+def test_renderer_denoises_with_shipped_weights(gpu, tmp_path):
+    from offlinerenderer_amd import scenes
+    d = scenes.large_scene(64, 48, 4, num_spheres=200, num_emitters=16)
+    s = wr.Scene()
+    d.build(s)
+    s.finalize()
+    cam = wr.Camera(**d.camera)
+    r = wr.Renderer()
+    r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=1 << 20))
+    p = wr.RenderParams(width=64, height=48, spp=4, maxBounces=8)
+    plain = r.render(s, cam, p)
+    p.denoiser.enabled = True
+    den = r.render(s, cam, p)
+    assert np.isfinite(den).all() and den.min() >= 0.0
+    assert not np.array_equal(den, plain)
+    # the network the renderer runs is the shipped one
+    net = denoiser.UNet()
+    net.load(str(wr.LIB_PATH.parent.parent / "models" / "denoiser_unet.f32"))
+    col, alb, nrm = r.renderWithGuides(s, cam, wr.RenderParams(width=64, height=48, spp=4, maxBounces=8))
+    dev = [torch.from_numpy(a).cuda().unsqueeze(0) for a in (col, alb, nrm)]
+    out = torch.empty_like(dev[0])
+    net.forward(*dev, out)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy()[0], den)
+    # a file of the wrong size is refused
+    bad = tmp_path / "bad.f32"
+    np.zeros(10, np.float32).tofile(bad)
+    with pytest.raises(RuntimeError):
+        r.loadDenoiser(bad)

@@ -84,3 +84,18 @@ def test_guides_match_oracle(gpu, oracle, which):
     assert np.array_equal(alb.reshape(-1, 3).view(np.uint32), oa.view(np.uint32))
     assert np.array_equal(nrm.reshape(-1, 3).view(np.uint32), on.view(np.uint32))
     assert np.count_nonzero(alb) > alb.size // 2
+
+
+def test_first_sample_offset_matches_oracle(gpu, oracle):
+    # samples [5, 8) of every pixel: the same paths the oracle numbers 5..7
+    d = scenes.cornell(24, 16, 3)
+    ps = G.product_scene(d)
+    os_ = oracle.build_oracle_scene(d, bvh=False)
+    r = wr.Renderer()
+    r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=8, maxPathsInFlight=1 << 20))
+    prm = G.params(d)
+    prm.firstSample = 5
+    img = r.render(ps, G.camera(d), prm)
+    acc = oracle.render_image(os_, d.camera, oracle.params_in(24, 16, 8, 3.0, iterationCap=8), 3, s0=5, use_bvh=False)
+    ref = acc * np.float32(1.0 / 3)
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))

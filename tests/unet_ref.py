@@ -42,7 +42,8 @@ def _q(t):
 
 
 def forward(params: np.ndarray, x: torch.Tensor, quantize=True):
-    """x: (n, 32, h, w) -> prediction (n, 32, h, w), log(1 + colour) in 0..2."""
+    """x: (n, 32, h, w) -> network output (n, 32, h, w); channels 0..2 are the
+    correction added to the input's log(1 + colour)."""
     q = _q if quantize else (lambda t: t)
     sl = denoiser.layer_slices()
     W = []
@@ -94,7 +95,7 @@ def loss_and_grads(params: np.ndarray, x: torch.Tensor, target: np.ndarray):
     d2 = conv(9, conv(8, torch.cat([F.interpolate(b3, scale_factor=2, mode="nearest"), e2], 1)))
     d1 = conv(11, conv(10, torch.cat([F.interpolate(d2, scale_factor=2, mode="nearest"), e1], 1)))
     d0 = conv(13, conv(12, torch.cat([F.interpolate(d1, scale_factor=2, mode="nearest"), e0], 1)))
-    pred = conv(14, d0, relu=False)[:, 0:3]
+    pred = conv(14, d0, relu=False)[:, 0:3] + x[:, 0:3]
     tgt = torch.from_numpy(np.log1p(np.maximum(target, 0.0))).permute(0, 3, 1, 2)
     loss = ((pred - tgt) ** 2).mean()
     loss.backward()
