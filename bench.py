@@ -74,7 +74,8 @@ def main():
     ap.add_argument("--dn-cams", type=int, default=48, help="denoise: cameras per training scene")
     ap.add_argument("--dn-target-spp", type=int, default=1024, help="denoise: spp of the training targets")
     ap.add_argument("--dn-steps", type=int, default=4000, help="denoise: training steps (16 crops of 128^2)")
-    ap.add_argument("--dn-loss", choices=["relmse", "log"], default="relmse", help="denoise: training loss")
+    ap.add_argument("--dn-loss", choices=["relmse_split", "relmse", "relmse_pred", "log"], default="relmse_split",
+                    help="denoise: training loss")
     ap.add_argument("--dn-lr", type=float, default=1e-3, help="denoise: peak learning rate")
     ap.add_argument("--dn-eval-size", type=str, default="1920x1080", help="denoise: evaluation resolution")
     ap.add_argument("--dn-ref-spp", type=int, default=4096, help="denoise: spp of the evaluation reference")

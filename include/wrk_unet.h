@@ -57,12 +57,14 @@ int wrk_unet_pack(const float* color, const float* albedo, const float* normal, 
 int wrk_unet_unpack(const uint16_t* pred, uint32_t pitch, const uint16_t* base, uint32_t basePitch, float* out,
                     uint64_t npix, void* stream);
 /* per-256-pixel partial sums of the loss and its gradient dpred, with q as
- * in wrk_unet_unpack; mode 0: (q - log(1 + t))^2, mode 1: relMSE
- * (exp(q) - 1 - t)^2 / (t^2 + 0.01) */
+ * in wrk_unet_unpack and y = exp(q) - 1; mode 0: (q - log(1 + t))^2,
+ * mode 1: relMSE (y - t)^2 / (t^2 + 0.01), mode 2: (y - t)^2 / (y^2 + 0.01)
+ * with the denominator constant in the gradient, mode 3: (y - t)^2 /
+ * (t2^2 + 0.01) with t2 from target2 (a target over disjoint samples) */
 uint64_t wrk_unet_loss_scratch_bytes(uint32_t n, uint32_t h, uint32_t w);
 int wrk_unet_loss(const uint16_t* pred, uint32_t pitch, const float* target, uint32_t srcH, uint32_t srcW,
-                  const uint32_t* origins, const uint16_t* base, uint32_t basePitch, uint16_t* dpred, uint32_t n,
-                  uint32_t h, uint32_t w, uint32_t mode, float* partial, void* stream);
+                  const uint32_t* origins, const float* target2, const uint16_t* base, uint32_t basePitch,
+                  uint16_t* dpred, uint32_t n, uint32_t h, uint32_t w, uint32_t mode, float* partial, void* stream);
 /* Adam on FP32 parameters (step counts from 1) */
 int wrk_adam(float* p, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
              uint32_t step, void* stream);
@@ -86,8 +88,8 @@ int wrk_unet_forward(void* net, const float* color, const float* albedo, const f
 /* one Adam step on the loss of wrk_unet_loss (lossMode) against target;
  * *loss (host) = the loss before the step */
 int wrk_unet_train_step(void* net, const float* color, const float* albedo, const float* normal, const float* target,
-                        uint32_t srcH, uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w,
-                        float lr, uint32_t lossMode, float* loss, void* stream);
+                        const float* target2, uint32_t srcH, uint32_t srcW, const uint32_t* origins, uint32_t n,
+                        uint32_t h, uint32_t w, float lr, uint32_t lossMode, float* loss, void* stream);
 
 #ifdef __cplusplus
 }

@@ -145,8 +145,8 @@ _SIGS = {
     "wrk_unet_pack": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "wrk_unet_unpack": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint64, _P]),
     "wrk_unet_loss_scratch_bytes": (C.c_uint64, [C.c_uint32] * 3),
-    "wrk_unet_loss": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, _P, C.c_uint32,
-                                C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]),
+    "wrk_unet_loss": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint32, _P,
+                                C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_adam": (C.c_int, [_P, _P, _P, _P, C.c_uint64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, _P]),
     "wrk_unet_last_error": (C.c_char_p, []),
     "wrk_unet_create": (_P, []),
@@ -158,8 +158,8 @@ _SIGS = {
     "wrk_unet_get_grads": (C.c_int, [_P, _P]),
     "wrk_unet_forward": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                                    _P, _P]),
-    "wrk_unet_train_step": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32,
-                                      C.c_uint32, C.c_float, C.c_uint32, _P, _P]),
+    "wrk_unet_train_step": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32,
+                                      C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, _P, _P]),
     # wrk_*
     "wrk_last_error": (C.c_char_p, []),
     "wrk_device_count": (C.c_int, [C.POINTER(C.c_int)]),

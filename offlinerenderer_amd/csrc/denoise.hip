@@ -378,11 +378,17 @@ __global__ __launch_bounds__(256) void k_unet_unpack(const bf16* __restrict__ pr
 // Per pixel and channel c < 3, with t the target and y = exp(pred) - 1:
 //   mode 0: (pred - log(1 + t))^2            (log-space MSE)
 //   mode 1: (y - t)^2 / (t^2 + 0.01)          (relMSE, the evaluation metric)
+//   mode 2: (y - t)^2 / (y^2 + 0.01), the denominator held constant in the
+//           gradient (Lehtinen et al. 2018, Noise2Noise)
+//   mode 3: (y - t)^2 / (t2^2 + 0.01) with t2 = target2, a second target
+//           over disjoint samples: numerator and denominator noise are
+//           independent, so the minimiser is E[t] (mode 1 biases it low)
 // averaged over pixels and 3 channels; dPred is its gradient (0 for the
 // padding channels). Per-block partial loss sums.
 __global__ __launch_bounds__(256) void k_unet_loss(const bf16* __restrict__ pred, uint32_t pitch,
                                                    const float* __restrict__ target, uint32_t srcH, uint32_t srcW,
                                                    const uint32_t* __restrict__ origins,
+                                                   const float* __restrict__ target2,
                                                    const bf16* __restrict__ base, uint32_t basePitch,
                                                    bf16* __restrict__ dpred, uint32_t N, uint32_t H, uint32_t W,
                                                    uint32_t mode, float* __restrict__ partial) {
@@ -405,10 +411,12 @@ __global__ __launch_bounds__(256) void k_unet_loss(const bf16* __restrict__ pred
             if (c < 3) {
                 const float t = fmaxf(target[s * 3 + c], 0.0f);
                 const float q = (float)pred[p * pitch + c] + (base ? (float)base[p * basePitch + c] : 0.0f);
-                if (mode == 1) {
+                if (mode >= 1) {
                     const float e = expf(fminf(q, 12.0f));
-                    const float inv = 1.0f / (t * t + 0.01f);
-                    const float d = (e - 1.0f) - t;
+                    const float y = e - 1.0f;
+                    const float t2 = mode == 3 ? fmaxf(target2[s * 3 + c], 0.0f) : t;
+                    const float inv = 1.0f / (mode == 2 ? y * y + 0.01f : t2 * t2 + 0.01f);
+                    const float d = y - t;
                     l += d * d * inv;
                     g = q < 12.0f ? d * inv * e * scale : 0.0f;
                 } else {
@@ -625,16 +633,18 @@ extern "C" uint64_t wrk_unet_loss_scratch_bytes(uint32_t n, uint32_t h, uint32_t
 }
 
 extern "C" int wrk_unet_loss(const uint16_t* pred, uint32_t pitch, const float* target, uint32_t srcH, uint32_t srcW,
-                             const uint32_t* origins, const uint16_t* base, uint32_t basePitch, uint16_t* dpred,
-                             uint32_t n, uint32_t h, uint32_t w, uint32_t mode, float* partial, void* stream) {
+                             const uint32_t* origins, const float* target2, const uint16_t* base, uint32_t basePitch,
+                             uint16_t* dpred, uint32_t n, uint32_t h, uint32_t w, uint32_t mode, float* partial,
+                             void* stream) {
     WRK_REQUIRE(pred && target && dpred && partial && pitch >= 3 && (!base || basePitch >= 3),
                 "wrk_unet_loss: bad argument");
+    WRK_REQUIRE(mode <= 3 && (mode != 3 || target2), "wrk_unet_loss: mode 3 needs target2");
     WRK_REQUIRE(origins || (srcH == h && srcW == w), "wrk_unet_loss: crops need origins");
     const uint64_t np = (uint64_t)n * h * w;
     if (np == 0) return WRK_OK;
     hipLaunchKernelGGL(k_unet_loss, dim3(ceil_div(np, 256)), dim3(256), 0, as_stream(stream), (const bf16*)pred,
-                       pitch, target, srcH, srcW, origins, (const bf16*)base, basePitch, (bf16*)dpred, n, h, w, mode,
-                       partial);
+                       pitch, target, srcH, srcW, origins, target2, (const bf16*)base, basePitch, (bf16*)dpred, n, h, w,
+                       mode, partial);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }

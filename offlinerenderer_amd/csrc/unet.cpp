@@ -137,13 +137,15 @@ public:
             wrk_ok(wrk_unet_unpack(pred_.buf.h(), 32, in_.buf.h(), 32, out, (uint64_t)n * h * w, s), "wrk_unet_unpack");
     }
 
-    float trainStep(const float* color, const float* albedo, const float* normal, const float* target, uint32_t srcH,
+    float trainStep(const float* color, const float* albedo, const float* normal, const float* target,
+                    const float* target2, uint32_t srcH,
                     uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w, float lr,
                     uint32_t lossMode, hipStream_t s) {
         shape(n, h, w, true);
         wrk_ok(wrk_unet_pack(color, albedo, normal, srcH, srcW, origins, in_.buf.h(), n, h, w, s), "wrk_unet_pack");
         runForward(s);
-        wrk_ok(wrk_unet_loss(pred_.buf.h(), 32, target, srcH, srcW, origins, in_.buf.h(), 32, gPred_.buf.h(), n, h, w, lossMode,
+        wrk_ok(wrk_unet_loss(pred_.buf.h(), 32, target, srcH, srcW, origins, target2, in_.buf.h(), 32, gPred_.buf.h(), n, h,
+                             w, lossMode,
                              lossPartial_.f(), s),
                "wrk_unet_loss");
         runBackward(s);
@@ -382,11 +384,11 @@ int wrk_unet_forward(void* net, const float* color, const float* albedo, const f
     });
 }
 int wrk_unet_train_step(void* net, const float* color, const float* albedo, const float* normal, const float* target,
-                        uint32_t srcH, uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w,
-                        float lr, uint32_t lossMode, float* loss, void* stream) {
+                        const float* target2, uint32_t srcH, uint32_t srcW, const uint32_t* origins, uint32_t n,
+                        uint32_t h, uint32_t w, float lr, uint32_t lossMode, float* loss, void* stream) {
     return guard([&] {
-        float l = static_cast<UNet*>(net)->trainStep(color, albedo, normal, target, srcH, srcW, origins, n, h, w, lr,
-                                                     lossMode, static_cast<hipStream_t>(stream));
+        float l = static_cast<UNet*>(net)->trainStep(color, albedo, normal, target, target2, srcH, srcW, origins, n, h,
+                                                     w, lr, lossMode, static_cast<hipStream_t>(stream));
         if (loss) *loss = l;
     });
 }

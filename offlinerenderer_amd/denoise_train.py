@@ -54,30 +54,35 @@ def rel_mse(x: np.ndarray, ref: np.ndarray, eps: float = 1e-2) -> float:
 
 def build_dataset(num_scenes=8, cams_per_scene=48, res=256, noisy_spps=(4, 16, 64), spp_target=1024, seed=100,
                   log=print):
-    """CUDA float32 tensors (N, res, res, 3): color, albedo, normal, target.
-    Every view is rendered at each noise level of noisy_spps (samples
-    0..spp-1); its target covers samples max(noisy_spps) .. + spp_target."""
+    """CUDA float32 tensors (N, res, res, 3): color, albedo, normal, target,
+    target2. Every view is rendered at each noise level of noisy_spps
+    (samples 0..spp-1); target and target2 cover two disjoint runs of
+    spp_target / 2 samples after those."""
     r = wr.Renderer()
     r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=1 << 26))
     rng = np.random.default_rng(seed)
     n = num_scenes * cams_per_scene * len(noisy_spps)
     out = {k: torch.empty((n, res, res, 3), dtype=torch.float32, device="cuda")
-           for k in ("color", "albedo", "normal", "target")}
+           for k in ("color", "albedo", "normal", "target", "target2")}
     t0 = time.perf_counter()
     i = 0
     for si in range(num_scenes):
         d, s = build_scene(seed + si, res, res)
         for _ in range(cams_per_scene):
             cam = wr.Camera(**large_camera(rng, res, res))
-            p = wr.RenderParams(width=res, height=res, spp=spp_target, maxBounces=d.maxBounces,
+            half = spp_target // 2
+            p = wr.RenderParams(width=res, height=res, spp=half, maxBounces=d.maxBounces,
                                 russianRouletteDepth=d.rrDepth, firstSample=max(noisy_spps))
             tgt = torch.from_numpy(r.render(s, cam, p))
+            p.firstSample = max(noisy_spps) + half
+            tgt2 = torch.from_numpy(r.render(s, cam, p))
             for spp in noisy_spps:
                 p.spp, p.firstSample = spp, 0
                 col, alb, nrm = r.renderWithGuides(s, cam, p)
                 for k, a in (("color", col), ("albedo", alb), ("normal", nrm)):
                     out[k][i].copy_(torch.from_numpy(a))
                 out["target"][i].copy_(tgt)
+                out["target2"][i].copy_(tgt2)
                 i += 1
         del s
         log(f"[denoise] data: scene {si + 1}/{num_scenes}, {i} images, {time.perf_counter() - t0:.1f} s")
@@ -96,8 +101,10 @@ def train(net: denoiser.UNet, data: dict, steps=3000, batch=16, crop=128, lr=1e-
         xy = torch.randint(0, res - crop + 1, (batch, 2), generator=g)
         origins = torch.stack([img, xy[:, 0], xy[:, 1]], 1).to(torch.int32).reshape(-1).cuda()
         step_lr = lr * 0.5 * (1.0 + math.cos(math.pi * it / steps)) + 1e-6
-        losses.append(net.train_step(data["color"], data["albedo"], data["normal"], data["target"], step_lr,
-                                     origins=origins, crop=(batch, crop, crop), loss=loss))
+        # the two target halves swap roles from step to step
+        ta, tb = (data["target"], data["target2"]) if it % 2 == 0 else (data["target2"], data["target"])
+        losses.append(net.train_step(data["color"], data["albedo"], data["normal"], ta, step_lr,
+                                     origins=origins, crop=(batch, crop, crop), loss=loss, target2=tb))
         if (it + 1) % 500 == 0 or it + 1 == steps:
             log(f"[denoise] train: step {it + 1}/{steps}, loss {np.mean(losses[-100:]):.5f}, "
                 f"{time.perf_counter() - t0:.1f} s")

@@ -94,13 +94,16 @@ class UNet:
                                            color.shape[2], _ptr(origins), n, h, w, _ptr(out), stream), "forward")
 
     def train_step(self, color, albedo, normal, target, lr, origins=None, crop=None, stream=None,
-                   loss: str = "log") -> float:
+                   loss: str = "log", target2=None) -> float:
         """One Adam step; loss "log" = MSE of log(1 + L), "relmse" = the
-        evaluation metric (y - t)^2 / (t^2 + 0.01)."""
+        evaluation metric (y - t)^2 / (t^2 + 0.01), "relmse_pred" = the same
+        with the prediction's (constant) square in the denominator, unbiased
+        under noisy targets, "relmse_split" = numerator against target and
+        denominator from target2 (disjoint samples), unbiased as well."""
         n, h, w = crop if crop else (color.shape[0], color.shape[1], color.shape[2])
-        mode = {"log": 0, "relmse": 1}[loss]
+        mode = {"log": 0, "relmse": 1, "relmse_pred": 2, "relmse_split": 3}[loss]
         out = C.c_float(0.0)
         self._check(lib().wrk_unet_train_step(self._h, _ptr(color), _ptr(albedo), _ptr(normal), _ptr(target),
-                                              color.shape[1], color.shape[2], _ptr(origins), n, h, w,
+                                              _ptr(target2), color.shape[1], color.shape[2], _ptr(origins), n, h, w,
                                               C.c_float(lr), mode, C.byref(out), stream), "train_step")
         return float(out.value)
