@@ -141,6 +141,17 @@ __global__ __launch_bounds__(1024) void k_reduce_bounds(const float* __restrict_
     }
 }
 
+#ifndef WRK_MORTON64
+#define WRK_MORTON64 1
+#endif
+#if WRK_MORTON64
+typedef uint64_t MortonKey;
+constexpr unsigned kMortonBits = 63;
+#else
+typedef uint32_t MortonKey;
+constexpr unsigned kMortonBits = 30;
+#endif
+
 __device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
     v = (v * 0x00010001u) & 0xFF0000FFu;
     v = (v * 0x00000101u) & 0x0F00F00Fu;
@@ -149,9 +160,23 @@ __device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
     return v;
 }
 
+// 21 bits per axis -> every third bit of a 63-bit code
+__device__ __forceinline__ uint64_t expand_bits21(uint64_t v) {
+    v &= 0x1FFFFFull;
+    v = (v | (v << 32)) & 0x001F00000000FFFFull;
+    v = (v | (v << 16)) & 0x001F0000FF0000FFull;
+    v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+    v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+// Morton code of the triangle centroid in the centroid bounds: 63 bits (21
+// per axis) so that neighbouring small triangles of a dense mesh get distinct
+// codes and the hierarchy splits them by position rather than by index.
 __global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ triLo, const float4* __restrict__ triHi,
                                                 uint32_t n, const uint32_t* __restrict__ counters,
-                                                uint32_t* keys, uint32_t* vals) {
+                                                MortonKey* keys, uint32_t* vals) {
     uint32_t t = blockIdx.x * 256 + threadIdx.x;
     if (t >= n) return;
     float4 lo = triLo[t], hi = triHi[t];
@@ -163,23 +188,36 @@ __global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ triLo
         float mx = unordered(counters[C_CMAX + a]);
         float ext = mx - mn;
         float u = ext > 0.0f ? (c[a] - mn) / ext : 0.0f;
+#if WRK_MORTON64
+        float s = fminf(fmaxf(u * 2097152.0f, 0.0f), 2097151.0f);
+#else
         float s = fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
+#endif
         q[a] = (uint32_t)s;
     }
+#if WRK_MORTON64
+    keys[t] = (expand_bits21(q[0]) << 2) | (expand_bits21(q[1]) << 1) | expand_bits21(q[2]);
+#else
     keys[t] = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+#endif
     vals[t] = t;
 }
 
 // Karras 2012: common prefix length of sorted keys i and j, with the index
 // appended to break ties between equal codes.
-__device__ __forceinline__ int delta(const uint32_t* keys, int n, int i, int j) {
+__device__ __forceinline__ int delta(const MortonKey* keys, int n, int i, int j) {
     if (j < 0 || j >= n) return -1;
-    uint32_t ki = keys[i], kj = keys[j];
+    MortonKey ki = keys[i], kj = keys[j];
+#if WRK_MORTON64
+    if (ki == kj) return 64 + __clz((uint32_t)i ^ (uint32_t)j);
+    return __clzll((unsigned long long)(ki ^ kj));
+#else
     if (ki == kj) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
     return __clz(ki ^ kj);
+#endif
 }
 
-__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ keys, int n, BinNode* bnodes) {
+__global__ __launch_bounds__(256) void k_karras(const MortonKey* __restrict__ keys, int n, BinNode* bnodes) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n - 1) return;
     int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) > 0 ? 1 : -1;
@@ -570,8 +608,8 @@ BuildLayout layout_for(uint32_t n) {
     size_t nn = std::max<size_t>(n, 2);
     L.triLo = take(nn * 16);
     L.triHi = take(nn * 16);
-    L.keysA = take(nn * 4);
-    L.keysB = take(nn * 4);
+    L.keysA = take(nn * sizeof(MortonKey));
+    L.keysB = take(nn * sizeof(MortonKey));
     L.valsA = take(nn * 4);
     L.valsB = take(nn * 4);
     L.bnodes = take(nn * sizeof(BinNode));
@@ -581,8 +619,8 @@ BuildLayout layout_for(uint32_t n) {
     L.tasksB = take(nn * 16);
     L.counters = take(C_COUNT * 4);
     size_t tmp = 0;
-    (void)rocprim::radix_sort_pairs(nullptr, tmp, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                    (uint32_t*)nullptr, (size_t)nn, 0u, 30u);
+    (void)rocprim::radix_sort_pairs(nullptr, tmp, (MortonKey*)nullptr, (MortonKey*)nullptr, (uint32_t*)nullptr,
+                                    (uint32_t*)nullptr, (size_t)nn, 0u, kMortonBits);
     L.sortTempBytes = tmp;
     L.sortTemp = take(tmp);
     L.total = off;
@@ -610,8 +648,8 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
     char* base = static_cast<char*>(scratch);
     float4* triLo = (float4*)(base + L.triLo);
     float4* triHi = (float4*)(base + L.triHi);
-    uint32_t* keysA = (uint32_t*)(base + L.keysA);
-    uint32_t* keysB = (uint32_t*)(base + L.keysB);
+    MortonKey* keysA = (MortonKey*)(base + L.keysA);
+    MortonKey* keysB = (MortonKey*)(base + L.keysB);
     uint32_t* valsA = (uint32_t*)(base + L.valsA);
     uint32_t* valsB = (uint32_t*)(base + L.valsB);
     BinNode* bnodes = (BinNode*)(base + L.bnodes);
@@ -630,7 +668,7 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
     WRK_CHECK_LAUNCH();
     size_t tmp = L.sortTempBytes;
     hipError_t sortErr = rocprim::radix_sort_pairs(base + L.sortTemp, tmp, keysA, keysB, valsA, valsB, (size_t)n, 0u,
-                                                   30u, s);
+                                                   kMortonBits, s);
     WRK_HIP_TRY(sortErr);
     if (n > 1) {
         hipLaunchKernelGGL(k_karras, dim3(ceil_div(n - 1, 256)), dim3(256), 0, s, keysB, (int)n, bnodes);
