@@ -434,8 +434,15 @@ __global__ __launch_bounds__(256) void k_reorder_tris(const uint32_t* __restrict
 // triangle the watertight test reports at t <= best (ties included). Ties
 // in t resolve to the smaller primitive id, so the result does not depend
 // on the traversal order.
+#ifndef WRK_TRACE_WAVES
+#define WRK_TRACE_WAVES 8
+#endif
 template <bool ANY, bool COUNT>
-__global__ __launch_bounds__(256) void k_trace(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
+__global__ __launch_bounds__(256)
+#if WRK_TRACE_WAVES
+__attribute__((amdgpu_waves_per_eu(WRK_TRACE_WAVES, WRK_TRACE_WAVES)))
+#endif
+void k_trace(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
                                                const float4* __restrict__ org, const float4* __restrict__ dir,
                                                const uint32_t* __restrict__ queue, uint32_t n,
                                                const uint32_t* __restrict__ countDev, uint2* __restrict__ hits,
