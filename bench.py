@@ -40,7 +40,6 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 128   # 4-wide node record
 TRI_BYTES = 48     # leaf-ordered triangle record (3 x float4)
 # closest-hit per ray: queue index 4 + ray (origin,tMin | dir,tMax) 32 + hit record 8
 CLOSEST_FIXED = 4 + 32 + 8
@@ -169,11 +168,11 @@ def main():
     mpaths = paths / elapsed / 1e6
     mrays = (rays_c + rays_s) / elapsed / 1e6
 
-    bytes_per_ray = CLOSEST_FIXED + nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES
+    bytes_per_ray = CLOSEST_FIXED + nodes_per_ray * bvh.nodeBytes + tris_per_ray * TRI_BYTES
     closest_s = totals["msTraceClosest"] / 1e3
     achieved = bytes_per_ray * rays_c / closest_s / 1e9
     avg_launch_ms = totals["msTraceClosest"] / max(launches_c, 1)
-    any_bytes_per_ray = ANY_FIXED + any_nodes * NODE_BYTES + any_tris * TRI_BYTES
+    any_bytes_per_ray = ANY_FIXED + any_nodes * bvh.nodeBytes + any_tris * TRI_BYTES
     any_gbs = any_bytes_per_ray * rays_s / max(totals["msTraceAny"] / 1e3, 1e-9) / 1e9
 
     if args.copy_probe_mib:

@@ -92,7 +92,7 @@ typedef struct wrk_bvh {
     uint32_t numNodes;
     uint32_t numTris;
     uint32_t maxDepth;
-    uint32_t _pad;
+    uint32_t nodeBytes; /* 96: compressed 8-wide nodes, 128: 4-wide nodes */
     float sceneLo[4];
     float sceneHi[4];
 } wrk_bvh;

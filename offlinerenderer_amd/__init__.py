@@ -29,7 +29,7 @@ _lib = None
 
 class wrk_bvh(C.Structure):
     _fields_ = [("nodes", C.c_uint64), ("tris", C.c_uint64), ("numNodes", C.c_uint32), ("numTris", C.c_uint32),
-                ("maxDepth", C.c_uint32), ("_pad", C.c_uint32), ("sceneLo", C.c_float * 4),
+                ("maxDepth", C.c_uint32), ("nodeBytes", C.c_uint32), ("sceneLo", C.c_float * 4),
                 ("sceneHi", C.c_float * 4)]
 
 

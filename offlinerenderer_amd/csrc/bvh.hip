@@ -312,13 +312,14 @@ __device__ __forceinline__ float half_area(const float lo[3], const float hi[3])
     return dx * dy + dy * dz + dz * dx;
 }
 
-// Collapse one binary subtree root into a 4-wide node: repeatedly open the
-// child with the largest surface area until 4 children, leaving subtrees of
-// at most kLeafMax triangles as leaves.
+// Collapse one binary subtree root into a W-wide node: repeatedly open the
+// child with the largest surface area until W children, leaving subtrees of
+// at most kLeafMax triangles as leaves. W = 8 writes the compressed Node8.
+template <int W>
 __global__ __launch_bounds__(256) void k_collapse(const uint4* __restrict__ tasksIn, uint32_t numTasks,
                                                   const BinNode* __restrict__ bnodes, const float4* __restrict__ triLo,
                                                   const float4* __restrict__ triHi, const uint32_t* __restrict__ vals,
-                                                  Node4* nodes, uint32_t* counters, uint4* tasksOut) {
+                                                  void* nodesOut, uint32_t* counters, uint4* tasksOut) {
     uint32_t ti = blockIdx.x * 256 + threadIdx.x;
     if (ti >= numTasks) return;
     uint4 task = tasksIn[ti];
@@ -326,7 +327,7 @@ __global__ __launch_bounds__(256) void k_collapse(const uint4* __restrict__ task
 #pragma unroll
     for (int a = 0; a < 3; ++a)
         sceneExtent = fmaxf(sceneExtent, unordered(counters[C_BMAX + a]) - unordered(counters[C_BMIN + a]));
-    uint32_t ref[4];
+    uint32_t ref[W];
     uint32_t m = 0;
     if (task.x & kLeafBit) {  // single-triangle scene
         ref[m++] = task.x;
@@ -337,7 +338,7 @@ __global__ __launch_bounds__(256) void k_collapse(const uint4* __restrict__ task
     auto count_of = [&](uint32_t r) -> uint32_t {
         return (r & kLeafBit) ? 1u : (bnodes[r].last - bnodes[r].first + 1u);
     };
-    while (m < 4) {
+    while (m < (uint32_t)W) {
         int best = -1;
         float bestA = -1.0f;
         for (uint32_t c = 0; c < m; ++c) {
@@ -353,13 +354,12 @@ __global__ __launch_bounds__(256) void k_collapse(const uint4* __restrict__ task
         ref[best] = __float_as_uint(bnodes[r].lo.w);
         ref[m++] = __float_as_uint(bnodes[r].hi.w);
     }
-    Node4 out;
-    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
-    uint32_t child[4];
-    for (uint32_t c = 0; c < 4; ++c) {
+    float blo[W][3], bhi[W][3];
+    uint32_t child[W];
+    for (uint32_t c = 0; c < (uint32_t)W; ++c) {
         if (c >= m) {
-            lox[c] = loy[c] = loz[c] = INFINITY;
-            hix[c] = hiy[c] = hiz[c] = -INFINITY;
+            blo[c][0] = blo[c][1] = blo[c][2] = INFINITY;
+            bhi[c][0] = bhi[c][1] = bhi[c][2] = -INFINITY;
             child[c] = kEmptyChild;
             continue;
         }
@@ -389,21 +389,73 @@ __global__ __launch_bounds__(256) void k_collapse(const uint4* __restrict__ task
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             float pad = (fmaxf(fabsf(lo[a]), fabsf(hi[a])) + sceneExtent) * kBoxPadRel;
-            lo[a] -= pad;
-            hi[a] += pad;
+            blo[c][a] = lo[a] - pad;
+            bhi[c][a] = hi[a] + pad;
         }
-        lox[c] = lo[0]; loy[c] = lo[1]; loz[c] = lo[2];
-        hix[c] = hi[0]; hiy[c] = hi[1]; hiz[c] = hi[2];
     }
-    out.lox = make_float4(lox[0], lox[1], lox[2], lox[3]);
-    out.hix = make_float4(hix[0], hix[1], hix[2], hix[3]);
-    out.loy = make_float4(loy[0], loy[1], loy[2], loy[3]);
-    out.hiy = make_float4(hiy[0], hiy[1], hiy[2], hiy[3]);
-    out.loz = make_float4(loz[0], loz[1], loz[2], loz[3]);
-    out.hiz = make_float4(hiz[0], hiz[1], hiz[2], hiz[3]);
-    out.child = make_uint4(child[0], child[1], child[2], child[3]);
-    out.pad = make_uint4(0, 0, 0, 0);
-    nodes[task.y] = out;
+    if constexpr (W == 4) {
+        Node4 out;
+        out.lox = make_float4(blo[0][0], blo[1][0], blo[2][0], blo[3][0]);
+        out.hix = make_float4(bhi[0][0], bhi[1][0], bhi[2][0], bhi[3][0]);
+        out.loy = make_float4(blo[0][1], blo[1][1], blo[2][1], blo[3][1]);
+        out.hiy = make_float4(bhi[0][1], bhi[1][1], bhi[2][1], bhi[3][1]);
+        out.loz = make_float4(blo[0][2], blo[1][2], blo[2][2], blo[3][2]);
+        out.hiz = make_float4(bhi[0][2], bhi[1][2], bhi[2][2], bhi[3][2]);
+        out.child = make_uint4(child[0], child[1], child[2], child[3]);
+        out.pad = make_uint4(0, 0, 0, 0);
+        static_cast<Node4*>(nodesOut)[task.y] = out;
+    } else {
+        // quantise: per axis the smallest power-of-two step with which 255
+        // steps cover the union, then each plane rounded outward and checked
+        // with the decoding expression the traversal uses, fma(q, s, origin)
+        // (q * s is exact, so the fused and unfused forms agree)
+        Node8 out;
+        float org[3];
+        uint32_t ebits[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float ulo = INFINITY, uhi = -INFINITY;
+            for (uint32_t c = 0; c < m; ++c) {
+                ulo = fminf(ulo, blo[c][a]);
+                uhi = fmaxf(uhi, bhi[c][a]);
+            }
+            org[a] = ulo;
+            int e = -126;
+            const float ext = uhi - ulo;
+            if (ext > 0.0f) {
+                int ex;
+                (void)frexpf(ext / 255.0f, &ex);  // ext / 255 < 2^ex
+                e = ex < -126 ? -126 : ex;
+            }
+            for (int attempt = 0; attempt < 8; ++attempt) {
+                const float sc = __uint_as_float((uint32_t)(e + 127) << 23);
+                bool ok = true;
+                for (uint32_t c = 0; c < (uint32_t)W; ++c) {
+                    if (c >= m) {
+                        out.qlo[a][c] = 255;
+                        out.qhi[a][c] = 0;
+                        continue;
+                    }
+                    float ql = floorf((blo[c][a] - ulo) / sc);
+                    ql = fminf(fmaxf(ql, 0.0f), 255.0f);
+                    while (ql > 0.0f && fmaf(ql, sc, ulo) > blo[c][a]) ql -= 1.0f;
+                    float qh = ceilf((bhi[c][a] - ulo) / sc);
+                    qh = fmaxf(qh, ql);
+                    while (qh <= 255.0f && fmaf(qh, sc, ulo) < bhi[c][a]) qh += 1.0f;
+                    if (qh > 255.0f) { ok = false; break; }
+                    out.qlo[a][c] = (uint8_t)ql;
+                    out.qhi[a][c] = (uint8_t)qh;
+                }
+                if (ok || e >= 127) break;
+                ++e;
+            }
+            ebits[a] = (uint32_t)(e + 127);
+        }
+        out.origin = make_float4(org[0], org[1], org[2], __uint_as_float(ebits[0] | (ebits[1] << 8) | (ebits[2] << 16)));
+#pragma unroll
+        for (int c = 0; c < W; ++c) out.child[c] = child[c];
+        static_cast<Node8*>(nodesOut)[task.y] = out;
+    }
     atomicMax(&counters[C_MAXDEPTH], task.z);
 }
 
@@ -437,12 +489,17 @@ __global__ __launch_bounds__(256) void k_reorder_tris(const uint32_t* __restrict
 #ifndef WRK_TRACE_WAVES
 #define WRK_TRACE_WAVES 8
 #endif
-template <bool ANY, bool COUNT>
-__global__ __launch_bounds__(256)
-#if WRK_TRACE_WAVES
-__attribute__((amdgpu_waves_per_eu(WRK_TRACE_WAVES, WRK_TRACE_WAVES)))
+#ifndef WRK_TRACE_WAVES8
+#define WRK_TRACE_WAVES8 8
 #endif
-void k_trace(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
+#ifndef WRK_LEAF_ON_STACK8
+#define WRK_LEAF_ON_STACK8 1
+#endif
+template <bool ANY, bool COUNT, int W>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(W == 8 ? WRK_TRACE_WAVES8 : WRK_TRACE_WAVES,
+                                   W == 8 ? WRK_TRACE_WAVES8 : WRK_TRACE_WAVES)))
+void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                                                const float4* __restrict__ org, const float4* __restrict__ dir,
                                                const uint32_t* __restrict__ queue, uint32_t n,
                                                const uint32_t* __restrict__ countDev, uint2* __restrict__ hits,
@@ -451,15 +508,18 @@ void k_trace(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
     __shared__ uint32_t sNode[kStackLds][256];
     __shared__ float sT[kStackLds][256];
     const uint32_t lane = threadIdx.x;
-    const uint32_t wlane = threadIdx.x & 63;
-    const uint64_t ltMask = wlane == 0 ? 0ull : (~0ull >> (64 - wlane));
     uint32_t cnt = n;
     if (countDev) cnt = min(cnt, *countDev);
-    const uint64_t waveId = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
-    const uint64_t chunkBegin = waveId * (64u * raysPerLane);
-    const uint64_t chunkEnd = min<uint64_t>(chunkBegin + 64u * raysPerLane, cnt);
-    uint64_t cursor = chunkBegin;  // wave-uniform
+    // wave-uniform chunk bounds, kept in scalar registers (the launch
+    // guarantees n + 256 * raysPerLane fits 32 bits)
+    const uint32_t waveId = __builtin_amdgcn_readfirstlane((blockIdx.x * 256u + threadIdx.x) >> 6);
+    const uint32_t chunkBegin = waveId * (64u * raysPerLane);
+    const uint32_t chunkEnd = min(chunkBegin + 64u * raysPerLane, cnt);
+    uint32_t cursor = chunkBegin;
     const float kRobust = 1.0000005f;
+    // 8-wide: leaf children are sorted and stacked like inner ones, so the
+    // triangle tests and the node decode never hold registers at once
+    constexpr bool kLeafOnStack = W == 8 && (WRK_LEAF_ON_STACK8 || !ANY);
 
     uint32_t spillN[kStackSpill];
     float spillT[kStackSpill];
@@ -476,7 +536,9 @@ void k_trace(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
         // refill lanes without a ray from the wave's chunk
         uint64_t need = __ballot(!active);
         if (need != 0 && cursor < chunkEnd) {
-            uint64_t qi = cursor + __popcll(need & ltMask);
+            // this lane's rank among the lanes that need a ray
+            const uint32_t qi = cursor + __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
             if (!active && qi < chunkEnd) {
                 ray = queue ? queue[qi] : (uint32_t)qi;
                 o = org[ray];
@@ -499,31 +561,71 @@ void k_trace(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
         if (!active) continue;
 
         // one traversal step: visit `cur` (if any), then pop if needed
-        if (cur != kEmptyChild) {
-            const Node4* nd = nodes + cur;
+        if (kLeafOnStack && cur != kEmptyChild && (cur & kLeafBit)) {
+            // a leaf taken from the stack (or the nearest child): its triangles
+            const uint32_t first = cur & 0x07FFFFFFu;
+            const uint32_t count = ((cur >> 27) & 0xFu) + 1u;
+            for (uint32_t k = 0; k < count; ++k) {
+                const float4* tp = tris + (size_t)(first + k) * 3;
+                if (COUNT) ++triFetches;
+                float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+                float th;
+                if (intersect_tri(rp, v0, v1, v2, tmin, tbest, &th)) {
+                    uint32_t prim = __float_as_uint(v0.w);
+                    if (ANY) { found = true; break; }
+                    if (th < tbest || prim < bestPrim) { tbest = th; bestPrim = prim; }
+                }
+            }
+            cur = kEmptyChild;
+            if (ANY && found) sp = 0;
+        } else if (cur != kEmptyChild) {
             if (COUNT) ++nodeFetches;
-            float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
-            uint4 ch = nd->child;
-            float blo[3][4] = {{lx.x, lx.y, lx.z, lx.w}, {ly.x, ly.y, ly.z, ly.w}, {lz.x, lz.y, lz.z, lz.w}};
-            float bhi[3][4] = {{hx.x, hx.y, hx.z, hx.w}, {hy.x, hy.y, hy.z, hy.w}, {hz.x, hz.y, hz.z, hz.w}};
-            uint32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
-            float key[4];
-            uint32_t code[4];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                float t0x = (blo[0][c] - o.x) * ix, t1x = (bhi[0][c] - o.x) * ix;
-                float t0y = (blo[1][c] - o.y) * iy, t1y = (bhi[1][c] - o.y) * iy;
-                float t0z = (blo[2][c] - o.z) * iz, t1z = (bhi[2][c] - o.z) * iz;
+            float key[W];
+            uint32_t code[W];
+            // slab test of one child box: near distance, or +inf on a miss
+            auto slab = [&](float lx, float ly, float lz, float hx, float hy, float hz, uint32_t cc) -> float {
+                float t0x = (lx - o.x) * ix, t1x = (hx - o.x) * ix;
+                float t0y = (ly - o.y) * iy, t1y = (hy - o.y) * iy;
+                float t0z = (lz - o.z) * iz, t1z = (hz - o.z) * iz;
                 float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
                 float tf = fminf(fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)), tbest) * kRobust;
-                bool hit = (cc[c] != kEmptyChild) && (tn <= tf);
-                key[c] = hit ? tn : INFINITY;
-                code[c] = cc[c];
-            }
-            // leaf children first
+                return (cc != kEmptyChild && tn <= tf) ? tn : INFINITY;
+            };
+            if constexpr (W == 4) {
+                const Node4* nd = static_cast<const Node4*>(nodesIn) + cur;
+                const float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
+                const uint4 ch = nd->child;
+                code[0] = ch.x; code[1] = ch.y; code[2] = ch.z; code[3] = ch.w;
+                key[0] = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, code[0]);
+                key[1] = slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, code[1]);
+                key[2] = slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, code[2]);
+                key[3] = slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, code[3]);
+            } else {
+                // six 16-byte loads: origin | child 0-3 | child 4-7 | 48 quantised planes
+                const uint4* raw = reinterpret_cast<const uint4*>(static_cast<const Node8*>(nodesIn) + cur);
+                const uint4 r0 = raw[0], r1 = raw[1], r2 = raw[2], r3 = raw[3], r4 = raw[4], r5 = raw[5];
+                const float ox = __uint_as_float(r0.x), oy = __uint_as_float(r0.y), oz = __uint_as_float(r0.z);
+                const float sx = __uint_as_float((r0.w & 0xFFu) << 23);
+                const float sy = __uint_as_float(((r0.w >> 8) & 0xFFu) << 23);
+                const float sz = __uint_as_float(((r0.w >> 16) & 0xFFu) << 23);
+                code[0] = r1.x; code[1] = r1.y; code[2] = r1.z; code[3] = r1.w;
+                code[4] = r2.x; code[5] = r2.y; code[6] = r2.z; code[7] = r2.w;
+                // plane words: qlo x (r3.x r3.y) y (r3.z r3.w) z (r4.x r4.y),
+                //              qhi x (r4.z r4.w) y (r5.x r5.y) z (r5.z r5.w)
+                const uint32_t qw[12] = {r3.x, r3.y, r3.z, r3.w, r4.x, r4.y, r4.z, r4.w, r5.x, r5.y, r5.z, r5.w};
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                if (key[c] != INFINITY && (code[c] & kLeafBit)) {
+                for (int c = 0; c < W; ++c) {
+                    const int wi = c >> 2, sh = 8 * (c & 3);
+                    auto q = [&](int word) { return (float)((qw[word + wi] >> sh) & 0xFFu); };
+                    key[c] = slab(fmaf(q(0), sx, ox), fmaf(q(2), sy, oy), fmaf(q(4), sz, oz),
+                                  fmaf(q(6), sx, ox), fmaf(q(8), sy, oy), fmaf(q(10), sz, oz), code[c]);
+
+                }
+            }
+            // leaf children first (unless leaves go through the stack)
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                if (!kLeafOnStack && key[c] != INFINITY && (code[c] & kLeafBit)) {
                     key[c] = INFINITY;
                     uint32_t first = code[c] & 0x07FFFFFFu;
                     uint32_t count = ((code[c] >> 27) & 0xFu) + 1u;
@@ -544,20 +646,38 @@ void k_trace(const Node4* __restrict__ nodes, const float4* __restrict__ tris,
                 cur = kEmptyChild;
                 sp = 0;
             } else {
-                // inner children near to far (sorting network on 4 keys)
+                // inner children near to far (sorting network on W keys)
+                // (leaves just tested may have shortened the ray; with leaves
+                // on the stack the keys are already clamped to the ray)
+                if constexpr (!kLeafOnStack) {
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    if (key[c] > tbest * kRobust) key[c] = INFINITY;
+                    for (int c = 0; c < W; ++c)
+                        if (key[c] > tbest * kRobust) key[c] = INFINITY;
+                }
+// branch-free compare-exchange (selects, no divergent blocks)
 #define WRK_CSWAP(a, b)                                                             \
-    if (key[b] < key[a]) {                                                          \
-        float tk = key[a]; key[a] = key[b]; key[b] = tk;                            \
-        uint32_t tc = code[a]; code[a] = code[b]; code[b] = tc;                     \
+    {                                                                               \
+        const float ka = key[a], kb = key[b];                                       \
+        const uint32_t ca = code[a], cb = code[b];                                  \
+        const bool sw = kb < ka;                                                    \
+        key[a] = sw ? kb : ka; key[b] = sw ? ka : kb;                               \
+        code[a] = sw ? cb : ca; code[b] = sw ? ca : cb;                             \
     }
-                WRK_CSWAP(0, 1) WRK_CSWAP(2, 3) WRK_CSWAP(0, 2) WRK_CSWAP(1, 3) WRK_CSWAP(1, 2)
+                if constexpr (W == 4) {
+                    WRK_CSWAP(0, 1) WRK_CSWAP(2, 3) WRK_CSWAP(0, 2) WRK_CSWAP(1, 3) WRK_CSWAP(1, 2)
+                } else {
+                    // Batcher odd-even merge network, 19 comparators
+                    WRK_CSWAP(0, 1) WRK_CSWAP(2, 3) WRK_CSWAP(4, 5) WRK_CSWAP(6, 7)
+                    WRK_CSWAP(0, 2) WRK_CSWAP(1, 3) WRK_CSWAP(4, 6) WRK_CSWAP(5, 7)
+                    WRK_CSWAP(1, 2) WRK_CSWAP(5, 6)
+                    WRK_CSWAP(0, 4) WRK_CSWAP(1, 5) WRK_CSWAP(2, 6) WRK_CSWAP(3, 7)
+                    WRK_CSWAP(2, 4) WRK_CSWAP(3, 5)
+                    WRK_CSWAP(1, 2) WRK_CSWAP(3, 4) WRK_CSWAP(5, 6)
+                }
 #undef WRK_CSWAP
                 // push far children (farthest first), descend into the nearest
 #pragma unroll
-                for (int c = 3; c >= 1; --c) {
+                for (int c = W - 1; c >= 1; --c) {
                     if (key[c] != INFINITY) {
                         if (sp >= kStackLds) {
                             uint32_t slot = sp & (kStackLds - 1);
@@ -639,7 +759,7 @@ extern "C" int wrk_bvh_query_sizes(uint32_t numTris, uint64_t* nodeBytes, uint64
                                    uint64_t* scratchBytes) {
     WRK_REQUIRE(nodeBytes && triBytes && scratchBytes, "wrk_bvh_query_sizes: null output");
     WRK_REQUIRE(numTris > 0 && numTris < (1u << 27), "wrk_bvh_query_sizes: numTris must be in [1, 2^27)");
-    *nodeBytes = (uint64_t)std::max<uint32_t>(numTris, 1u) * sizeof(Node4);
+    *nodeBytes = (uint64_t)std::max<uint32_t>(numTris, 1u) * (kBvhWidth == 8 ? sizeof(Node8) : sizeof(Node4));
     *triBytes = (uint64_t)numTris * 48u;
     *scratchBytes = layout_for(numTris).total;
     return WRK_OK;
@@ -719,8 +839,8 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
     uint4* tout = tasksB;
     while (numTasks > 0) {
         WRK_HIP_TRY(hipMemsetAsync(counters + C_TASKS_NEXT, 0, 4, s));
-        hipLaunchKernelGGL(k_collapse, dim3(ceil_div(numTasks, 256)), dim3(256), 0, s, tin, numTasks, bnodes, triLo,
-                           triHi, valsB, (Node4*)nodes, counters, tout);
+        hipLaunchKernelGGL(k_collapse<kBvhWidth>, dim3(ceil_div(numTasks, 256)), dim3(256), 0, s, tin, numTasks, bnodes,
+                           triLo, triHi, valsB, nodes, counters, tout);
         WRK_CHECK_LAUNCH();
         WRK_HIP_TRY(hipMemcpyAsync(&numTasks, counters + C_TASKS_NEXT, 4, hipMemcpyDeviceToHost, s));
         WRK_HIP_TRY(hipStreamSynchronize(s));
@@ -737,11 +857,12 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
     out->numNodes = hc[C_NODES];
     out->numTris = n;
     out->maxDepth = hc[C_MAXDEPTH];
+    out->nodeBytes = kBvhWidth == 8 ? (uint32_t)sizeof(Node8) : (uint32_t)sizeof(Node4);
     for (int a = 0; a < 3; ++a) {
         out->sceneLo[a] = unordered(hc[C_BMIN + a]);
         out->sceneHi[a] = unordered(hc[C_BMAX + a]);
     }
-    if (3u * out->maxDepth + 4u > kStackTotal) {
+    if ((kBvhWidth - 1u) * out->maxDepth + 8u > kStackTotal) {
         set_error("wrk_bvh_build: BVH depth " + std::to_string(out->maxDepth) + " exceeds the traversal stack");
         return WRK_ERR_CAPACITY;
     }
@@ -768,19 +889,34 @@ static int launch_trace(const wrk_bvh* bvh, const void* org4, const void* dir4, 
                         const uint32_t* countDev, uint32_t* hits, uint8_t* occl, unsigned long long* fetches,
                         void* stream) {
     WRK_REQUIRE(bvh && bvh->nodes && bvh->tris && org4 && dir4, "wrk_trace: null argument");
-    WRK_REQUIRE(3u * bvh->maxDepth + 4u <= kStackTotal, "wrk_trace: BVH deeper than the traversal stack");
+    WRK_REQUIRE(bvh->nodeBytes == sizeof(Node8) || bvh->nodeBytes == sizeof(Node4), "wrk_trace: unknown node format");
+    WRK_REQUIRE((bvh->nodeBytes == sizeof(Node8) ? 7u : 3u) * bvh->maxDepth + 8u <= kStackTotal,
+                "wrk_trace: BVH deeper than the traversal stack");
     if (n == 0) return WRK_OK;
     const uint32_t rpl = rays_per_lane(n);
+    WRK_REQUIRE((uint64_t)n + 256ull * rpl <= 0xFFFFFFFFull, "wrk_trace: too many rays for one launch");
     dim3 grid(ceil_div(n, 256u * rpl)), block(256);
     hipStream_t s = as_stream(stream);
+    const void* nd = (const void*)bvh->nodes;
+    const bool w8 = bvh->nodeBytes == sizeof(Node8);
     if (fetches) {
-        hipLaunchKernelGGL((k_trace<ANY, true>), grid, block, 0, s, (const Node4*)bvh->nodes, (const float4*)bvh->tris,
-                           (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl, fetches,
-                           rpl);
+        if (w8)
+            hipLaunchKernelGGL((k_trace<ANY, true, 8>), grid, block, 0, s, nd, (const float4*)bvh->tris,
+                               (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl,
+                               fetches, rpl);
+        else
+            hipLaunchKernelGGL((k_trace<ANY, true, 4>), grid, block, 0, s, nd, (const float4*)bvh->tris,
+                               (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl,
+                               fetches, rpl);
     } else {
-        hipLaunchKernelGGL((k_trace<ANY, false>), grid, block, 0, s, (const Node4*)bvh->nodes, (const float4*)bvh->tris,
-                           (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl, nullptr,
-                           rpl);
+        if (w8)
+            hipLaunchKernelGGL((k_trace<ANY, false, 8>), grid, block, 0, s, nd, (const float4*)bvh->tris,
+                               (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl,
+                               nullptr, rpl);
+        else
+            hipLaunchKernelGGL((k_trace<ANY, false, 4>), grid, block, 0, s, nd, (const float4*)bvh->tris,
+                               (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl,
+                               nullptr, rpl);
     }
     WRK_CHECK_LAUNCH();
     return WRK_OK;

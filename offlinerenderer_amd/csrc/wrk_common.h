@@ -48,6 +48,24 @@ struct alignas(16) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "Node4 must be 128 bytes");
 
+// Compressed 8-wide node (96 bytes): child boxes quantised to 8 bits per
+// plane on a per-axis power-of-two grid anchored at `origin` (decoded as
+// origin + q * 2^e, rounded outward at build time so the decoded box always
+// contains the exact one). origin.w holds the three biased float exponents
+// (bits 0-7 x, 8-15 y, 16-23 z).
+struct alignas(16) Node8 {
+    float4 origin;
+    uint32_t child[8];
+    uint8_t qlo[3][8];
+    uint8_t qhi[3][8];
+};
+static_assert(sizeof(Node8) == 96, "Node8 must be 96 bytes");
+
+#ifndef WRK_BVH_WIDTH
+#define WRK_BVH_WIDTH 8
+#endif
+constexpr uint32_t kBvhWidth = WRK_BVH_WIDTH;
+
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
 constexpr uint32_t kLeafMax = 4;        // triangles per leaf

@@ -1,7 +1,7 @@
-"""Invariants of the GPU LBVH (4-wide nodes): every triangle appears in
-exactly one leaf, leaf boxes contain their triangles, child boxes nest in
-their parent's entry, primitive ids form a permutation, depth is as
-reported."""
+"""Invariants of the GPU LBVH (compressed 8-wide or 4-wide nodes): every
+triangle appears in exactly one leaf, decoded leaf boxes contain their
+triangles, child boxes nest in their parent's entry, primitive ids form a
+permutation, depth is as reported."""
 import ctypes as C
 
 import numpy as np
@@ -28,15 +28,41 @@ def d2h(ptr, nbytes):
     return buf
 
 
+def decode_nodes(b):
+    """(lo, hi) as (node, child, axis) float32 and child codes (node, child)."""
+    raw = d2h(b.nodes, b.numNodes * b.nodeBytes)
+    if b.nodeBytes == 128:
+        f = raw.view(np.float32).reshape(b.numNodes, 32)
+        u = raw.view(np.uint32).reshape(b.numNodes, 32)
+        lo = np.stack([f[:, 0:4], f[:, 8:12], f[:, 16:20]], axis=2)
+        hi = np.stack([f[:, 4:8], f[:, 12:16], f[:, 20:24]], axis=2)
+        return lo, hi, u[:, 24:28]
+    assert b.nodeBytes == 96
+    rec = raw.reshape(b.numNodes, 96)
+    origin = rec[:, 0:12].copy().view(np.float32)                 # node, axis
+    ebits = rec[:, 12:15].astype(np.uint32)                         # biased exponents x, y, z
+    assert np.all((ebits >= 1) & (ebits <= 254)) and np.all(rec[:, 15] == 0)
+    scale = (ebits << 23).view(np.float32)                          # 2^(e - 127), exact
+    child = rec[:, 16:48].copy().view(np.uint32)
+    qlo = rec[:, 48:72].reshape(-1, 3, 8).transpose(0, 2, 1).astype(np.float32)
+    qhi = rec[:, 72:96].reshape(-1, 3, 8).transpose(0, 2, 1).astype(np.float32)
+    # the traversal's decode: origin + q * scale in float32 (product exact)
+    lo = origin[:, None, :] + qlo * scale[:, None, :]
+    hi = origin[:, None, :] + qhi * scale[:, None, :]
+    used = child != 0xFFFFFFFF
+    # empty slots decode to an inverted box; used ones to a proper box
+    assert np.all(qlo[~used] == 255) and np.all(qhi[~used] == 0)
+    assert np.all(qlo[used] <= qhi[used])
+    # children fill the low slots
+    assert np.all(used[:, :-1] | ~used[:, 1:])
+    return lo.astype(np.float32), hi.astype(np.float32), child
+
+
 def check_bvh(desc):
     ps = G.product_scene(desc)
     b = ps.getBVH()
-    raw = d2h(b.nodes, b.numNodes * 128)
-    f = raw.view(np.float32).reshape(b.numNodes, 32)
-    u = raw.view(np.uint32).reshape(b.numNodes, 32)
-    lo = np.stack([f[:, 0:4], f[:, 8:12], f[:, 16:20]], axis=2)   # node, child, axis
-    hi = np.stack([f[:, 4:8], f[:, 12:16], f[:, 20:24]], axis=2)
-    child = u[:, 24:28]
+    lo, hi, child = decode_nodes(b)
+    width = child.shape[1]
     tris = d2h(b.tris, b.numTris * 48).view(np.float32).reshape(b.numTris, 3, 4)
     prim = tris[:, 0, 3].view(np.uint32)
     V, F, _ = flatten(desc)
@@ -44,28 +70,41 @@ def check_bvh(desc):
     assert np.allclose(tris[:, :, :3], V[F[prim]])
     covered = np.zeros(b.numTris, dtype=np.int32)
     seen = np.zeros(b.numNodes, dtype=np.int32)
-    stack = [(0, 1)]
+    order, stack = [], [(0, 1)]
     maxd = 0
     while stack:
         node, depth = stack.pop()
         seen[node] += 1
+        order.append(node)
         maxd = max(maxd, depth)
-        for c in range(4):
+        for c in range(width):
+            code = int(child[node, c])
+            if code != 0xFFFFFFFF and not code & 0x80000000:
+                stack.append((code, depth + 1))
+    # children before parents: exact bounds of the triangles under each node
+    sub_lo = np.full((b.numNodes, 3), np.inf, dtype=np.float32)
+    sub_hi = np.full((b.numNodes, 3), -np.inf, dtype=np.float32)
+    for node in reversed(order):
+        for c in range(width):
             code = int(child[node, c])
             if code == 0xFFFFFFFF:
                 continue
-            blo, bhi = lo[node, c], hi[node, c]
             if code & 0x80000000:
                 first, count = code & 0x07FFFFFF, ((code >> 27) & 0xF) + 1
                 covered[first:first + count] += 1
                 v = tris[first:first + count, :, :3].reshape(-1, 3)
-                assert np.all(v >= blo) and np.all(v <= bhi)
+                clo, chi = v.min(0), v.max(0)
             else:
-                # the child's own children boxes nest inside this entry
-                clo = lo[code][child[code] != 0xFFFFFFFF].min(0)
-                chi = hi[code][child[code] != 0xFFFFFFFF].max(0)
-                assert np.all(clo >= blo) and np.all(chi <= bhi)
-                stack.append((code, depth + 1))
+                clo, chi = sub_lo[code], sub_hi[code]
+                if width == 4:
+                    # uncompressed boxes: the child's entries nest in this one
+                    used = child[code] != 0xFFFFFFFF
+                    assert np.all(lo[code][used].min(0) >= lo[node, c])
+                    assert np.all(hi[code][used].max(0) <= hi[node, c])
+            # every entry holds all triangles beneath it
+            assert np.all(clo >= lo[node, c]) and np.all(chi <= hi[node, c]), (node, c)
+            sub_lo[node] = np.minimum(sub_lo[node], clo)
+            sub_hi[node] = np.maximum(sub_hi[node], chi)
     assert np.all(covered == 1)
     assert np.all(seen == 1)
     assert maxd == b.maxDepth
