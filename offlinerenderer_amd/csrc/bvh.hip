@@ -877,8 +877,15 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
     return WRK_OK;
 }
 
-// Rays per lane for a launch of n rays: enough waves for two full rounds of
-// every wave slot (CUs x 4 SIMDs x 8 waves) before any lane loops over rays.
+#ifndef WRK_TRACE_ROUNDS
+#define WRK_TRACE_ROUNDS 4
+#endif
+// Rays per lane for a launch of n rays: up to kRaysPerLane, while the grid
+// still fills every wave slot (CUs x 4 SIMDs x 8 waves) WRK_TRACE_ROUNDS
+// times over. Lanes that refill from their wave's chunk keep more lanes
+// busy than one ray per lane (a third of the lanes are active per VALU
+// instruction in the closest-hit kernel); at 39M rays per launch 8 rays per
+// lane gave +12%, 4 or 16 within 2% of it.
 static uint32_t rays_per_lane(uint32_t n) {
     static int cus = 0;
     if (cus == 0) {
@@ -887,7 +894,7 @@ static uint32_t rays_per_lane(uint32_t n) {
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 256;
     }
-    uint64_t slots = (uint64_t)cus * 4u * 8u * 64u * 2u;
+    uint64_t slots = (uint64_t)cus * 4u * 8u * 64u * WRK_TRACE_ROUNDS;
     uint64_t r = n / slots;
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(r, kRaysPerLane));
 }

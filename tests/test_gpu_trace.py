@@ -76,3 +76,25 @@ def test_large_incoherent_scene_trace(gpu, oracle):
     t_b, p_b = oracle.trace_closest(os_, org4[:300], dir4[:300], use_bvh=False)
     _compare(t_g[:300], p_g[:300], t_b, p_b)
     assert np.array_equal(G.trace_any(bvh, org4, dir4), oracle.trace_any(os_, org4, dir4, use_bvh=True))
+
+
+def test_many_rays_per_lane_match_one_per_lane(gpu, oracle):
+    """A launch of 9.4M rays refills lanes from each wave's chunk (several
+    rays per lane on a 256-CU device); 1M-ray launches take one ray per lane.
+    Both must give the same hits, and a slice is checked against the oracle."""
+    d = scenes.material_scene(96, 54, 1, tex_size=64)
+    ps = G.product_scene(d)
+    bvh = ps.getBVH()
+    n, m = 9 << 20, 1 << 20
+    org4, dir4 = make_rays(d, n, 11)
+    t_all, p_all = G.trace_closest(bvh, org4, dir4)
+    occ_all = G.trace_any(bvh, org4, dir4)
+    for k in range(0, n, m):
+        t_k, p_k = G.trace_closest(bvh, org4[k:k + m], dir4[k:k + m])
+        _compare(t_all[k:k + m], p_all[k:k + m], t_k, p_k)
+        assert np.array_equal(occ_all[k:k + m], G.trace_any(bvh, org4[k:k + m], dir4[k:k + m]))
+    os_ = oracle.build_oracle_scene(d, bvh=True)
+    sl = slice(n - 20_000, n)
+    t_o, p_o = oracle.trace_closest(os_, org4[sl], dir4[sl], use_bvh=True)
+    _compare(t_all[sl], p_all[sl], t_o, p_o)
+    assert np.array_equal(occ_all[sl], oracle.trace_any(os_, org4[sl], dir4[sl], use_bvh=True))
