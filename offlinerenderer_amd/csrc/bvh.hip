@@ -495,6 +495,15 @@ __global__ __launch_bounds__(256) void k_reorder_tris(const uint32_t* __restrict
 #ifndef WRK_TRACE_WAVES
 #define WRK_TRACE_WAVES 8
 #endif
+#ifndef WRK_LEAF_BATCH
+#define WRK_LEAF_BATCH 1
+#endif
+#ifndef WRK_LEAF_BATCH_NUM
+#define WRK_LEAF_BATCH_NUM 5
+#endif
+#ifndef WRK_LEAF_BATCH_DEN
+#define WRK_LEAF_BATCH_DEN 8
+#endif
 template <bool ANY, bool COUNT, int W>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(WRK_TRACE_WAVES, WRK_TRACE_WAVES)))
@@ -521,7 +530,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
     uint32_t spillN[kStackSpill];
     float spillT[kStackSpill];
     bool active = false;
-    uint32_t ray = 0, sp = 0, cur = kEmptyChild, bestPrim = 0xFFFFFFFFu;
+    uint32_t ray = 0, sp = 0, cur = kEmptyChild, leaf = kEmptyChild, bestPrim = 0xFFFFFFFFu;
     float4 o = make_float4(0, 0, 0, 0);
     float tmin = 0.0f, tbest = 0.0f, ix = 0.0f, iy = 0.0f, iz = 0.0f;
     RayPre rp;
@@ -549,6 +558,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                 iz = 1.0f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
                 sp = 0;
                 cur = 0;  // root
+                leaf = kEmptyChild;
                 found = false;
                 active = true;
             }
@@ -557,11 +567,29 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
         if (__ballot(active) == 0) break;
         if (!active) continue;
 
-        // one traversal step: visit `cur` (if any), then pop if needed
-        if (kLeafOnStack && cur != kEmptyChild && (cur & kLeafBit)) {
-            // a leaf taken from the stack (or the nearest child): its triangles
-            const uint32_t first = cur & 0x07FFFFFFu;
-            const uint32_t count = ((cur >> 27) & 0xFu) + 1u;
+        // one traversal step: a wave-wide leaf step or node step, then pop
+        bool leafStep = false;
+        if constexpr (kLeafOnStack && WRK_LEAF_BATCH) {
+            // Leaves found while descending wait in `leaf` (one per lane)
+            // while the lane keeps visiting inner nodes; the wave tests its
+            // pending leaves together once 5/8 of its active lanes hold one,
+            // or when no lane has an inner node left to visit. This keeps the
+            // node steps and the leaf steps each on most lanes of the wave
+            // (measured: 1/4 -> 1/2 -> 5/8 -> 3/4 -> all of the lanes gave
+            // material 1269, 1411, 1434, 1420, 1259 Mpaths/s; without the
+            // batching, a lane tests a leaf as soon as it reaches one, 1140).
+            const uint32_t nAct = __popcll(__ballot(true));
+            const uint32_t nPend = __popcll(__ballot(leaf != kEmptyChild));
+            const bool canStep = __ballot(cur != kEmptyChild && !(cur & kLeafBit)) != 0;
+            leafStep = nPend != 0 && (!canStep || nPend * WRK_LEAF_BATCH_DEN >= nAct * WRK_LEAF_BATCH_NUM);
+        } else if constexpr (kLeafOnStack) {
+            if (cur != kEmptyChild && (cur & kLeafBit)) { leaf = cur; cur = kEmptyChild; leafStep = true; }
+        }
+        if (leafStep) {
+            if (leaf != kEmptyChild) {
+            // a pending leaf: its triangles
+            const uint32_t first = leaf & 0x07FFFFFFu;
+            const uint32_t count = ((leaf >> 27) & 0xFu) + 1u;
             for (uint32_t k = 0; k < count; ++k) {
                 const float4* tp = tris + (size_t)(first + k) * 3;
                 if (COUNT) ++triFetches;
@@ -573,9 +601,10 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                     if (th < tbest || prim < bestPrim) { tbest = th; bestPrim = prim; }
                 }
             }
-            cur = kEmptyChild;
-            if (ANY && found) sp = 0;
-        } else if (cur != kEmptyChild) {
+            leaf = kEmptyChild;
+            if (ANY && found) { cur = kEmptyChild; sp = 0; }
+            }
+        } else if (cur != kEmptyChild && !(kLeafOnStack && (cur & kLeafBit))) {
             if (COUNT) ++nodeFetches;
             float key[W];
             uint32_t code[W];
@@ -700,6 +729,10 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                 cur = (key[0] != INFINITY) ? code[0] : kEmptyChild;
             }
         }
+        if (kLeafOnStack && WRK_LEAF_BATCH && cur != kEmptyChild && (cur & kLeafBit) && leaf == kEmptyChild) {
+            leaf = cur;  // the nearest child is a leaf: it waits for a leaf step
+            cur = kEmptyChild;
+        }
         if (cur == kEmptyChild) {
             bool done = true;
             while (sp > 0) {
@@ -711,8 +744,17 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                     sNode[slot][lane] = spillN[sp - kStackLds];
                     sT[slot][lane] = spillT[sp - kStackLds];
                 }
-                if (tt <= tbest * kRobust) { cur = nn; done = false; break; }
+                if (tt <= tbest * kRobust) {
+                    if (kLeafOnStack && WRK_LEAF_BATCH && (nn & kLeafBit) && leaf == kEmptyChild) {
+                        leaf = nn;  // park it and look for an inner node
+                        continue;
+                    }
+                    cur = nn;
+                    done = false;
+                    break;
+                }
             }
+            if (kLeafOnStack && leaf != kEmptyChild) done = false;
             if (done) {
                 if (ANY) occl[ray] = found ? 1 : 0;
                 else hits[ray] = make_uint2(__float_as_uint(tbest), bestPrim);
