@@ -499,10 +499,10 @@ __global__ __launch_bounds__(256) void k_reorder_tris(const uint32_t* __restrict
 #define WRK_LEAF_BATCH 1
 #endif
 #ifndef WRK_LEAF_BATCH_NUM
-#define WRK_LEAF_BATCH_NUM 5
+#define WRK_LEAF_BATCH_NUM 1
 #endif
 #ifndef WRK_LEAF_BATCH_DEN
-#define WRK_LEAF_BATCH_DEN 8
+#define WRK_LEAF_BATCH_DEN 2
 #endif
 template <bool ANY, bool COUNT, int W>
 __global__ __launch_bounds__(256)
@@ -572,12 +572,14 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
         if constexpr (kLeafOnStack && WRK_LEAF_BATCH) {
             // Leaves found while descending wait in `leaf` (one per lane)
             // while the lane keeps visiting inner nodes; the wave tests its
-            // pending leaves together once 5/8 of its active lanes hold one,
-            // or when no lane has an inner node left to visit. This keeps the
-            // node steps and the leaf steps each on most lanes of the wave
-            // (measured: 1/4 -> 1/2 -> 5/8 -> 3/4 -> all of the lanes gave
-            // material 1269, 1411, 1434, 1420, 1259 Mpaths/s; without the
-            // batching, a lane tests a leaf as soon as it reaches one, 1140).
+            // pending leaves together once half of its active lanes hold
+            // one, or when no lane has an inner node left to visit. This
+            // keeps the node steps and the leaf steps each on most lanes of
+            // the wave (measured with 4-triangle leaves: 1/4, 1/2, 5/8, 3/4,
+            // all of the lanes gave material 1269, 1411, 1434, 1420, 1259
+            // Mpaths/s; without the batching, a lane tests a leaf as soon as
+            // it reaches one, 1140; with 2-triangle leaves 1/2, 5/8, 3/4 gave
+            // material 1486, 1485, 1446 and large 1061, 1038, 979).
             const uint32_t nAct = __popcll(__ballot(true));
             const uint32_t nPend = __popcll(__ballot(leaf != kEmptyChild));
             const bool canStep = __ballot(cur != kEmptyChild && !(cur & kLeafBit)) != 0;

@@ -68,7 +68,14 @@ constexpr uint32_t kBvhWidth = WRK_BVH_WIDTH;
 
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
-constexpr uint32_t kLeafMax = 4;        // triangles per leaf
+#ifndef WRK_LEAF_MAX
+#define WRK_LEAF_MAX 2
+#endif
+// triangles per leaf (at most 16: 4-bit count). With 8-wide nodes and
+// batched leaf tests, 2 measured best: leaves of 1, 2, 3, 4, 6, 8 gave
+// material 1436, 1486, 1457, 1434, 1355, 1285 Mpaths/s (leaf-batch
+// threshold 5/8 for all six).
+constexpr uint32_t kLeafMax = WRK_LEAF_MAX;
 constexpr float kBoxPadRel = 9.5367431640625e-07f;  // 2^-20 outward box padding
 #ifndef WRK_STACK_LDS
 #define WRK_STACK_LDS 8
