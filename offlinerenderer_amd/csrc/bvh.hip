@@ -290,26 +290,94 @@ __global__ __launch_bounds__(256) void k_expand_level(const uint32_t* __restrict
     if (num > 1) next[pos + 1] = k1;
 }
 
-// Bottom-up: boxes of one level from its children, whose boxes (deeper
-// levels) were written by earlier launches.
+__device__ __forceinline__ float half_area(const float lo[3], const float hi[3]) {
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// SAH costs for the collapse to W-wide nodes (Ylitie, Karras, Laine 2017,
+// "Efficient incoherent ray traversal on GPUs through compressed wide
+// BVHs", section 4.1). For a binary node n and i = 1..W, C(n, i) is the
+// least cost of representing n's subtree as at most i children of a wide
+// node: C(n, 1) = min(leaf cost, A(n) * c_node + best split of W slots
+// between n's two children), C(n, i > 1) = min(C(n, i - 1), best split of
+// i slots). A is the surface area. The collapse follows the argmins top-down,
+// recomputing the same float sums, so its choices match the table exactly.
+#ifndef WRK_SAH_COLLAPSE
+#define WRK_SAH_COLLAPSE 1
+#endif
+#ifndef WRK_SAH_CPRIM
+#define WRK_SAH_CPRIM 0.3f
+#endif
+constexpr float kSahNode = 1.0f;
+
+__device__ __forceinline__ float sah_leaf_cost(float area, uint32_t count) {
+    return count <= kLeafMax ? area * (WRK_SAH_CPRIM * (float)count) : INFINITY;
+}
+
+// cost row of a child reference: a single triangle costs its leaf cost for
+// every i; an inner binary node has its row in `cost`
+template <int W>
+__device__ __forceinline__ void sah_row(const float* cost, const float4* triLo, const float4* triHi,
+                                        const uint32_t* vals, uint32_t ref, float row[W]) {
+    if (ref & kLeafBit) {
+        uint32_t t = vals[ref & ~kLeafBit];
+        float4 a = triLo[t], b = triHi[t];
+        float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
+        float c = sah_leaf_cost(half_area(lo, hi), 1u);
+#pragma unroll
+        for (int i = 0; i < W; ++i) row[i] = c;
+    } else {
+#pragma unroll
+        for (int i = 0; i < W; ++i) row[i] = cost[(size_t)ref * W + i];
+    }
+}
+
+// best split of i slots (i >= 2) between two children: min over k of
+// cl(k) + cr(i - k); *kBest gets the smallest minimising k
+template <int W>
+__device__ __forceinline__ float sah_split(const float cl[W], const float cr[W], int i, int* kBest) {
+    float best = INFINITY;
+    int kb = 1;
+    for (int k = 1; k < i; ++k) {
+        float c = cl[k - 1] + cr[i - k - 1];
+        if (c < best) { best = c; kb = k; }
+    }
+    *kBest = kb;
+    return best;
+}
+
+// Bottom-up: boxes (and SAH cost rows) of one level from its children, whose
+// boxes (deeper levels) were written by earlier launches.
+template <int W>
 __global__ __launch_bounds__(256) void k_refit_level(const uint32_t* __restrict__ level, uint32_t count,
                                                      BinNode* bnodes, const float4* __restrict__ triLo,
                                                      const float4* __restrict__ triHi,
-                                                     const uint32_t* __restrict__ vals) {
+                                                     const uint32_t* __restrict__ vals, float* cost) {
     uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= count) return;
-    BinNode* b = &bnodes[level[i]];
+    const uint32_t node = level[i];
+    BinNode* b = &bnodes[node];
     uint32_t l = __float_as_uint(b->lo.w), r = __float_as_uint(b->hi.w);
     float llo[3], lhi[3], rlo[3], rhi[3];
     child_box(bnodes, triLo, triHi, vals, l, llo, lhi);
     child_box(bnodes, triLo, triHi, vals, r, rlo, rhi);
-    b->lo = make_float4(fminf(llo[0], rlo[0]), fminf(llo[1], rlo[1]), fminf(llo[2], rlo[2]), __uint_as_float(l));
-    b->hi = make_float4(fmaxf(lhi[0], rhi[0]), fmaxf(lhi[1], rhi[1]), fmaxf(lhi[2], rhi[2]), __uint_as_float(r));
-}
-
-__device__ __forceinline__ float half_area(const float lo[3], const float hi[3]) {
-    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
-    return dx * dy + dy * dz + dz * dx;
+    float lo[3] = {fminf(llo[0], rlo[0]), fminf(llo[1], rlo[1]), fminf(llo[2], rlo[2])};
+    float hi[3] = {fmaxf(lhi[0], rhi[0]), fmaxf(lhi[1], rhi[1]), fmaxf(lhi[2], rhi[2])};
+    b->lo = make_float4(lo[0], lo[1], lo[2], __uint_as_float(l));
+    b->hi = make_float4(hi[0], hi[1], hi[2], __uint_as_float(r));
+    if (WRK_SAH_COLLAPSE) {
+        float cl[W], cr[W], row[W];
+        sah_row<W>(cost, triLo, triHi, vals, l, cl);
+        sah_row<W>(cost, triLo, triHi, vals, r, cr);
+        const float area = half_area(lo, hi);
+        int k;
+        const float inner = sah_split<W>(cl, cr, W, &k) + area * kSahNode;
+        row[0] = fminf(sah_leaf_cost(area, b->last - b->first + 1u), inner);
+        for (int j = 2; j <= W; ++j) row[j - 1] = fminf(row[j - 2], sah_split<W>(cl, cr, j, &k));
+#pragma unroll
+        for (int j = 0; j < W; ++j) cost[(size_t)node * W + j] = row[j];
+    }
 }
 
 // Collapse one binary subtree root into a W-wide node: repeatedly open the
@@ -319,7 +387,8 @@ template <int W>
 __global__ __launch_bounds__(256) void k_collapse(const uint4* __restrict__ tasksIn, uint32_t numTasks,
                                                   const BinNode* __restrict__ bnodes, const float4* __restrict__ triLo,
                                                   const float4* __restrict__ triHi, const uint32_t* __restrict__ vals,
-                                                  void* nodesOut, uint32_t* counters, uint4* tasksOut) {
+                                                  const float* __restrict__ cost, void* nodesOut, uint32_t* counters,
+                                                  uint4* tasksOut) {
     uint32_t ti = blockIdx.x * 256 + threadIdx.x;
     if (ti >= numTasks) return;
     uint4 task = tasksIn[ti];
@@ -329,16 +398,50 @@ __global__ __launch_bounds__(256) void k_collapse(const uint4* __restrict__ task
         sceneExtent = fmaxf(sceneExtent, unordered(counters[C_BMAX + a]) - unordered(counters[C_BMIN + a]));
     uint32_t ref[W];
     uint32_t m = 0;
+    auto count_of = [&](uint32_t r) -> uint32_t {
+        return (r & kLeafBit) ? 1u : (bnodes[r].last - bnodes[r].first + 1u);
+    };
     if (task.x & kLeafBit) {  // single-triangle scene
         ref[m++] = task.x;
+    } else if (WRK_SAH_COLLAPSE) {
+        // follow the SAH table: split W slots between the root's children,
+        // then each (subtree, slots) pair either into one child (a leaf or
+        // an inner node, as its cost row decided) or on into its children
+        uint32_t sr[2 * W], sj[2 * W];
+        int top = 0;
+        {
+            const uint32_t l = __float_as_uint(bnodes[task.x].lo.w), r = __float_as_uint(bnodes[task.x].hi.w);
+            float cl[W], cr[W];
+            sah_row<W>(cost, triLo, triHi, vals, l, cl);
+            sah_row<W>(cost, triLo, triHi, vals, r, cr);
+            int k;
+            (void)sah_split<W>(cl, cr, W, &k);
+            sr[top] = r; sj[top++] = (uint32_t)(W - k);
+            sr[top] = l; sj[top++] = (uint32_t)k;
+        }
+        while (top > 0) {
+            --top;
+            const uint32_t r = sr[top], j = sj[top];
+            if (r & kLeafBit) { ref[m++] = r; continue; }
+            float row[W];
+            sah_row<W>(cost, triLo, triHi, vals, r, row);
+            uint32_t jj = 1;  // fewest slots that reach C(r, j)
+            while (jj < j && row[jj - 1] != row[j - 1]) ++jj;
+            if (jj == 1) { ref[m++] = r; continue; }
+            const uint32_t cl_ = __float_as_uint(bnodes[r].lo.w), cr_ = __float_as_uint(bnodes[r].hi.w);
+            float cl[W], cr[W];
+            sah_row<W>(cost, triLo, triHi, vals, cl_, cl);
+            sah_row<W>(cost, triLo, triHi, vals, cr_, cr);
+            int k;
+            (void)sah_split<W>(cl, cr, (int)jj, &k);
+            sr[top] = cr_; sj[top++] = jj - (uint32_t)k;
+            sr[top] = cl_; sj[top++] = (uint32_t)k;
+        }
     } else {
         ref[m++] = __float_as_uint(bnodes[task.x].lo.w);
         ref[m++] = __float_as_uint(bnodes[task.x].hi.w);
     }
-    auto count_of = [&](uint32_t r) -> uint32_t {
-        return (r & kLeafBit) ? 1u : (bnodes[r].last - bnodes[r].first + 1u);
-    };
-    while (m < (uint32_t)W) {
+    while (!WRK_SAH_COLLAPSE && m < (uint32_t)W) {
         int best = -1;
         float bestA = -1.0f;
         for (uint32_t c = 0; c < m; ++c) {
@@ -374,7 +477,15 @@ __global__ __launch_bounds__(256) void k_collapse(const uint4* __restrict__ task
             const BinNode& b = bnodes[r];
             lo[0] = b.lo.x; lo[1] = b.lo.y; lo[2] = b.lo.z; hi[0] = b.hi.x; hi[1] = b.hi.y; hi[2] = b.hi.z;
             uint32_t cnt = b.last - b.first + 1u;
-            if (cnt <= kLeafMax) {
+            bool asLeaf = cnt <= kLeafMax;
+            if (WRK_SAH_COLLAPSE && asLeaf) {
+                // a leaf where its cost row chose the leaf cost
+                float row[W];
+                sah_row<W>(cost, triLo, triHi, vals, r, row);
+                const float blo[3] = {b.lo.x, b.lo.y, b.lo.z}, bhi[3] = {b.hi.x, b.hi.y, b.hi.z};
+                asLeaf = row[0] == sah_leaf_cost(half_area(blo, bhi), cnt);
+            }
+            if (asLeaf) {
                 child[c] = kLeafBit | ((cnt - 1u) << 27) | b.first;
             } else {
                 uint32_t slot = atomicAdd(&counters[C_NODES], 1u);
@@ -776,7 +887,8 @@ using namespace wrk;
 
 namespace {
 struct BuildLayout {
-    size_t triLo, triHi, keysA, keysB, valsA, valsB, bnodes, levels, partials, tasksA, tasksB, counters, sortTemp;
+    size_t triLo, triHi, keysA, keysB, valsA, valsB, bnodes, levels, partials, tasksA, tasksB, counters, costs,
+        sortTemp;
     size_t sortTempBytes, total;
 };
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -797,6 +909,7 @@ BuildLayout layout_for(uint32_t n) {
     L.tasksA = take(nn * 16);
     L.tasksB = take(nn * 16);
     L.counters = take(C_COUNT * 4);
+    L.costs = take(WRK_SAH_COLLAPSE ? nn * kBvhWidth * 4 : 4);
     size_t tmp = 0;
     (void)rocprim::radix_sort_pairs(nullptr, tmp, (MortonKey*)nullptr, (MortonKey*)nullptr, (uint32_t*)nullptr,
                                     (uint32_t*)nullptr, (size_t)nn, 0u, kMortonBits);
@@ -837,6 +950,7 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
     uint4* tasksA = (uint4*)(base + L.tasksA);
     uint4* tasksB = (uint4*)(base + L.tasksB);
     uint32_t* counters = (uint32_t*)(base + L.counters);
+    float* costs = (float*)(base + L.costs);
     const uint32_t n = numTris;
     const uint32_t g = ceil_div(n, 256);
 
@@ -876,8 +990,8 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
         // ... then boxes bottom-up, one launch per level
         for (size_t L0 = off.size() - 1; L0-- > 0;) {
             uint32_t cnt = off[L0 + 1] - off[L0];
-            hipLaunchKernelGGL(k_refit_level, dim3(ceil_div(cnt, 256)), dim3(256), 0, s, levels + off[L0], cnt, bnodes,
-                               triLo, triHi, valsB);
+            hipLaunchKernelGGL(k_refit_level<kBvhWidth>, dim3(ceil_div(cnt, 256)), dim3(256), 0, s, levels + off[L0],
+                               cnt, bnodes, triLo, triHi, valsB, costs);
         }
         WRK_CHECK_LAUNCH();
     }
@@ -892,7 +1006,7 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
     while (numTasks > 0) {
         WRK_HIP_TRY(hipMemsetAsync(counters + C_TASKS_NEXT, 0, 4, s));
         hipLaunchKernelGGL(k_collapse<kBvhWidth>, dim3(ceil_div(numTasks, 256)), dim3(256), 0, s, tin, numTasks, bnodes,
-                           triLo, triHi, valsB, nodes, counters, tout);
+                           triLo, triHi, valsB, (const float*)costs, nodes, counters, tout);
         WRK_CHECK_LAUNCH();
         WRK_HIP_TRY(hipMemcpyAsync(&numTasks, counters + C_TASKS_NEXT, 4, hipMemcpyDeviceToHost, s));
         WRK_HIP_TRY(hipStreamSynchronize(s));
