@@ -6,6 +6,7 @@
 // optixTrace (libWR/kernels/trace.cu:27-51,127-151).
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cstring>
@@ -27,6 +28,7 @@ enum Counter : uint32_t {
     C_NODES = 0,      // 4-wide nodes allocated
     C_TASKS_NEXT = 1, // tasks produced for the next level
     C_MAXDEPTH = 2,
+    C_BIN_NODES = 3,  // binary nodes created by the agglomerative build
     C_CMIN = 4,       // ordered-uint centroid min (3)
     C_CMAX = 8,       // ordered-uint centroid max (3)
     C_BMIN = 12,      // ordered-uint scene min (3)
@@ -258,6 +260,150 @@ __device__ __forceinline__ void child_box(const BinNode* bnodes, const float4* t
         lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
         hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
     }
+}
+
+// ---- agglomerative binary build (PLOC: Meister and Bittner 2018, "Parallel
+// locally-ordered clustering for bounding volume hierarchy construction").
+// Clusters start as the triangles in Morton order; each round every cluster
+// finds the neighbour within WRK_PLOC_RADIUS positions whose merged box has
+// the least surface area, mutual nearest neighbours merge into a new binary
+// node, and the survivors are compacted in order. The result is a binary
+// tree of SAH quality close to a top-down sweep, unlike the Morton-split
+// hierarchy whose splits ignore triangle sizes.
+#ifndef WRK_PLOC
+#define WRK_PLOC 1
+#endif
+#ifndef WRK_PLOC_RADIUS
+#define WRK_PLOC_RADIUS 8
+#endif
+
+__global__ __launch_bounds__(256) void k_ploc_init(const float4* __restrict__ triLo, const float4* __restrict__ triHi,
+                                                   const uint32_t* __restrict__ vals, uint32_t n, uint32_t* cid,
+                                                   float4* clo, float4* chi) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint32_t t = vals[i];
+    cid[i] = kLeafBit | i;  // leaf reference: position in Morton order
+    clo[i] = triLo[t];
+    chi[i] = triHi[t];
+}
+
+__device__ __forceinline__ float merged_area(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+    float dx = fmaxf(ahi.x, bhi.x) - fminf(alo.x, blo.x);
+    float dy = fmaxf(ahi.y, bhi.y) - fminf(alo.y, blo.y);
+    float dz = fmaxf(ahi.z, bhi.z) - fminf(alo.z, blo.z);
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// Nearest neighbour of every cluster within the radius (ties: the smaller
+// index, which orders all pairs strictly, so the closest pair is mutual).
+__global__ __launch_bounds__(256) void k_ploc_nn(const float4* __restrict__ clo, const float4* __restrict__ chi,
+                                                 uint32_t c, uint32_t* nn) {
+    constexpr int R = WRK_PLOC_RADIUS;
+    __shared__ float4 sLo[256 + 2 * R], sHi[256 + 2 * R];
+    const int base = (int)(blockIdx.x * 256) - R;
+    for (int k = threadIdx.x; k < 256 + 2 * R; k += 256) {
+        int g = base + k;
+        if (g >= 0 && g < (int)c) { sLo[k] = clo[g]; sHi[k] = chi[g]; }
+    }
+    __syncthreads();
+    const int i = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (i >= (int)c) return;
+    const int li = (int)threadIdx.x + R;
+    const float4 alo = sLo[li], ahi = sHi[li];
+    float best = INFINITY;
+    int bj = i;
+    const int j0 = max(i - R, 0), j1 = min(i + R, (int)c - 1);
+    for (int j = j0; j <= j1; ++j) {
+        if (j == i) continue;
+        const float a = merged_area(alo, ahi, sLo[j - base], sHi[j - base]);
+        if (a < best) { best = a; bj = j; }  // ascending j: the first minimum is the smallest index
+    }
+    nn[i] = (uint32_t)bj;
+}
+
+// Mutual nearest neighbours i < j merge into a new binary node kept at i;
+// j leaves. keep[i] = 1 for survivors.
+__global__ __launch_bounds__(256) void k_ploc_merge(uint32_t c, const uint32_t* __restrict__ nn, uint32_t* cid,
+                                                    float4* clo, float4* chi, BinNode* bnodes, uint32_t* counters,
+                                                    uint32_t* keep) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= c) return;
+    const uint32_t j = nn[i];
+    const bool mutual = j != i && nn[j] == i;
+    if (mutual && i > j) { keep[i] = 0; return; }
+    keep[i] = 1;
+    if (!mutual) return;
+    const float4 alo = clo[i], ahi = chi[i], blo = clo[j], bhi = chi[j];
+    const uint32_t k = atomicAdd(&counters[C_BIN_NODES], 1u);
+    const float4 lo = make_float4(fminf(alo.x, blo.x), fminf(alo.y, blo.y), fminf(alo.z, blo.z), 0.0f);
+    const float4 hi = make_float4(fmaxf(ahi.x, bhi.x), fmaxf(ahi.y, bhi.y), fmaxf(ahi.z, bhi.z), 0.0f);
+    BinNode& b = bnodes[k];
+    b.lo = make_float4(lo.x, lo.y, lo.z, __uint_as_float(cid[i]));
+    b.hi = make_float4(hi.x, hi.y, hi.z, __uint_as_float(cid[j]));
+    cid[i] = k;
+    clo[i] = lo;
+    chi[i] = hi;
+}
+
+__global__ __launch_bounds__(256) void k_ploc_compact(uint32_t c, const uint32_t* __restrict__ keep,
+                                                      const uint32_t* __restrict__ pos, const uint32_t* __restrict__ cidIn,
+                                                      const float4* __restrict__ loIn, const float4* __restrict__ hiIn,
+                                                      uint32_t* cidOut, float4* loOut, float4* hiOut,
+                                                      uint32_t* counters) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= c) return;
+    if (i == c - 1) counters[C_TASKS_NEXT] = pos[i] + keep[i];
+    if (!keep[i]) return;
+    const uint32_t d = pos[i];
+    cidOut[d] = cidIn[i];
+    loOut[d] = loIn[i];
+    hiOut[d] = hiIn[i];
+}
+
+// Subtree triangle counts, bottom-up by level (held in `last` until the
+// ranges are assigned).
+__global__ __launch_bounds__(256) void k_ploc_sizes(const uint32_t* __restrict__ level, uint32_t count,
+                                                    BinNode* bnodes) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    BinNode& b = bnodes[level[i]];
+    const uint32_t l = __float_as_uint(b.lo.w), r = __float_as_uint(b.hi.w);
+    b.last = ((l & kLeafBit) ? 1u : bnodes[l].last) + ((r & kLeafBit) ? 1u : bnodes[r].last);
+}
+
+// Top-down by level: each node's subtree gets the contiguous triangle range
+// [first, first + size); leaves are renumbered to their new positions and
+// valsOut lists the triangles in that order, so every subtree of the binary
+// tree is one contiguous range, as the collapse and the leaves require.
+__global__ __launch_bounds__(256) void k_ploc_ranges(const uint32_t* __restrict__ level, uint32_t count,
+                                                     bool rootLevel, BinNode* bnodes,
+                                                     const uint32_t* __restrict__ valsIn, uint32_t* valsOut) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    BinNode& b = bnodes[level[i]];
+    const uint32_t first = rootLevel ? 0u : b.first;
+    const uint32_t size = b.last;
+    uint32_t l = __float_as_uint(b.lo.w), r = __float_as_uint(b.hi.w);
+    uint32_t sizeL;
+    if (l & kLeafBit) {
+        valsOut[first] = valsIn[l & ~kLeafBit];
+        l = kLeafBit | first;
+        sizeL = 1;
+    } else {
+        bnodes[l].first = first;
+        sizeL = bnodes[l].last;
+    }
+    if (r & kLeafBit) {
+        valsOut[first + sizeL] = valsIn[r & ~kLeafBit];
+        r = kLeafBit | (first + sizeL);
+    } else {
+        bnodes[r].first = first + sizeL;
+    }
+    b.lo.w = __uint_as_float(l);
+    b.hi.w = __uint_as_float(r);
+    b.first = first;
+    b.last = first + size - 1u;
 }
 
 // Top-down: the internal children of one level's nodes form the next level
@@ -889,6 +1035,7 @@ namespace {
 struct BuildLayout {
     size_t triLo, triHi, keysA, keysB, valsA, valsB, bnodes, levels, partials, tasksA, tasksB, counters, costs,
         sortTemp;
+    size_t cidA, cidB, cloA, cloB, chiA, chiB, nn, keep, pos, scanTemp, scanTempBytes;
     size_t sortTempBytes, total;
 };
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -915,6 +1062,22 @@ BuildLayout layout_for(uint32_t n) {
                                     (uint32_t*)nullptr, (size_t)nn, 0u, kMortonBits);
     L.sortTempBytes = tmp;
     L.sortTemp = take(tmp);
+    if (WRK_PLOC) {
+        L.cidA = take(nn * 4);
+        L.cidB = take(nn * 4);
+        L.cloA = take(nn * 16);
+        L.cloB = take(nn * 16);
+        L.chiA = take(nn * 16);
+        L.chiB = take(nn * 16);
+        L.nn = take(nn * 4);
+        L.keep = take(nn * 4);
+        L.pos = take(nn * 4);
+        size_t st = 0;
+        (void)rocprim::exclusive_scan(nullptr, st, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u, (size_t)nn,
+                                      rocprim::plus<uint32_t>());
+        L.scanTempBytes = st;
+        L.scanTemp = take(st);
+    }
     L.total = off;
     return L;
 }
@@ -963,12 +1126,58 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
     hipError_t sortErr = rocprim::radix_sort_pairs(base + L.sortTemp, tmp, keysA, keysB, valsA, valsB, (size_t)n, 0u,
                                                    kMortonBits, s);
     WRK_HIP_TRY(sortErr);
+    uint32_t root = 0;
+    const uint32_t* valsF = valsB;  // triangles in leaf order
     if (n > 1) {
-        hipLaunchKernelGGL(k_karras, dim3(ceil_div(n - 1, 256)), dim3(256), 0, s, keysB, (int)n, bnodes);
-        WRK_CHECK_LAUNCH();
+        if (WRK_PLOC) {
+            uint32_t* cid[2] = {(uint32_t*)(base + L.cidA), (uint32_t*)(base + L.cidB)};
+            float4* clo[2] = {(float4*)(base + L.cloA), (float4*)(base + L.cloB)};
+            float4* chi[2] = {(float4*)(base + L.chiA), (float4*)(base + L.chiB)};
+            uint32_t* nnb = (uint32_t*)(base + L.nn);
+            uint32_t* keep = (uint32_t*)(base + L.keep);
+            uint32_t* pos = (uint32_t*)(base + L.pos);
+            hipLaunchKernelGGL(k_ploc_init, dim3(g), dim3(256), 0, s, triLo, triHi, valsB, n, cid[0], clo[0], chi[0]);
+            WRK_CHECK_LAUNCH();
+            uint32_t c = n;
+            int cur = 0;
+            for (uint32_t round = 0; c > 1; ++round) {
+                if (round > 4u * n) {
+                    set_error("wrk_bvh_build: clustering made no progress");
+                    return WRK_ERR_INVALID;
+                }
+                const uint32_t gc = ceil_div(c, 256);
+                hipLaunchKernelGGL(k_ploc_nn, dim3(gc), dim3(256), 0, s, clo[cur], chi[cur], c, nnb);
+                hipLaunchKernelGGL(k_ploc_merge, dim3(gc), dim3(256), 0, s, c, nnb, cid[cur], clo[cur], chi[cur],
+                                   bnodes, counters, keep);
+                WRK_CHECK_LAUNCH();
+                size_t st = L.scanTempBytes;
+                WRK_HIP_TRY(rocprim::exclusive_scan(base + L.scanTemp, st, keep, pos, 0u, (size_t)c,
+                                                    rocprim::plus<uint32_t>(), s));
+                hipLaunchKernelGGL(k_ploc_compact, dim3(gc), dim3(256), 0, s, c, keep, pos, cid[cur], clo[cur],
+                                   chi[cur], cid[cur ^ 1], clo[cur ^ 1], chi[cur ^ 1], counters);
+                WRK_CHECK_LAUNCH();
+                uint32_t next = 0;
+                WRK_HIP_TRY(hipMemcpyAsync(&next, counters + C_TASKS_NEXT, 4, hipMemcpyDeviceToHost, s));
+                WRK_HIP_TRY(hipStreamSynchronize(s));
+                if (next >= c || next == 0) {
+                    set_error("wrk_bvh_build: clustering made no progress");
+                    return WRK_ERR_INVALID;
+                }
+                c = next;
+                cur ^= 1;
+            }
+            WRK_HIP_TRY(hipMemcpyAsync(&root, cid[cur], 4, hipMemcpyDeviceToHost, s));
+            WRK_HIP_TRY(hipStreamSynchronize(s));
+            if (root & kLeafBit) {
+                set_error("wrk_bvh_build: clustering ended on a leaf");
+                return WRK_ERR_INVALID;
+            }
+        } else {
+            hipLaunchKernelGGL(k_karras, dim3(ceil_div(n - 1, 256)), dim3(256), 0, s, keysB, (int)n, bnodes);
+            WRK_CHECK_LAUNCH();
+        }
         // binary levels top-down (each internal node lands in exactly one level) ...
-        uint32_t zero = 0;
-        WRK_HIP_TRY(hipMemcpyAsync(levels, &zero, 4, hipMemcpyHostToDevice, s));
+        WRK_HIP_TRY(hipMemcpyAsync(levels, &root, 4, hipMemcpyHostToDevice, s));
         std::vector<uint32_t> off = {0u, 1u};
         uint32_t next = 0;
         for (;;) {
@@ -987,16 +1196,32 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
             }
             off.push_back(off.back() + next);
         }
+        if (WRK_PLOC) {
+            // contiguous triangle ranges per subtree: sizes bottom-up, then
+            // ranges top-down; valsA receives the triangles in leaf order
+            for (size_t L0 = off.size() - 1; L0-- > 0;) {
+                uint32_t cnt = off[L0 + 1] - off[L0];
+                hipLaunchKernelGGL(k_ploc_sizes, dim3(ceil_div(cnt, 256)), dim3(256), 0, s, levels + off[L0], cnt,
+                                   bnodes);
+            }
+            for (size_t L0 = 0; L0 + 1 < off.size(); ++L0) {
+                uint32_t cnt = off[L0 + 1] - off[L0];
+                hipLaunchKernelGGL(k_ploc_ranges, dim3(ceil_div(cnt, 256)), dim3(256), 0, s, levels + off[L0], cnt,
+                                   L0 == 0, bnodes, valsB, valsA);
+            }
+            WRK_CHECK_LAUNCH();
+            valsF = valsA;
+        }
         // ... then boxes bottom-up, one launch per level
         for (size_t L0 = off.size() - 1; L0-- > 0;) {
             uint32_t cnt = off[L0 + 1] - off[L0];
             hipLaunchKernelGGL(k_refit_level<kBvhWidth>, dim3(ceil_div(cnt, 256)), dim3(256), 0, s, levels + off[L0],
-                               cnt, bnodes, triLo, triHi, valsB, costs);
+                               cnt, bnodes, triLo, triHi, valsF, costs);
         }
         WRK_CHECK_LAUNCH();
     }
     // level-by-level collapse; root 4-wide node is slot 0
-    uint4 rootTask = make_uint4(n > 1 ? 0u : kLeafBit, 0u, 1u, 0u);
+    uint4 rootTask = make_uint4(n > 1 ? root : kLeafBit, 0u, 1u, 0u);
     WRK_HIP_TRY(hipMemcpyAsync(tasksA, &rootTask, sizeof(uint4), hipMemcpyHostToDevice, s));
     uint32_t one = 1;
     WRK_HIP_TRY(hipMemcpyAsync(counters + C_NODES, &one, 4, hipMemcpyHostToDevice, s));
@@ -1006,13 +1231,13 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
     while (numTasks > 0) {
         WRK_HIP_TRY(hipMemsetAsync(counters + C_TASKS_NEXT, 0, 4, s));
         hipLaunchKernelGGL(k_collapse<kBvhWidth>, dim3(ceil_div(numTasks, 256)), dim3(256), 0, s, tin, numTasks, bnodes,
-                           triLo, triHi, valsB, (const float*)costs, nodes, counters, tout);
+                           triLo, triHi, valsF, (const float*)costs, nodes, counters, tout);
         WRK_CHECK_LAUNCH();
         WRK_HIP_TRY(hipMemcpyAsync(&numTasks, counters + C_TASKS_NEXT, 4, hipMemcpyDeviceToHost, s));
         WRK_HIP_TRY(hipStreamSynchronize(s));
         std::swap(tin, tout);
     }
-    hipLaunchKernelGGL(k_reorder_tris, dim3(g), dim3(256), 0, s, valsB, vertices, indices, n, (float4*)tris);
+    hipLaunchKernelGGL(k_reorder_tris, dim3(g), dim3(256), 0, s, valsF, vertices, indices, n, (float4*)tris);
     WRK_CHECK_LAUNCH();
     uint32_t hc[C_COUNT];
     WRK_HIP_TRY(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
