@@ -78,10 +78,24 @@ typedef struct wrk_scene_view {
     uint32_t numTextures;
     uint32_t numEmissive;
     float envRadiance[4];
+    /* Optional guide of the CDF search (0 = plain binary search): for a
+     * power of two G = emissiveGuideSize, guide[k] (k = 0..G) is the first
+     * index i with emissiveCdf[i] >= k / G. A sample u lies in bucket
+     * k = floor(u * G) (exact: G is a power of two), so the search runs over
+     * [guide[k], guide[k + 1]] and returns the same index as a search over
+     * the whole CDF. */
+    uint64_t emissiveGuide;  /* uint32[emissiveGuideSize + 1] or 0 */
+    uint32_t emissiveGuideSize;
+    uint32_t _pad2[3];
 } wrk_scene_view;
 
-/* 4-wide BVH over leaf-ordered triangles. Node = 128 bytes: child boxes as
- * lo.x[4] hi.x[4] lo.y[4] hi.y[4] lo.z[4] hi.z[4], then child[4], then pad[4].
+/* BVH over leaf-ordered triangles; nodeBytes gives the node format.
+ * 96 (default): compressed 8-wide node = float origin[3] and the three
+ * biased float exponents e_x, e_y, e_z in bytes 12..14 (16 B), child[8]
+ * (32 B), then 8-bit planes qlo.x[8] qlo.y[8] qlo.z[8] qhi.x[8] qhi.y[8]
+ * qhi.z[8] (48 B); plane = fma(q, 2^(e - 127), origin), rounded outward at
+ * build time. 128: 4-wide node = child boxes as lo.x[4] hi.x[4] lo.y[4]
+ * hi.y[4] lo.z[4] hi.z[4], then child[4], then pad[4].
  * child: bit31 clear -> inner node index; bit31 set -> leaf with
  * count-1 in bits 27..30 and first leaf-ordered triangle in bits 0..26;
  * 0xFFFFFFFF -> empty slot (its box is inverted). Triangles: 3 float4 each,

@@ -38,7 +38,8 @@ class wrk_scene_view(C.Structure):
                 ("triMaterialIds", C.c_uint64), ("materials", C.c_uint64), ("textures", C.c_uint64),
                 ("emissiveTris", C.c_uint64), ("emissiveCdf", C.c_uint64), ("numTris", C.c_uint32),
                 ("numMaterials", C.c_uint32), ("numTextures", C.c_uint32), ("numEmissive", C.c_uint32),
-                ("envRadiance", C.c_float * 4)]
+                ("envRadiance", C.c_float * 4), ("emissiveGuide", C.c_uint64), ("emissiveGuideSize", C.c_uint32),
+                ("_pad2", C.c_uint32 * 3)]
 
 
 class wrk_material(C.Structure):

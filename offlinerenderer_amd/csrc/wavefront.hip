@@ -53,7 +53,8 @@ struct SceneDev {
     const wrk_texture* textures;
     const uint32_t* emissiveTris;
     const float* emissiveCdf;
-    uint32_t numTris, numMaterials, numTextures, numEmissive;
+    const uint32_t* emissiveGuide;
+    uint32_t numTris, numMaterials, numTextures, numEmissive, guideSize;
     float env[3];
 };
 static SceneDev scene_of(const wrk_scene_view* v) {
@@ -64,6 +65,8 @@ static SceneDev scene_of(const wrk_scene_view* v) {
     s.emissiveTris = (const uint32_t*)v->emissiveTris; s.emissiveCdf = (const float*)v->emissiveCdf;
     s.numTris = v->numTris; s.numMaterials = v->numMaterials; s.numTextures = v->numTextures;
     s.numEmissive = v->numEmissive;
+    s.emissiveGuide = (const uint32_t*)v->emissiveGuide;
+    s.guideSize = v->emissiveGuideSize;
     s.env[0] = v->envRadiance[0]; s.env[1] = v->envRadiance[1]; s.env[2] = v->envRadiance[2];
     return s;
 }
@@ -247,6 +250,14 @@ __device__ __forceinline__ void setup_next(PathState& r, const Hit& h, f3 nd, co
 // sampling.cuh:116-135
 __device__ __forceinline__ uint32_t sample_emissive(const SceneDev& S, float u) {
     uint32_t left = 0, right = S.numEmissive;
+    if (S.emissiveGuide) {
+        // the answer lies between the guides of bucket floor(u * G); u * G
+        // is exact for a power-of-two G, so the result equals the search
+        // over the whole CDF (u = 1 falls in the last bucket)
+        const uint32_t k = min((uint32_t)(u * (float)S.guideSize), S.guideSize - 1u);
+        left = S.emissiveGuide[k];
+        right = S.emissiveGuide[k + 1u];
+    }
     while (left < right) {
         uint32_t mid = (left + right) / 2;
         if (S.emissiveCdf[mid] < u) left = mid + 1; else right = mid;

@@ -88,7 +88,8 @@ struct SceneImpl {
     wrhost::ImageF envMap;
     DevBuf d_envMap;
 
-    DevBuf d_vertices, d_normals, d_uvs, d_indices, d_triMat, d_materials, d_textures, d_emissive, d_cdf;
+    DevBuf d_vertices, d_normals, d_uvs, d_indices, d_triMat, d_materials, d_textures, d_emissive, d_cdf, d_guide;
+    uint32_t guideSize = 0;
     DevBuf d_nodes, d_tris, d_bvhRecord;
     wrk_bvh bvh{};
     wrk_scene_view view{};
@@ -282,6 +283,20 @@ void Scene::finalize(const SceneBuildConfig& config) {
         for (float& c : s.emissiveTriangleCDF) c /= totalPower;
         s.d_emissive.upload(s.emissiveTriangles.data(), s.emissiveTriangles.size() * sizeof(uint32_t));
         s.d_cdf.upload(s.emissiveTriangleCDF.data(), s.emissiveTriangleCDF.size() * sizeof(float));
+        // search guide: guide[k] = first i with cdf[i] >= k / G (G a power of
+        // two >= the light count, 256..2^20), same float comparison as the
+        // device search, so each sample searches about one entry
+        uint32_t G = 256;
+        while (G < s.emissiveTriangleCDF.size() && G < (1u << 20)) G <<= 1;
+        std::vector<uint32_t> guide(G + 1);
+        size_t i = 0;
+        for (uint32_t k = 0; k <= G; ++k) {
+            const float x = (float)k / (float)G;
+            while (i < s.emissiveTriangleCDF.size() && s.emissiveTriangleCDF[i] < x) ++i;
+            guide[k] = (uint32_t)i;
+        }
+        s.d_guide.upload(guide.data(), guide.size() * sizeof(uint32_t));
+        s.guideSize = G;
         if (g_logLevel >= 4)
             std::cout << "[Scene] Built light list: " << s.emissiveTriangles.size() << " emissive triangles" << std::endl;
     }
@@ -311,6 +326,8 @@ void Scene::finalize(const SceneBuildConfig& config) {
     v.materials = s.d_materials.addr();
     v.emissiveTris = s.d_emissive.addr();
     v.emissiveCdf = s.d_cdf.addr();
+    v.emissiveGuide = s.guideSize ? s.d_guide.addr() : 0;
+    v.emissiveGuideSize = s.guideSize;
     v.numTris = s.numTriangles;
     v.numMaterials = (uint32_t)s.materials.size();
     v.numEmissive = (uint32_t)s.emissiveTriangles.size();
