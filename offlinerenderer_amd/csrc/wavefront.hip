@@ -533,9 +533,14 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_para
     r.neeDone = (st.z & FLAG_NEE_DONE) != 0;
     r.pixel = st.w;
     const bool defer = prm.iterationCap != 0;
+    // Lambertian, glass and GGX-transmission shaders never add to L (their
+    // light contribution goes through the shadow ray), so their L is read
+    // only for a flush and is not written back
+    const bool lazyL = seg == WRK_MAT_LAMBERTIAN || seg == WRK_MAT_GLASS || seg == WRK_MAT_GGX_TRANSMISSION;
     float4 o4 = P.org[p];
     float4 d4 = P.dir[p];
-    float4 T4 = P.thr[p], L4 = P.rad[p];
+    float4 T4 = P.thr[p];
+    const float4 L4 = lazyL ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : P.rad[p];
     r.o = xyz(o4); r.tMin = o4.w; r.d = xyz(d4); r.tMax = d4.w;
     r.T = xyz(T4); r.prevPdf = T4.w; r.L = xyz(L4);
     r.flushed = false; r.done = false; r.toShadow = false;
@@ -577,6 +582,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_para
         else shade_glass(S, prm, r, eh, mat);
     }
 
+    if (lazyL && r.flushed) r.flushL = clamp_fireflies(xyz(P.rad[p]), prm.fireflyClamp);
     uint32_t next;
     if (r.done && r.toShadow) {
         next = NEXT_SHADOW;  // flushed once the shadow result is in
@@ -599,7 +605,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_para
         P.org[p] = f4(r.o, r.tMin);
         P.dir[p] = f4(r.d, r.tMax);
         P.thr[p] = f4(r.T, r.prevPdf);
-        P.rad[p] = f4(r.L, 0.0f);
+        if (!lazyL) P.rad[p] = f4(r.L, 0.0f);
         uint32_t flags = (r.depth & FLAG_DEPTH_MASK) | (r.prevDelta ? FLAG_PREV_DELTA : 0u) |
                          (r.neeDone ? FLAG_NEE_DONE : 0u) | (min(iter, 31u) << FLAG_ITER_SHIFT) |
                          (slot << FLAG_SLOT_SHIFT);
