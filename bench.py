@@ -67,6 +67,8 @@ def main():
                     "8M for serving)")
     ap.add_argument("--requests", type=int, default=256, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrival rate, requests/s")
+    ap.add_argument("--round-robin", action="store_true",
+                    help="serving: one chunk per pending request per round (default: in-order admission)")
     ap.add_argument("--policy", choices=["fifo", "shortest"], default="shortest",
                     help="serving: admission order of pending requests")
     ap.add_argument("--dn-scenes", type=int, default=8, help="denoise: training scenes")
@@ -295,7 +297,8 @@ def serving(args):
 
     def run_server(arrive):
         srv = wr.RenderServer(wr.ServerConfig(maxPathsInFlight=pool, maxChunkPaths=chunk,
-                                              shortestFirst=args.policy == "shortest"))
+                                              shortestFirst=args.policy == "shortest",
+                                              roundRobin=args.round_robin))
         ids, done_t, equal = [None] * len(jobs), np.zeros(len(jobs)), 0
         open_ids = {}
         t0 = srv.clock()
@@ -351,7 +354,8 @@ def serving(args):
             "workload": f"serving_{len(jobs)}req_poisson{args.rate:g}",
             "requests": len(jobs), "total_paths": int(total_paths), "resolutions": list(scenes.SERVING_RESOLUTIONS),
             "spp": list(scenes.SERVING_SPP), "max_depth": 8, "arrival_rate_per_s": args.rate,
-            "paths_in_flight": pool, "chunk_paths": chunk, "admission": args.policy,
+            "paths_in_flight": pool, "chunk_paths": chunk,
+            "admission": args.policy + (" round-robin" if args.round_robin else " in order"),
         },
         "serial_mpaths_per_s": round(serial_mpaths, 3),
         "speedup_vs_serial": round(server_mpaths / serial_mpaths, 3),

@@ -122,6 +122,11 @@ struct ServerConfig {
     // admission order of pending requests: false = arrival order, true =
     // fewest remaining paths first (short requests pass long ones)
     bool shortestFirst = false;
+    // false: the first pending request takes every chunk that fits before
+    // the next one gets any (requests finish in order, as served one after
+    // another, while the pool stays full); true: one chunk per pending
+    // request per round (all requests progress together)
+    bool roundRobin = false;
 };
 
 struct RequestStatus {

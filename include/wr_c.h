@@ -83,7 +83,8 @@ int wr_c_render_guides(void* renderer, void* scene, const wr_c_camera* cam, cons
 typedef struct wr_c_server_config {
     uint32_t maxPathsInFlight, maxChunkPaths;
     uint64_t memoryBudgetBytes;
-    uint32_t iterationCap, flags; /* flags bit0: time kernels, bit1: shortest request first */
+    uint32_t iterationCap, flags; /* flags bit0: time kernels, bit1: shortest request first,
+                                     bit2: round-robin admission (one chunk per request per round) */
 } wr_c_server_config;
 
 typedef struct wr_c_request_status {

@@ -476,6 +476,7 @@ class ServerConfig:
     iterationCap: int = 0
     timeKernels: bool = False
     shortestFirst: bool = False
+    roundRobin: bool = False
 
 
 class RenderServer:
@@ -486,7 +487,8 @@ class RenderServer:
     def __init__(self, cfg: ServerConfig | None = None):
         c = cfg or ServerConfig()
         cc = wr_c_server_config(int(c.maxPathsInFlight), int(c.maxChunkPaths), int(c.memoryBudgetBytes),
-                                int(c.iterationCap), (1 if c.timeKernels else 0) | (2 if c.shortestFirst else 0))
+                                int(c.iterationCap), (1 if c.timeKernels else 0) | (2 if c.shortestFirst else 0)
+                                | (4 if c.roundRobin else 0))
         self._h = lib().wr_c_server_create(C.byref(cc))
         if not self._h:
             raise RuntimeError(f"RenderServer: {_err()}")

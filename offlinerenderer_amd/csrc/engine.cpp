@@ -486,10 +486,13 @@ bool Engine::step() {
     }
     if (total) stats_.iterations++;
 
-    // --- admission: new chunks join behind the shaded paths. Pending
-    // requests take one chunk each per round, in arrival order, until slots
-    // or pool ranges run out; a request that does not fit the memory budget
-    // stops admission of the requests behind it.
+    // --- admission: new chunks join behind the shaded paths, in the order of
+    // the pending list (arrival, or fewest remaining paths first). By default
+    // the first request takes every chunk that fits before the next one gets
+    // any, so requests complete in order as they would served one after
+    // another while the pool stays full; with roundRobin each pending
+    // request takes one chunk per round. A request that does not fit (slots,
+    // pool ranges or the memory budget) stops admission of those behind it.
     uint32_t added = 0;
     if (cfg_.shortestFirst && pending_.size() > 1) {
         std::stable_sort(pending_.begin(), pending_.end(), [&](uint64_t x, uint64_t y) {
@@ -498,7 +501,14 @@ bool Engine::step() {
             return a.endPath - a.nextPath < b.endPath - b.nextPath;
         });
     }
-    for (bool progress = true; progress && !pending_.empty();) {
+    if (!cfg_.roundRobin) {
+        while (!pending_.empty()) {
+            RequestState& r = *requests_.at(pending_.front());
+            if (!admitChunk(r, &added, total)) break;
+            if (r.nextPath == r.endPath) pending_.erase(pending_.begin());
+        }
+    }
+    for (bool progress = cfg_.roundRobin; progress && !pending_.empty();) {
         progress = false;
         for (auto it = pending_.begin(); it != pending_.end();) {
             RequestState& r = *requests_.at(*it);

@@ -58,6 +58,7 @@ struct EngineConfig {
     bool timeKernels = false;
     bool countTraversal = false;
     bool shortestFirst = false;          // admission order: fewest remaining paths first
+    bool roundRobin = false;             // one chunk per pending request per round (else the first takes all that fit)
 };
 
 class Engine {

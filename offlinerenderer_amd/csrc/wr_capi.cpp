@@ -182,6 +182,7 @@ void* wr_c_server_create(const wr_c_server_config* cfg) {
         c.iterationCap = cfg->iterationCap;
         c.timeKernels = (cfg->flags & 1u) != 0;
         c.shortestFirst = (cfg->flags & 2u) != 0;
+        c.roundRobin = (cfg->flags & 4u) != 0;
         out = new wr::RenderServer(c);
     });
     return out;

@@ -548,6 +548,7 @@ struct ServerImpl {
         e.memoryBudgetBytes = c.memoryBudgetBytes;
         e.timeKernels = c.timeKernels;
         e.shortestFirst = c.shortestFirst;
+        e.roundRobin = c.roundRobin;
         eng = std::make_unique<detail::Engine>(e);
         eng->ensureCapacity(e.capacity);
     }

@@ -69,11 +69,12 @@ def _same(a, b):
     return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-@pytest.mark.parametrize("shortest_first", [False, True])
-def test_mixed_requests_match_alone(world, shortest_first):
+@pytest.mark.parametrize("shortest_first,round_robin", [(False, False), (True, False), (False, True), (True, True)])
+def test_mixed_requests_match_alone(world, shortest_first, round_robin):
     reqs = _requests(world)
     ref = _alone(reqs, cap=0)
-    got, srv = _serve(reqs, wr.ServerConfig(maxPathsInFlight=1 << 20, shortestFirst=shortest_first), stagger=True)
+    got, srv = _serve(reqs, wr.ServerConfig(maxPathsInFlight=1 << 20, shortestFirst=shortest_first,
+                                            roundRobin=round_robin), stagger=True)
     for i, (a, b) in enumerate(zip(got, ref)):
         assert _same(a, b), f"request {i} differs from its stand-alone render"
     assert srv.numPending() == 0 and srv.numInFlight() == 0
@@ -85,9 +86,33 @@ def test_chunked_small_pool_matches_alone(world):
     reqs = _requests(world)
     ref = _alone(reqs, cap=0)
     # pool smaller than the largest request; chunks smaller than the pool
-    got, _ = _serve(reqs, wr.ServerConfig(maxPathsInFlight=12000, maxChunkPaths=5000), stagger=False)
-    for i, (a, b) in enumerate(zip(got, ref)):
-        assert _same(a, b), f"request {i} differs when split into chunks"
+    for rr in (False, True):
+        got, _ = _serve(reqs, wr.ServerConfig(maxPathsInFlight=12000, maxChunkPaths=5000, roundRobin=rr),
+                        stagger=False)
+        for i, (a, b) in enumerate(zip(got, ref)):
+            assert _same(a, b), f"request {i} differs when split into chunks (round robin {rr})"
+
+
+def test_in_order_admission(world):
+    """Without round-robin, a request is first admitted only after every
+    request submitted before it has had all its chunks admitted."""
+    reqs = _requests(world)
+    ref = _alone(reqs, cap=0)
+    srv = wr.RenderServer(wr.ServerConfig(maxPathsInFlight=12000, maxChunkPaths=3000))
+    ids = [srv.submit(s, c, p) for s, c, p in reqs]
+    admitted_before = []  # per step: which requests have been admitted so far
+    for _ in range(10000):
+        more = srv.step()
+        admitted_before.append([srv.status(i)["admitted"] for i in ids])
+        if not more:
+            break
+    for row in admitted_before:
+        # admitted requests form a prefix of the submission order
+        k = row.index(False) if False in row else len(row)
+        assert all(not a for a in row[k:])
+    for i, rid in enumerate(ids):
+        assert srv.status(rid)["done"]
+        assert _same(srv.fetch(rid), ref[i])
 
 
 def test_reference_mode_cap_with_staggered_admission(world):
