@@ -186,6 +186,20 @@ int wrk_trace_closest(const wrk_bvh* bvh, const void* org4, const void* dir4, co
 int wrk_trace_any(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue,
                   uint32_t n, const uint32_t* countDev, uint8_t* occluded, void* stream);
 
+/* Shade categories in the triangle records: for every leaf-ordered triangle,
+ * 1 + its category (wrk_classify's material-kind mapping of its material id)
+ * is written into the w word of its second vertex (0 = untagged, as
+ * wrk_bvh_build leaves it). Run again after the materials change. No
+ * counterpart in libWR: it lets the closest-hit launch hand over the shade
+ * category so the classify stage skips the material lookups of hits. */
+int wrk_bvh_tag_categories(const wrk_bvh* bvh, const wrk_scene_view* scene, void* stream);
+/* Closest hit as wrk_trace_closest, plus categories[i] for queue position i:
+ * the tagged category of the hit triangle, or 0xFF for a miss or an untagged
+ * triangle. fetches may be NULL (see the counted variants below). */
+int wrk_trace_closest_categorized(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue,
+                                  uint32_t n, const uint32_t* countDev, uint32_t* hits, uint8_t* categories,
+                                  unsigned long long* fetches, void* stream);
+
 /* The same traversals built with fetch counters: fetches[0] += inner nodes
  * visited, fetches[1] += triangles tested (device uint64 pair). They give the
  * mean node/triangle fetches per ray that the algorithmic-bytes model of the
@@ -234,10 +248,13 @@ int wrk_generate(const wrk_camera* cam, const wrk_render_params* params, const w
  * shadow ray held in savedOrg/savedDir for the last two), and assigns each
  * path a shade category 0..4 (material kinds of gpu_types.h:11-17) or
  * 5 = finished (trace.cu:27-51 shadow resolve, trace.cu:143-148 miss flush).
- * paramsTable is a device array indexed by the path's slot. */
+ * paramsTable is a device array indexed by the path's slot.
+ * flags bit0: category[i] of the first two parts already holds the category
+ * of a tagged hit (wrk_trace_closest_categorized), 0xFF otherwise; those
+ * entries are taken as they are. */
 int wrk_classify(const wrk_scene_view* scene, const wrk_render_params* paramsTable, const wrk_paths* paths,
                  const uint32_t* queue, uint32_t numTrace, uint32_t numTraceShadow, uint32_t numShadow,
-                 uint8_t* category, void* stream);
+                 uint8_t* category, uint32_t flags, void* stream);
 /* Runs the material shaders over the material-sorted queue of one scene
  * (offsets as from wrk_partition with 6 categories, the sixth holding the
  * paths classify finished): shade.cu:24-85 and the materials/ headers.

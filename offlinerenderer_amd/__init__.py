@@ -177,7 +177,9 @@ _SIGS = {
     "wrk_partition": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint64, _P]),
     "wrk_generate": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32,
                                _P]),
-    "wrk_classify": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]),
+    "wrk_classify": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_uint32, _P]),
+    "wrk_bvh_tag_categories": (C.c_int, [C.POINTER(wrk_bvh), _P, _P]),
+    "wrk_trace_closest_categorized": (C.c_int, [C.POINTER(wrk_bvh), _P, _P, _P, C.c_uint32, _P, _P, _P, _P, _P]),
     "wrk_shade": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_slot_finish": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
     "wrk_accumulate": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P, _P, _P]),

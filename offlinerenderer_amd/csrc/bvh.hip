@@ -772,6 +772,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                                                uint32_t raysPerLane) {
     __shared__ uint32_t sNode[kStackLds][256];
     __shared__ float sT[kStackLds][256];
+    __shared__ uint32_t sQi[ANY ? 1 : 256];  // closest hit: queue position of each lane's ray
     const uint32_t lane = threadIdx.x;
     uint32_t cnt = n;
     if (countDev) cnt = min(cnt, *countDev);
@@ -804,6 +805,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
             if (!active && qi < chunkEnd) {
                 ray = queue ? queue[qi] : (uint32_t)qi;
+                if (!ANY && occl) sQi[lane] = qi;
                 o = org[ray];
                 float4 d = dir[ray];
                 tmin = o.w;
@@ -857,7 +859,11 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                 if (intersect_tri(rp, v0, v1, v2, tmin, tbest, &th)) {
                     uint32_t prim = __float_as_uint(v0.w);
                     if (ANY) { found = true; break; }
-                    if (th < tbest || prim < bestPrim) { tbest = th; bestPrim = prim; }
+                    // bestPrim keeps the triangle's shade-category tag in bits 28..30
+                    if (th < tbest || prim < (bestPrim & kPrimMask)) {
+                        tbest = th;
+                        bestPrim = prim | ((__float_as_uint(v1.w) & 7u) << 28);
+                    }
                 }
             }
             leaf = kEmptyChild;
@@ -922,7 +928,10 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                         if (intersect_tri(rp, v0, v1, v2, tmin, tbest, &th)) {
                             uint32_t prim = __float_as_uint(v0.w);
                             if (ANY) { found = true; break; }
-                            if (th < tbest || prim < bestPrim) { tbest = th; bestPrim = prim; }
+                            if (th < tbest || prim < (bestPrim & kPrimMask)) {
+                                tbest = th;
+                                bestPrim = prim | ((__float_as_uint(v1.w) & 7u) << 28);
+                            }
                         }
                     }
                 }
@@ -1023,7 +1032,17 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
             if (kLeafOnStack && leaf != kEmptyChild) done = false;
             if (done) {
                 if (ANY) occl[ray] = found ? 1 : 0;
-                else hits[ray] = make_uint2(__float_as_uint(tbest), bestPrim);
+                else {
+                    const bool hit = bestPrim != 0xFFFFFFFFu;
+                    hits[ray] = make_uint2(__float_as_uint(tbest), hit ? (bestPrim & kPrimMask) : 0xFFFFFFFFu);
+                    if (occl) {
+                        // closest hit with categories: the shade category of
+                        // the hit triangle at this queue position (0xFF: a
+                        // miss or an untagged triangle)
+                        const uint32_t tag = bestPrim >> 28;
+                        occl[sQi[lane]] = (hit && tag != 0u) ? (uint8_t)(tag - 1u) : (uint8_t)0xFF;
+                    }
+                }
                 active = false;
             }
         }
@@ -1331,6 +1350,35 @@ extern "C" int wrk_trace_closest(const wrk_bvh* bvh, const void* org4, const voi
                                  uint32_t n, const uint32_t* countDev, uint32_t* hits, void* stream) {
     WRK_REQUIRE(hits, "wrk_trace_closest: null hits");
     return launch_trace<false>(bvh, org4, dir4, queue, n, countDev, hits, nullptr, nullptr, stream);
+}
+
+extern "C" int wrk_trace_closest_categorized(const wrk_bvh* bvh, const void* org4, const void* dir4,
+                                             const uint32_t* queue, uint32_t n, const uint32_t* countDev,
+                                             uint32_t* hits, uint8_t* categories, unsigned long long* fetches,
+                                             void* stream) {
+    WRK_REQUIRE(hits && categories, "wrk_trace_closest_categorized: null argument");
+    return launch_trace<false>(bvh, org4, dir4, queue, n, countDev, hits, categories, fetches, stream);
+}
+
+__global__ __launch_bounds__(256) void k_tag_categories(float4* tris, uint32_t n, const uint32_t* __restrict__ triMat,
+                                                        const wrk_material* __restrict__ materials,
+                                                        uint32_t numMaterials) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t prim = __float_as_uint(tris[3 * (size_t)k].w);
+    tris[3 * (size_t)k + 1].w = __uint_as_float(1u + shade_category(triMat[prim], numMaterials, materials));
+}
+
+extern "C" int wrk_bvh_tag_categories(const wrk_bvh* bvh, const wrk_scene_view* scene, void* stream) {
+    WRK_REQUIRE(bvh && bvh->tris && scene && scene->triMaterialIds, "wrk_bvh_tag_categories: null argument");
+    WRK_REQUIRE(scene->numMaterials == 0 || scene->materials, "wrk_bvh_tag_categories: null materials");
+    WRK_REQUIRE(bvh->numTris == scene->numTris, "wrk_bvh_tag_categories: BVH and scene differ in triangles");
+    if (bvh->numTris == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_tag_categories, dim3(ceil_div(bvh->numTris, 256)), dim3(256), 0, as_stream(stream),
+                       (float4*)bvh->tris, bvh->numTris, (const uint32_t*)scene->triMaterialIds,
+                       (const wrk_material*)scene->materials, scene->numMaterials);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
 }
 
 extern "C" int wrk_trace_any(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue,

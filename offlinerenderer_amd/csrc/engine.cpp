@@ -441,14 +441,12 @@ bool Engine::step() {
         const uint32_t nC = nT + nTS, nA = nTS + nS;
         if (nC) {
             timed(T_CLOSEST, ss, [&] {
-                if (cfg_.countTraversal)
-                    wrk_check(wrk_trace_closest_counted(&bvh, org_.ptr, dir_.ptr, qlive + a, nC, nullptr,
-                                                        hit_.as<uint32_t>(), fc, ss),
-                              "wrk_trace_closest_counted");
-                else
-                    wrk_check(wrk_trace_closest(&bvh, org_.ptr, dir_.ptr, qlive + a, nC, nullptr, hit_.as<uint32_t>(),
-                                                ss),
-                              "wrk_trace_closest");
+                // hits plus the shade category of tagged triangles, written
+                // at the queue positions classify reads
+                wrk_check(wrk_trace_closest_categorized(&bvh, org_.ptr, dir_.ptr, qlive + a, nC, nullptr,
+                                                        hit_.as<uint32_t>(), cat + a,
+                                                        cfg_.countTraversal ? fc : nullptr, ss),
+                          "wrk_trace_closest_categorized");
             });
             stats_.launchesTraceClosest++;
         }
@@ -468,7 +466,7 @@ bool Engine::step() {
         stats_.closestRays += nC;
         stats_.shadowRays += nA;
         timed(T_CLASSIFY, ss, [&] {
-            wrk_check(wrk_classify(&view, prmTable, &paths_, qlive + a, nT, nTS, nS, cat + a, ss),
+            wrk_check(wrk_classify(&view, prmTable, &paths_, qlive + a, nT, nTS, nS, cat + a, nC ? 1u : 0u, ss),
                       "wrk_classify");
         });
         timed(T_PARTITION, ss, [&] {

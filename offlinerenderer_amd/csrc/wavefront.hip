@@ -17,7 +17,7 @@ namespace wrk {
 // continuation ray, trace it together with a shadow ray, or trace a shadow
 // ray alone (its continuation either ended or, under an iteration cap,
 // waits for the shadow result as in libWR's stage machine).
-enum : uint32_t { CAT_DONE = 5, NEXT_TRACE = 0, NEXT_TRACE_SHADOW = 1, NEXT_SHADOW = 2, NEXT_DONE = 3 };
+enum : uint32_t { CAT_DONE = kCatDone, NEXT_TRACE = 0, NEXT_TRACE_SHADOW = 1, NEXT_SHADOW = 2, NEXT_DONE = 3 };
 
 struct Pool {
     float4* org;
@@ -140,9 +140,12 @@ __global__ __launch_bounds__(256) void k_generate(wrk_camera cam, wrk_render_par
 // flight, so the radiance is flushed now that the contribution is in.
 __global__ __launch_bounds__(256) void k_classify(SceneDev S, const wrk_render_params* __restrict__ prmTable, Pool P,
                                                   const uint32_t* queue, uint32_t nT, uint32_t nTS, uint32_t nS,
-                                                  uint8_t* cat) {
+                                                  uint8_t* cat, uint32_t presetCats) {
     uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= nT + nTS + nS) return;
+    // the closest-hit launch already wrote the category of tagged hits
+    const uint32_t pre = (presetCats && i < nT + nTS) ? cat[i] : 0xFFu;
+    if (pre != 0xFFu && i < nT) return;
     uint32_t p = queue[i];
     uint32_t c;
     if (i >= nT) {
@@ -152,7 +155,9 @@ __global__ __launch_bounds__(256) void k_classify(SceneDev S, const wrk_render_p
             P.rad[p] = make_float4(L.x + C.x, L.y + C.y, L.z + C.z, L.w);
         }
     }
-    if (i < nT + nTS) {
+    if (pre != 0xFFu) {
+        return;
+    } else if (i < nT + nTS) {
         uint2 h = P.hit[p];
         if (h.y == 0xFFFFFFFFu) {
             float4 T = P.thr[p], L = P.rad[p];
@@ -163,13 +168,7 @@ __global__ __launch_bounds__(256) void k_classify(SceneDev S, const wrk_render_p
             P.result[p] = f4(Lc, 0.0f);
             c = CAT_DONE;
         } else {
-            uint32_t m = S.triMat[h.y];
-            if (m >= S.numMaterials) {
-                c = CAT_DONE;
-            } else {
-                uint32_t type = S.materials[m].type;
-                c = type <= WRK_MAT_GGX_TRANSMISSION ? type : (uint32_t)WRK_MAT_EMISSIVE;
-            }
+            c = shade_category(S.triMat[h.y], S.numMaterials, S.materials);
         }
     } else if (P.st[p].z & FLAG_NEE_DONE) {
         c = WRK_MAT_LAMBERTIAN;
@@ -727,13 +726,13 @@ extern "C" int wrk_generate(const wrk_camera* cam, const wrk_render_params* para
 
 extern "C" int wrk_classify(const wrk_scene_view* scene, const wrk_render_params* paramsTable, const wrk_paths* paths,
                             const uint32_t* queue, uint32_t numTrace, uint32_t numTraceShadow, uint32_t numShadow,
-                            uint8_t* category, void* stream) {
+                            uint8_t* category, uint32_t flags, void* stream) {
     WRK_REQUIRE(scene && paramsTable && paths && queue && category, "wrk_classify: null argument");
     const uint64_t n = (uint64_t)numTrace + numTraceShadow + numShadow;
     WRK_REQUIRE(n <= paths->capacity, "wrk_classify: queue longer than the path pool");
     if (n == 0) return WRK_OK;
     hipLaunchKernelGGL(k_classify, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), scene_of(scene),
-                       paramsTable, pool_of(paths), queue, numTrace, numTraceShadow, numShadow, category);
+                       paramsTable, pool_of(paths), queue, numTrace, numTraceShadow, numShadow, category, flags & 1u);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }

@@ -335,6 +335,9 @@ void Scene::finalize(const SceneBuildConfig& config) {
     v.envRadiance[1] = s.environmentRadiance.y;
     v.envRadiance[2] = s.environmentRadiance.z;
     v.envRadiance[3] = 0.0f;
+    // shade categories into the triangle records (hits then carry them)
+    wrk_check(wrk_bvh_tag_categories(&s.bvh, &v, nullptr), "wrk_bvh_tag_categories");
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     s.uploadTextures();
     if (g_logLevel >= 4)
         std::cout << "[Scene] BVH built: " << s.bvh.numNodes << " nodes, depth " << s.bvh.maxDepth << ", "

@@ -67,6 +67,19 @@ static_assert(sizeof(Node8) == 96, "Node8 must be 96 bytes");
 constexpr uint32_t kBvhWidth = WRK_BVH_WIDTH;
 
 constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kPrimMask = 0x0FFFFFFFu;  // primitive id bits of a hit (ids < 2^27)
+
+// Shade category of a triangle (the queue its hits go to): the material kind
+// 0..4, an unknown kind joins the emissive queue (whose shader flushes it as
+// the default case of shade.cu:79-81 does), and an out-of-range material id
+// ends the path (shade.cu:35-38).
+constexpr uint32_t kCatDone = 5;
+__host__ __device__ inline uint32_t shade_category(uint32_t materialId, uint32_t numMaterials,
+                                                   const wrk_material* materials) {
+    if (materialId >= numMaterials) return kCatDone;
+    const uint32_t type = materials[materialId].type;
+    return type <= WRK_MAT_GGX_TRANSMISSION ? type : (uint32_t)WRK_MAT_EMISSIVE;
+}
 constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
 #ifndef WRK_LEAF_MAX
 #define WRK_LEAF_MAX 2

@@ -98,3 +98,48 @@ def test_many_rays_per_lane_match_one_per_lane(gpu, oracle):
     t_o, p_o = oracle.trace_closest(os_, org4[sl], dir4[sl], use_bvh=True)
     _compare(t_all[sl], p_all[sl], t_o, p_o)
     assert np.array_equal(occ_all[sl], oracle.trace_any(os_, org4[sl], dir4[sl], use_bvh=True))
+
+
+def test_closest_hit_categories(gpu):
+    """wrk_trace_closest_categorized: hits equal wrk_trace_closest's, and the
+    category at each queue position is the material kind of the hit
+    triangle (unknown kinds -> emissive queue, bad ids -> finished), 0xFF on
+    a miss."""
+    import ctypes as C
+
+    import torch
+
+    import offlinerenderer_amd as wr
+    from tests.test_gpu_bvh import d2h
+
+    d = scenes.material_scene(96, 54, 1, tex_size=64)
+    ps = G.product_scene(d)
+    bvh = ps.getBVH()
+    org4, dir4 = make_rays(d, 50_000, 13)
+    t_ref, p_ref = G.trace_closest(bvh, org4, dir4)
+    n = org4.shape[0]
+    rng = np.random.default_rng(1)
+    queue = rng.permutation(n).astype(np.uint32)
+    o, dd = G.dev(org4), G.dev(dir4)
+    q = G.dev(queue)
+    hits = torch.zeros((n, 2), dtype=torch.int32, device="cuda")
+    cats = torch.full((n,), 0x77, dtype=torch.uint8, device="cuda")
+    G.check(wr.lib().wrk_trace_closest_categorized(C.byref(bvh), G.vp(o), G.vp(dd), G.vp(q), n, None,
+                                                   G.vp(hits), G.vp(cats), None, G.stream()))
+    torch.cuda.synchronize()
+    h = hits.cpu().numpy().view(np.uint32)
+    assert np.array_equal(h[:, 1], p_ref) and np.array_equal(h[:, 0].view(np.float32).view(np.uint32),
+                                                             t_ref.view(np.uint32))
+    view = ps.getDeviceView()
+    mats = d2h(view.materials, view.numMaterials * 64).view(np.uint32).reshape(-1, 16)
+    kind = mats[:, 9]  # wrk_material.type
+    tri_mat = d2h(view.triMaterialIds, view.numTris * 4).view(np.uint32)
+    prim = p_ref[queue]
+    hit = prim != 0xFFFFFFFF
+    m = tri_mat[prim[hit]]
+    want = np.where(m >= view.numMaterials, 5, np.where(kind[np.minimum(m, view.numMaterials - 1)] <= 4,
+                                                        kind[np.minimum(m, view.numMaterials - 1)], 1))
+    got = cats.cpu().numpy()
+    assert np.array_equal(got[hit], want)
+    assert np.all(got[~hit] == 0xFF)
+    assert len(np.unique(want)) >= 3
