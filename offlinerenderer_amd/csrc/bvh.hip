@@ -295,8 +295,11 @@ __device__ __forceinline__ float merged_area(float4 alo, float4 ahi, float4 blo,
     return dx * dy + dy * dz + dz * dx;
 }
 
-// Nearest neighbour of every cluster within the radius (ties: the smaller
-// index, which orders all pairs strictly, so the closest pair is mutual).
+// Nearest neighbour of every cluster within the radius. Pairs are ordered
+// strictly by (merged area, not an aligned pair {2k, 2k+1}, lower index,
+// higher index), so the least pair is always mutual (at least one merge per
+// round), and equal areas (a regular grid, duplicated triangles) resolve to
+// aligned pairs that are all mutual, so such inputs still halve each round.
 __global__ __launch_bounds__(256) void k_ploc_nn(const float4* __restrict__ clo, const float4* __restrict__ chi,
                                                  uint32_t c, uint32_t* nn) {
     constexpr int R = WRK_PLOC_RADIUS;
@@ -317,7 +320,9 @@ __global__ __launch_bounds__(256) void k_ploc_nn(const float4* __restrict__ clo,
     for (int j = j0; j <= j1; ++j) {
         if (j == i) continue;
         const float a = merged_area(alo, ahi, sLo[j - base], sHi[j - base]);
-        if (a < best) { best = a; bj = j; }  // ascending j: the first minimum is the smallest index
+        // ascending j: among equal areas the first candidate is the smallest
+        // (min, max) pair, so only the aligned partner i ^ 1 can displace it
+        if (a < best || (a == best && j == (i ^ 1))) { best = a; bj = j; }
     }
     nn[i] = (uint32_t)bj;
 }

@@ -132,3 +132,30 @@ def test_bvh_tiny_scenes(gpu):
         v = rng.uniform(-1, 1, (ntri * 3, 3)).astype(np.float32)
         d.ops.append(("mesh", v.ravel(), np.arange(ntri * 3, dtype=np.uint32), None, None, 0))
         check_bvh(d)
+
+
+def _mesh_desc(v, idx):
+    d = scenes.SceneDesc(width=4, height=4)
+    d.ops.append(("lambert", (0.5, 0.5, 0.5), 0))
+    d.ops.append(("mesh", np.ascontiguousarray(v, dtype=np.float32).ravel(),
+                  np.ascontiguousarray(idx, dtype=np.uint32).ravel(), None, None, 0))
+    return d
+
+
+def test_bvh_degenerate_inputs_build_fast(gpu):
+    """Equal merge costs (duplicated triangles, a regular grid) must not
+    slow the agglomerative build to one merge per round: ties resolve to
+    aligned pairs, which all merge together."""
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], dtype=np.float32)
+    n = 20_000
+    same = _mesh_desc(np.tile(tri, (n, 1)), np.arange(3 * n))
+    g = 150  # g x g quads in a plane
+    xs, ys = np.meshgrid(np.arange(g + 1, dtype=np.float32), np.arange(g + 1, dtype=np.float32))
+    verts = np.stack([xs.ravel(), ys.ravel(), np.zeros(xs.size, np.float32)], 1)
+    q = (np.arange(g)[None, :] + (g + 1) * np.arange(g)[:, None]).ravel()
+    quads = np.stack([q, q + 1, q + g + 2, q, q + g + 2, q + g + 1], 1).reshape(-1, 3)
+    grid = _mesh_desc(verts, quads)
+    for d in (same, grid):
+        ps = G.product_scene(d)
+        assert ps.getBuildSeconds() < 1.0
+        check_bvh(d)
