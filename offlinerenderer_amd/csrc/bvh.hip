@@ -314,12 +314,15 @@ __global__ __launch_bounds__(256) void k_ploc_nn(const float4* __restrict__ clo,
     if (i >= (int)c) return;
     const int li = (int)threadIdx.x + R;
     const float4 alo = sLo[li], ahi = sHi[li];
+    // (c >= 2 here; a NaN area, from invalid vertices, counts as +inf, so
+    // every cluster still names a neighbour and the rounds make progress)
     float best = INFINITY;
-    int bj = i;
+    int bj = (i ^ 1) < (int)c ? (i ^ 1) : i - 1;
     const int j0 = max(i - R, 0), j1 = min(i + R, (int)c - 1);
     for (int j = j0; j <= j1; ++j) {
         if (j == i) continue;
-        const float a = merged_area(alo, ahi, sLo[j - base], sHi[j - base]);
+        float a = merged_area(alo, ahi, sLo[j - base], sHi[j - base]);
+        if (a != a) a = INFINITY;
         // ascending j: among equal areas the first candidate is the smallest
         // (min, max) pair, so only the aligned partner i ^ 1 can displace it
         if (a < best || (a == best && j == (i ^ 1))) { best = a; bj = j; }
