@@ -73,8 +73,8 @@ def main():
                     help="serving: admission order of pending requests")
     ap.add_argument("--dn-scenes", type=int, default=8, help="denoise: training scenes")
     ap.add_argument("--dn-cams", type=int, default=48, help="denoise: cameras per training scene")
-    ap.add_argument("--dn-target-spp", type=int, default=1024, help="denoise: spp of the training targets")
-    ap.add_argument("--dn-steps", type=int, default=4000, help="denoise: training steps (16 crops of 128^2)")
+    ap.add_argument("--dn-target-spp", type=int, default=2048, help="denoise: spp of the training targets")
+    ap.add_argument("--dn-steps", type=int, default=16000, help="denoise: training steps (16 crops of 128^2)")
     ap.add_argument("--dn-loss", choices=["relmse_split", "relmse", "relmse_pred", "log"], default="relmse_split",
                     help="denoise: training loss")
     ap.add_argument("--dn-lr", type=float, default=1e-3, help="denoise: peak learning rate")
@@ -411,7 +411,8 @@ def denoise(args):
         "data": "synthetic (tracer-generated: seeded large-scene generator, random cameras)",
         "config": {
             "workload": f"denoise_{W}x{H}_4spp_unet",
-            "train_views": args.dn_scenes * args.dn_cams, "train_input_spp": [4, 16, 64], "train_res": 256, "train_target_spp": args.dn_target_spp,
+            "train_views": args.dn_scenes * args.dn_cams, "train_input_spp": [4, 16, 64, 256], "train_res": 256,
+            "train_target_spp": args.dn_target_spp,
             "train_steps": args.dn_steps, "batch": "16 x 128^2 crops", "params": net.num_params,
             "loss": args.dn_loss,
             "eval_reference_spp": args.dn_ref_spp,
@@ -471,6 +472,7 @@ def denoise_infer(args):
     q = wr.RenderParams(width=W, height=H, spp=4, maxBounces=d.maxBounces, russianRouletteDepth=d.rrDepth)
     dev = [torch.from_numpy(a).cuda().unsqueeze(0) for a in r.renderWithGuides(s, cam, q)]
     out = torch.empty_like(dev[0])
+    net.set_sample_counts([4])
     net.forward(*dev, out)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

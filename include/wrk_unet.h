@@ -48,10 +48,14 @@ int wrk_up2_fwd(const uint16_t* x, uint32_t xPitch, uint32_t c, uint16_t* y, uin
 int wrk_up2_bwd(const uint16_t* dy, uint32_t dyPitch, uint32_t c, uint16_t* dx, uint32_t dxPitch, uint32_t n,
                 uint32_t h, uint32_t w, void* stream);
 /* Network input (32 channels): log(1 + max(colour, 0)), albedo, normal,
- * zeros, from FP32 (img, srcH, srcW, 3) layers; origins (device, 3 per
- * crop: image, x0, y0) select n crops of h x w, or NULL for whole images. */
+ * a per-image feature (channel 9: feature[image] for image < numFeature,
+ * else 0; the network's is log2(spp) / 16), zeros, from FP32
+ * (img, srcH, srcW, 3) layers; origins (device, 3 per crop: image, x0, y0)
+ * select n crops of h x w, or NULL for whole images. feature is a device
+ * array or NULL. */
 int wrk_unet_pack(const float* color, const float* albedo, const float* normal, uint32_t srcH, uint32_t srcW,
-                  const uint32_t* origins, uint16_t* out, uint32_t n, uint32_t h, uint32_t w, void* stream);
+                  const uint32_t* origins, const float* feature, uint32_t numFeature, uint16_t* out, uint32_t n,
+                  uint32_t h, uint32_t w, void* stream);
 /* out (npix, 3) FP32 = max(exp(q) - 1, 0) of channels 0..2, q = pred, or
  * pred + base (the packed input: a correction of its log colour) */
 int wrk_unet_unpack(const uint16_t* pred, uint32_t pitch, const uint16_t* base, uint32_t basePitch, float* out,
@@ -81,6 +85,10 @@ int wrk_unet_get_params(void* net, float* hostOut);
 int wrk_unet_set_params(void* net, const float* hostIn);
 /* gradient of the last wrk_unet_train_step (layout of the parameters) */
 int wrk_unet_get_grads(void* net, float* hostOut);
+/* Sample counts of the source images (host array, count entries) that the
+ * next forward / training calls see: input channel 9 = log2(spp) / 16 of
+ * each pixel's image (0 for images beyond count, or before any call). */
+int wrk_unet_set_sample_counts(void* net, const uint32_t* spp, uint32_t count);
 /* denoised colour (n, h, w, 3) FP32 into out; h, w multiples of 8 */
 int wrk_unet_forward(void* net, const float* color, const float* albedo, const float* normal, uint32_t srcH,
                      uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w, float* out,

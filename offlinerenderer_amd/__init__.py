@@ -143,7 +143,8 @@ _SIGS = {
                                 C.c_uint32, _P]),
     "wrk_up2_fwd": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "wrk_up2_bwd": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
-    "wrk_unet_pack": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "wrk_unet_pack": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32,
+                                C.c_uint32, _P]),
     "wrk_unet_unpack": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint64, _P]),
     "wrk_unet_loss_scratch_bytes": (C.c_uint64, [C.c_uint32] * 3),
     "wrk_unet_loss": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint32, _P,
@@ -157,6 +158,7 @@ _SIGS = {
     "wrk_unet_get_params": (C.c_int, [_P, _P]),
     "wrk_unet_set_params": (C.c_int, [_P, _P]),
     "wrk_unet_get_grads": (C.c_int, [_P, _P]),
+    "wrk_unet_set_sample_counts": (C.c_int, [_P, _P, C.c_uint32]),
     "wrk_unet_forward": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                                    _P, _P]),
     "wrk_unet_train_step": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32,

@@ -331,8 +331,9 @@ __global__ __launch_bounds__(256) void k_up2_bwd(const bf16* __restrict__ dy, ui
 // starting at (sx0, sy0) of an image of srcH x srcW (crops for training).
 __global__ __launch_bounds__(256) void k_unet_pack(const float* __restrict__ color, const float* __restrict__ albedo,
                                                    const float* __restrict__ normal, uint32_t srcH, uint32_t srcW,
-                                                   const uint32_t* __restrict__ origins, bf16* __restrict__ out,
-                                                   uint32_t N, uint32_t H, uint32_t W) {
+                                                   const uint32_t* __restrict__ origins,
+                                                   const float* __restrict__ feature, uint32_t numFeature,
+                                                   bf16* __restrict__ out, uint32_t N, uint32_t H, uint32_t W) {
     const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (p >= (uint64_t)N * H * W) return;
     const uint32_t xx = (uint32_t)(p % W), yy = (uint32_t)((p / W) % H), n = (uint32_t)(p / ((uint64_t)W * H));
@@ -352,6 +353,9 @@ __global__ __launch_bounds__(256) void k_unet_pack(const float* __restrict__ col
         v[3 + c] = albedo[s * 3 + c];
         v[6 + c] = normal[s * 3 + c];
     }
+    // per-image feature (the sample count, log2(spp) / 16): tells the
+    // network how noisy its input is
+    if (feature && img < numFeature) v[9] = feature[img];
     union { uint4 u[4]; bf16 h[32]; } o;
 #pragma unroll
     for (int c = 0; c < 32; ++c) o.h[c] = f2bf(v[c]);
@@ -606,14 +610,14 @@ extern "C" int wrk_up2_bwd(const uint16_t* dy, uint32_t dyPitch, uint32_t c, uin
 }
 
 extern "C" int wrk_unet_pack(const float* color, const float* albedo, const float* normal, uint32_t srcH,
-                             uint32_t srcW, const uint32_t* origins, uint16_t* out, uint32_t n, uint32_t h, uint32_t w,
-                             void* stream) {
+                             uint32_t srcW, const uint32_t* origins, const float* feature, uint32_t numFeature,
+                             uint16_t* out, uint32_t n, uint32_t h, uint32_t w, void* stream) {
     WRK_REQUIRE(color && albedo && normal && out, "wrk_unet_pack: null argument");
     WRK_REQUIRE(origins || (srcH == h && srcW == w), "wrk_unet_pack: crops need origins");
     const uint64_t np = (uint64_t)n * h * w;
     if (np == 0) return WRK_OK;
     hipLaunchKernelGGL(k_unet_pack, dim3(ceil_div(np, 256)), dim3(256), 0, as_stream(stream), color, albedo, normal,
-                       srcH, srcW, origins, (bf16*)out, n, h, w);
+                       srcH, srcW, origins, feature, numFeature, (bf16*)out, n, h, w);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }

@@ -124,6 +124,17 @@ public:
     void getParams(float* host) const {
         hip_ok(hipMemcpy(host, P_.p, nParams_ * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
     }
+    // input channel 9 of image k: log2(spp[k]) / 16
+    void setSampleCounts(const uint32_t* spp, uint32_t count) {
+        if (count && !spp) throw std::runtime_error("wrk_unet_set_sample_counts: null counts");
+        std::vector<float> f(count);
+        for (uint32_t k = 0; k < count; ++k) f[k] = spp[k] ? std::log2((float)spp[k]) / 16.0f : 0.0f;
+        if (count) {
+            feat_.need((size_t)count * 4);
+            hip_ok(hipMemcpy(feat_.p, f.data(), (size_t)count * 4, hipMemcpyHostToDevice), "hipMemcpy H2D");
+        }
+        featCount_ = count;
+    }
     void getGrads(float* host) const {
         hip_ok(hipMemcpy(host, G_.p, nParams_ * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
     }
@@ -131,7 +142,9 @@ public:
     void forward(const float* color, const float* albedo, const float* normal, uint32_t srcH, uint32_t srcW,
                  const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w, float* out, hipStream_t s) {
         shape(n, h, w, false);
-        wrk_ok(wrk_unet_pack(color, albedo, normal, srcH, srcW, origins, in_.buf.h(), n, h, w, s), "wrk_unet_pack");
+        wrk_ok(wrk_unet_pack(color, albedo, normal, srcH, srcW, origins, featCount_ ? feat_.f() : nullptr, featCount_,
+                             in_.buf.h(), n, h, w, s),
+               "wrk_unet_pack");
         runForward(s);
         if (out)
             wrk_ok(wrk_unet_unpack(pred_.buf.h(), 32, in_.buf.h(), 32, out, (uint64_t)n * h * w, s), "wrk_unet_unpack");
@@ -142,11 +155,12 @@ public:
                     uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w, float lr,
                     uint32_t lossMode, hipStream_t s) {
         shape(n, h, w, true);
-        wrk_ok(wrk_unet_pack(color, albedo, normal, srcH, srcW, origins, in_.buf.h(), n, h, w, s), "wrk_unet_pack");
+        wrk_ok(wrk_unet_pack(color, albedo, normal, srcH, srcW, origins, featCount_ ? feat_.f() : nullptr, featCount_,
+                             in_.buf.h(), n, h, w, s),
+               "wrk_unet_pack");
         runForward(s);
         wrk_ok(wrk_unet_loss(pred_.buf.h(), 32, target, srcH, srcW, origins, target2, in_.buf.h(), 32, gPred_.buf.h(), n, h,
-                             w, lossMode,
-                             lossPartial_.f(), s),
+                             w, lossMode, lossPartial_.f(), s),
                "wrk_unet_loss");
         runBackward(s);
         ++step_;
@@ -335,7 +349,8 @@ private:
     uint32_t n_ = 0, h_ = 0, w_ = 0;
     Act in_, a0_, cat0_, t0_, d0_, pred_, p1_, a1_, cat1_, t1_, d1_, p2_, a2_, cat2_, t2_, d2_, p3_, a3_, b3_;
     Act gPred_, g0a_, g0b_, gCat0_, g1a_, g1b_, gCat1_, gP1_, g2a_, g2b_, gCat2_, gP2_, g3a_, g3b_, gP3_;
-    Buf lossPartial_, wgradScratch_;
+    Buf lossPartial_, wgradScratch_, feat_;
+    uint32_t featCount_ = 0;
 };
 
 thread_local std::string g_unetError;
@@ -374,6 +389,9 @@ int wrk_unet_set_params(void* net, const float* in) {
 }
 int wrk_unet_get_grads(void* net, float* out) {
     return guard([&] { static_cast<UNet*>(net)->getGrads(out); });
+}
+int wrk_unet_set_sample_counts(void* net, const uint32_t* spp, uint32_t count) {
+    return guard([&] { static_cast<UNet*>(net)->setSampleCounts(spp, count); });
 }
 int wrk_unet_forward(void* net, const float* color, const float* albedo, const float* normal, uint32_t srcH,
                      uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w, float* out,

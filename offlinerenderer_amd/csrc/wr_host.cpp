@@ -466,7 +466,9 @@ struct RendererImpl {
         if (denoise) {
             const size_t bytes = (size_t)rp.width * rp.height * 3 * sizeof(float);
             denoised.alloc(bytes);
-            if (wrk_unet_forward(unet, e.deviceImage(id), e.deviceAlbedo(id), e.deviceNormal(id), rp.height,
+            const uint32_t spp = rp.spp;  // the network is told how noisy its input is
+            if (wrk_unet_set_sample_counts(unet, &spp, 1) != WRK_OK ||
+                wrk_unet_forward(unet, e.deviceImage(id), e.deviceAlbedo(id), e.deviceNormal(id), rp.height,
                                  rp.width, nullptr, 1, rp.height, rp.width, denoised.as<float>(), e.stream()) != WRK_OK)
                 throw std::runtime_error(std::string("denoiser: ") + wrk_unet_last_error());
             hip_check(hipStreamSynchronize(e.stream()), "hipStreamSynchronize");

@@ -52,18 +52,20 @@ def rel_mse(x: np.ndarray, ref: np.ndarray, eps: float = 1e-2) -> float:
     return float(np.mean((x - ref) ** 2 / (ref * ref + eps)))
 
 
-def build_dataset(num_scenes=8, cams_per_scene=48, res=256, noisy_spps=(4, 16, 64), spp_target=1024, seed=100,
+def build_dataset(num_scenes=8, cams_per_scene=48, res=256, noisy_spps=(4, 16, 64, 256), spp_target=1024, seed=100,
                   log=print):
     """CUDA float32 tensors (N, res, res, 3): color, albedo, normal, target,
-    target2. Every view is rendered at each noise level of noisy_spps
-    (samples 0..spp-1); target and target2 cover two disjoint runs of
-    spp_target / 2 samples after those."""
+    target2, and "spp" (numpy uint32, N): the input's sample count. Every
+    view is rendered at each noise level of noisy_spps (samples
+    0..spp-1); target and target2 cover two disjoint runs of spp_target / 2
+    samples after those."""
     r = wr.Renderer()
     r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=1 << 26))
     rng = np.random.default_rng(seed)
     n = num_scenes * cams_per_scene * len(noisy_spps)
     out = {k: torch.empty((n, res, res, 3), dtype=torch.float32, device="cuda")
            for k in ("color", "albedo", "normal", "target", "target2")}
+    out["spp"] = np.zeros(n, np.uint32)
     t0 = time.perf_counter()
     i = 0
     for si in range(num_scenes):
@@ -83,6 +85,7 @@ def build_dataset(num_scenes=8, cams_per_scene=48, res=256, noisy_spps=(4, 16, 6
                     out[k][i].copy_(torch.from_numpy(a))
                 out["target"][i].copy_(tgt)
                 out["target2"][i].copy_(tgt2)
+                out["spp"][i] = spp
                 i += 1
         del s
         log(f"[denoise] data: scene {si + 1}/{num_scenes}, {i} images, {time.perf_counter() - t0:.1f} s")
@@ -91,13 +94,18 @@ def build_dataset(num_scenes=8, cams_per_scene=48, res=256, noisy_spps=(4, 16, 6
 
 def train(net: denoiser.UNet, data: dict, steps=3000, batch=16, crop=128, lr=1e-3, seed=7, loss="relmse",
           log=print):
-    """Adam on random crops; cosine learning-rate decay. Returns the losses."""
+    """Adam on random crops; cosine learning-rate decay. Images are drawn
+    with probability proportional to log2(spp), so the cleaner inputs, whose
+    losses are small, are not drowned out by the noisy ones (weighting their
+    loss by sqrt(spp / 4) as well measured no better). Returns the losses."""
     n, res = data["color"].shape[0], data["color"].shape[1]
+    net.set_sample_counts(data["spp"])  # input channel 9 per source image
     g = torch.Generator(device="cpu").manual_seed(seed)
+    weight = torch.from_numpy(np.log2(np.maximum(data["spp"], 2)).astype(np.float64))
     losses = []
     t0 = time.perf_counter()
     for it in range(steps):
-        img = torch.randint(0, n, (batch,), generator=g)
+        img = torch.multinomial(weight, batch, replacement=True, generator=g)
         xy = torch.randint(0, res - crop + 1, (batch, 2), generator=g)
         origins = torch.stack([img, xy[:, 0], xy[:, 1]], 1).to(torch.int32).reshape(-1).cuda()
         step_lr = lr * 0.5 * (1.0 + math.cos(math.pi * it / steps)) + 1e-6
@@ -174,6 +182,7 @@ def evaluate(net: denoiser.UNet, width=1920, height=1080, seed=9001, spp_noisy=4
     # inference alone, device-resident inputs
     dev = [torch.from_numpy(a).cuda().unsqueeze(0) for a in r.renderWithGuides(s, cam, params(spp_noisy))]
     out = torch.empty_like(dev[0])
+    net.set_sample_counts([spp_noisy])
     net.forward(*dev, out)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

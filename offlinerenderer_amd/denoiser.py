@@ -86,6 +86,13 @@ class UNet:
     def load(self, path):
         self.set_params(np.fromfile(path, dtype=np.float32))
 
+    def set_sample_counts(self, spp):
+        """Samples per pixel of each source image (input channel 9 =
+        log2(spp) / 16); forward and training calls use them from now on."""
+        a = np.ascontiguousarray(np.atleast_1d(spp), dtype=np.uint32)
+        self._check(lib().wrk_unet_set_sample_counts(self._h, a.ctypes.data_as(C.c_void_p), a.size),
+                    "set_sample_counts")
+
     def forward(self, color, albedo, normal, out, origins=None, crop=None, stream=None):
         """color/albedo/normal: CUDA float32 (imgs, H, W, 3); out: (n, h, w, 3).
         crop = (n, h, w) with origins (CUDA uint32, 3 per crop) for crops."""

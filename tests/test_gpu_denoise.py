@@ -166,6 +166,42 @@ def test_unet_forward_matches_torch(gpu):
     assert err.mean() < 2e-3 and err.max() < 5e-2, (err.mean(), err.max())
 
 
+def test_unet_sample_count_channel(gpu):
+    """wrk_unet_set_sample_counts feeds log2(spp) / 16 of each pixel's image
+    into input channel 9; the output matches the reference with that input
+    and differs from the output without it."""
+    rng = np.random.default_rng(21)
+    n, h, w = 2, 16, 16
+    color, albedo, nrm = _images(rng, n, h, w)
+    net = denoiser.UNet(seed=5)
+    # the last layer starts at zero (identity network); random weights make
+    # the output depend on every input channel
+    params = (rng.normal(size=net.num_params) * 0.05).astype(np.float32)
+    net.set_params(params)
+    dev = [torch.from_numpy(a).cuda() for a in (color, albedo, nrm)]
+    out0 = torch.zeros((n, h, w, 3), dtype=torch.float32, device="cuda")
+    net.forward(*dev, out0)
+    spp = np.array([4, 1024], np.uint32)
+    net.set_sample_counts(spp)
+    out = torch.zeros_like(out0)
+    net.forward(*dev, out)
+    torch.cuda.synchronize()
+    x = U.pack_input(color, albedo, nrm, spp=spp)
+    pred = (U.forward(params, x)[:, 0:3] + x[:, 0:3]).permute(0, 2, 3, 1).numpy()
+    ref = np.maximum(np.expm1(pred), 0.0)
+    got = out.cpu().numpy()
+    err = np.abs(np.log1p(got) - np.log1p(ref))
+    assert err.mean() < 2e-3 and err.max() < 5e-2, (err.mean(), err.max())
+    assert np.abs(got - out0.cpu().numpy()).max() > 1e-3
+    # crops take the feature of their source image
+    net.set_sample_counts([16])
+    origins = torch.tensor([[0, 0, 0]], dtype=torch.int32, device="cuda").view(-1)
+    crop = torch.zeros((1, 8, 8, 3), dtype=torch.float32, device="cuda")
+    net.forward(*dev, crop, origins=origins, crop=(1, 8, 8))
+    torch.cuda.synchronize()
+    assert np.isfinite(crop.cpu().numpy()).all()
+
+
 def test_unet_gradients_and_training(gpu):
     rng = np.random.default_rng(12)
     n, h, w = 2, 32, 32
@@ -218,6 +254,7 @@ def test_renderer_denoises_with_shipped_weights(gpu, tmp_path):
     col, alb, nrm = r.renderWithGuides(s, cam, wr.RenderParams(width=64, height=48, spp=4, maxBounces=8))
     dev = [torch.from_numpy(a).cuda().unsqueeze(0) for a in (col, alb, nrm)]
     out = torch.empty_like(dev[0])
+    net.set_sample_counts([4])  # the renderer tells the network its spp
     net.forward(*dev, out)
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy()[0], den)
