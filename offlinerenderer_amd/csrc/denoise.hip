@@ -65,24 +65,52 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, ui
 #pragma unroll
         for (int c = 0; c < CT; ++c) acc[r][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    for (uint32_t ci0 = 0; ci0 < cin; ci0 += 32) {
-        for (int e = t; e < kIH * kIW * 4; e += 256) {
-            const int pix = e >> 2, part = e & 3;
-            const int gy = y0 + pix / kIW - 1, gx = x0 + pix % kIW - 1;
+    // Software pipeline: the input and weight tiles of the next 32 input
+    // channels are loaded into registers while the MFMAs run on the current
+    // ones in LDS, so global-load latency overlaps the math.
+    constexpr int kInParts = kIH * kIW * 4, kWParts = 9 * CT * 16 * 4;
+    constexpr int kInPer = (kInParts + 255) / 256, kWPer = (kWParts + 255) / 256;
+    uint4 rIn[kInPer], rW[kWPer];
+    auto fetch = [&](uint32_t ci0) {
+#pragma unroll
+        for (int k = 0; k < kInPer; ++k) {
+            const int e = t + 256 * k;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (gy >= 0 && gy < (int)H && gx >= 0 && gx < (int)W)
-                v = *(const uint4*)(x + (img + (size_t)gy * W + gx) * xPitch + ci0 + part * 8);
-            *(uint4*)(sIn + pix * 32 + part * 8) = v;
+            if (e < kInParts) {
+                const int pix = e >> 2, part = e & 3;
+                const int gy = y0 + pix / kIW - 1, gx = x0 + pix % kIW - 1;
+                if (gy >= 0 && gy < (int)H && gx >= 0 && gx < (int)W)
+                    v = *(const uint4*)(x + (img + (size_t)gy * W + gx) * xPitch + ci0 + part * 8);
+            }
+            rIn[k] = v;
         }
-        for (int e = t; e < 9 * CT * 16 * 4; e += 256) {
-            const int row = e >> 2, part = e & 3;
-            const int tap = row / (CT * 16), col = row % (CT * 16);
+#pragma unroll
+        for (int k = 0; k < kWPer; ++k) {
+            const int e = t + 256 * k;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (co0 + col < (int)cout)
-                v = *(const uint4*)(w + ((size_t)tap * cout + co0 + col) * cin + ci0 + part * 8);
-            *(uint4*)(sW + row * 32 + part * 8) = v;
+            if (e < kWParts) {
+                const int row = e >> 2, part = e & 3;
+                const int tap = row / (CT * 16), col = row % (CT * 16);
+                if (co0 + col < (int)cout)
+                    v = *(const uint4*)(w + ((size_t)tap * cout + co0 + col) * cin + ci0 + part * 8);
+            }
+            rW[k] = v;
+        }
+    };
+    fetch(0);
+    for (uint32_t ci0 = 0; ci0 < cin; ci0 += 32) {
+#pragma unroll
+        for (int k = 0; k < kInPer; ++k) {
+            const int e = t + 256 * k;
+            if (e < kInParts) *(uint4*)(sIn + (e >> 2) * 32 + (e & 3) * 8) = rIn[k];
+        }
+#pragma unroll
+        for (int k = 0; k < kWPer; ++k) {
+            const int e = t + 256 * k;
+            if (e < kWParts) *(uint4*)(sW + (e >> 2) * 32 + (e & 3) * 8) = rW[k];
         }
         __syncthreads();
+        if (ci0 + 32 < cin) fetch(ci0 + 32);
 #pragma unroll
         for (int tap = 0; tap < 9; ++tap) {
             const int dy = tap / 3, dx = tap % 3;
