@@ -69,6 +69,14 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrival rate, requests/s")
     ap.add_argument("--round-robin", action="store_true",
                     help="serving: one chunk per pending request per round (default: in-order admission)")
+    ap.add_argument("--express-max-paths", type=int, default=1 << 22,
+                    help="serving: requests of at most this many paths take the express lane (0 = off)")
+    ap.add_argument("--express-pool", type=int, default=1 << 23, help="serving: express lane paths in flight")
+    ap.add_argument("--express-iters", type=int, default=32,
+                    help="serving: express lane iterations per server step")
+    ap.add_argument("--express-overlap", action="store_true",
+                    help="serving: express iterations run while the main iteration is on the device "
+                    "(default: before it)")
     ap.add_argument("--policy", choices=["fifo", "shortest"], default="shortest",
                     help="serving: admission order of pending requests")
     ap.add_argument("--dn-scenes", type=int, default=8, help="denoise: training scenes")
@@ -298,8 +306,12 @@ def serving(args):
     def run_server(arrive):
         srv = wr.RenderServer(wr.ServerConfig(maxPathsInFlight=pool, maxChunkPaths=chunk,
                                               shortestFirst=args.policy == "shortest",
-                                              roundRobin=args.round_robin))
-        ids, done_t, equal = [None] * len(jobs), np.zeros(len(jobs)), 0
+                                              roundRobin=args.round_robin,
+                                              expressMaxPaths=args.express_max_paths,
+                                              expressPathsInFlight=args.express_pool,
+                                              expressIterations=args.express_iters,
+                                              expressOverlap=args.express_overlap))
+        ids, done_t, images = [None] * len(jobs), np.zeros(len(jobs)), [None] * len(jobs)
         open_ids = {}
         t0 = srv.clock()
         nxt, steps = 0, 0
@@ -317,8 +329,10 @@ def serving(args):
             for rid in srv.takeCompleted():
                 i = open_ids.pop(rid)
                 done_t[i] = srv.status(rid)["doneTime"] - t0
-                equal += digest(srv.fetch(rid)) == ref[i]
+                images[i] = srv.fetch(rid)
         stats = srv.getStats()
+        # hashing stays outside the served loop: the GPU would idle meanwhile
+        equal = sum(digest(img) == ref[i] for i, img in enumerate(images))
         return done_t, equal, steps, stats
 
     # saturation: every request at t = 0
@@ -356,11 +370,21 @@ def serving(args):
             "spp": list(scenes.SERVING_SPP), "max_depth": 8, "arrival_rate_per_s": args.rate,
             "paths_in_flight": pool, "chunk_paths": chunk,
             "admission": args.policy + (" round-robin" if args.round_robin else " in order"),
+            "express_lane": ({"max_paths": args.express_max_paths, "paths_in_flight": args.express_pool,
+                              "iterations_per_step": args.express_iters,
+                              "schedule": "overlapped" if args.express_overlap else "time-sliced"}
+                             if args.express_max_paths else None),
         },
         "serial_mpaths_per_s": round(serial_mpaths, 3),
         "speedup_vs_serial": round(server_mpaths / serial_mpaths, 3),
         "latency_server": pct(server_lat),
         "latency_serial_fifo": pct(serial_lat),
+        # the same percentiles over requests of at most 4M paths (10 of the
+        # 16 size classes, ~8% of the paths) and over the larger ones
+        "latency_by_size": {
+            cls: {"requests": int(m.sum()), "server": pct(server_lat[m]), "serial_fifo": pct(serial_lat[m])}
+            for cls, m in (("le_4M_paths", paths <= (1 << 22)), ("gt_4M_paths", paths > (1 << 22)))
+            if m.any()},
         "offered_load": round(float(paths.sum() / arrivals[-1]) / (serial_mpaths * 1e6), 3) if arrivals[-1] > 0 else None,
         "server_steps": {"saturated": steps_sat, "open_loop": steps_ol},
         "mixed_equals_alone": {"saturated": f"{eq_sat}/{len(jobs)}", "open_loop": f"{eq_ol}/{len(jobs)}"},

@@ -127,6 +127,15 @@ struct ServerConfig {
     // another, while the pool stays full); true: one chunk per pending
     // request per round (all requests progress together)
     bool roundRobin = false;
+    // express lane: requests of at most expressMaxPaths paths (0 = off) run
+    // in a second pool of expressPathsInFlight paths, so short requests do
+    // not wait out the long ones' bounces. Each step() runs up to
+    // expressIterations of its iterations before one iteration of the main
+    // pool, or, with expressOverlap, while that iteration is on the device
+    uint32_t expressMaxPaths = 0;
+    uint32_t expressPathsInFlight = 1u << 23;
+    uint32_t expressIterations = 32;
+    bool expressOverlap = false;
 };
 
 struct RequestStatus {

@@ -84,7 +84,12 @@ typedef struct wr_c_server_config {
     uint32_t maxPathsInFlight, maxChunkPaths;
     uint64_t memoryBudgetBytes;
     uint32_t iterationCap, flags; /* flags bit0: time kernels, bit1: shortest request first,
-                                     bit2: round-robin admission (one chunk per request per round) */
+                                     bit2: round-robin admission (one chunk per request per round),
+                                     bit3: express iterations overlap the main iteration */
+    uint32_t expressMaxPaths;       /* ServerConfig::expressMaxPaths (0 = no express lane) */
+    uint32_t expressPathsInFlight;  /* express pool size (0 = ServerConfig's default) */
+    uint32_t expressIterations;     /* express iterations per step (0 = ServerConfig's default) */
+    uint32_t reserved;
 } wr_c_server_config;
 
 typedef struct wr_c_request_status {

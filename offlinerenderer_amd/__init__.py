@@ -78,7 +78,8 @@ class wrk_paths(C.Structure):
 
 class wr_c_server_config(C.Structure):
     _fields_ = [("maxPathsInFlight", C.c_uint32), ("maxChunkPaths", C.c_uint32), ("memoryBudgetBytes", C.c_uint64),
-                ("iterationCap", C.c_uint32), ("flags", C.c_uint32)]
+                ("iterationCap", C.c_uint32), ("flags", C.c_uint32), ("expressMaxPaths", C.c_uint32),
+                ("expressPathsInFlight", C.c_uint32), ("expressIterations", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class wr_c_request_status(C.Structure):
@@ -473,7 +474,8 @@ class Renderer:
 @dataclass
 class ServerConfig:
     """Continuous-batching server (wr::RenderServer): path pool, admission
-    chunk, memory budget for accumulation/image buffers (0 = unlimited)."""
+    chunk, memory budget for accumulation/image buffers (0 = unlimited), and
+    the express lane for requests of at most expressMaxPaths paths (0 = off)."""
     maxPathsInFlight: int = 1 << 26
     maxChunkPaths: int = 1 << 24
     memoryBudgetBytes: int = 0
@@ -481,6 +483,10 @@ class ServerConfig:
     timeKernels: bool = False
     shortestFirst: bool = False
     roundRobin: bool = False
+    expressMaxPaths: int = 0
+    expressPathsInFlight: int = 1 << 23
+    expressIterations: int = 32
+    expressOverlap: bool = False
 
 
 class RenderServer:
@@ -492,7 +498,9 @@ class RenderServer:
         c = cfg or ServerConfig()
         cc = wr_c_server_config(int(c.maxPathsInFlight), int(c.maxChunkPaths), int(c.memoryBudgetBytes),
                                 int(c.iterationCap), (1 if c.timeKernels else 0) | (2 if c.shortestFirst else 0)
-                                | (4 if c.roundRobin else 0))
+                                | (4 if c.roundRobin else 0) | (8 if c.expressOverlap else 0),
+                                int(c.expressMaxPaths), int(c.expressPathsInFlight),
+                                int(c.expressIterations), 0)
         self._h = lib().wr_c_server_create(C.byref(cc))
         if not self._h:
             raise RuntimeError(f"RenderServer: {_err()}")

@@ -231,7 +231,7 @@ void Engine::release(uint64_t id) {
     auto it = requests_.find(id);
     if (it == requests_.end()) return;
     if (!it->second->done) throw std::runtime_error("request still in flight");
-    footprintUsed_ -= it->second->admitted ? it->second->footprint : 0;
+    footprintUsed() -= it->second->admitted ? it->second->footprint : 0;
     requests_.erase(it);
 }
 
@@ -338,8 +338,8 @@ void Engine::complete(uint32_t s) {
 // when no slot, no memory or no large enough range is free.
 bool Engine::admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase) {
     if (freeSlots_.empty() || capacity_ == 0) return false;
-    if (!r.admitted && cfg_.memoryBudgetBytes && footprintUsed_ != 0 &&
-        footprintUsed_ + r.footprint > cfg_.memoryBudgetBytes)
+    if (!r.admitted && cfg_.memoryBudgetBytes && footprintUsed() != 0 &&
+        footprintUsed() + r.footprint > cfg_.memoryBudgetBytes)
         return false;  // back-pressure: wait for memory to be released
     const uint64_t want = r.endPath - r.nextPath;
     const uint64_t chunkCap = cfg_.maxChunkPaths ? std::min<uint64_t>(cfg_.maxChunkPaths, capacity_) : capacity_;
@@ -375,7 +375,7 @@ bool Engine::admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase) {
         }
         r.admitted = true;
         r.tAdmit = now();
-        footprintUsed_ += r.footprint;
+        footprintUsed() += r.footprint;
     }
     const uint32_t s = freeSlots_.back();
     freeSlots_.pop_back();
@@ -410,7 +410,8 @@ bool Engine::admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase) {
     return true;
 }
 
-bool Engine::step() {
+void Engine::launch() {
+    if (launched_) throw std::runtime_error("wavefront iteration launched twice");
     const uint32_t S = (uint32_t)scenes_.size();
     uint32_t* qa = qA_.as<uint32_t>();
     uint32_t* qb = qB_.as<uint32_t>();
@@ -524,8 +525,8 @@ bool Engine::step() {
         // an idle pool admits any request unless the memory budget holds it
         // back until the caller fetches finished images
         const RequestState& r = *requests_.at(pending_.front());
-        const bool memoryBound = cfg_.memoryBudgetBytes && footprintUsed_ != 0 &&
-                                 footprintUsed_ + r.footprint > cfg_.memoryBudgetBytes;
+        const bool memoryBound = cfg_.memoryBudgetBytes && footprintUsed() != 0 &&
+                                 footprintUsed() + r.footprint > cfg_.memoryBudgetBytes;
         if (!memoryBound) throw std::runtime_error("wavefront admission made no progress on an idle pool");
     }
 
@@ -548,6 +549,22 @@ bool Engine::step() {
     if (numActiveSlots())
         hip_check(hipMemcpyAsync(hostLive_, live, WRK_MAX_SLOTS * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
                   "hipMemcpyAsync");
+    launchedItems_ = items;
+    launchedScenes_ = S2;
+    launched_ = true;
+}
+
+bool Engine::iterationDone() const {
+    const hipError_t e = hipStreamQuery(stream_);
+    if (e == hipErrorNotReady) return false;
+    hip_check(e, "hipStreamQuery");
+    return true;
+}
+
+bool Engine::finish() {
+    if (!launched_) throw std::runtime_error("wavefront iteration finished before it was launched");
+    launched_ = false;
+    const uint32_t items = launchedItems_, S2 = launchedScenes_;
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     deadCount_ = items ? hostOffs_[1] : 0;
     for (uint32_t s = 0; s < S2; ++s) {

@@ -59,6 +59,7 @@ struct EngineConfig {
     bool countTraversal = false;
     bool shortestFirst = false;          // admission order: fewest remaining paths first
     bool roundRobin = false;             // one chunk per pending request per round (else the first takes all that fit)
+    uint64_t* sharedFootprint = nullptr; // budget counter shared with another engine (nullptr = this engine's own)
 };
 
 class Engine {
@@ -104,7 +105,18 @@ public:
                     uint64_t firstPath, uint64_t numPaths, Mode mode, float* pathsOut, bool guides = false);
     // Admission + one wavefront iteration + completions. Returns true while
     // any request is pending or in flight.
-    bool step();
+    bool step() {
+        launch();
+        return finish();
+    }
+    // The two halves of step(): launch() enqueues the iteration, the
+    // admission of new chunks and the queue rebuild on this engine's streams
+    // without waiting; finish() waits for them and records completions.
+    // Another engine may run whole steps in between, on its own streams.
+    void launch();
+    bool finish();
+    // true once the launched iteration has finished on the device
+    bool iterationDone() const;
     bool isDone(uint64_t id) const;
     const RequestState& request(uint64_t id) const;
     size_t takeCompleted(uint64_t* ids, size_t maxIds) {
@@ -126,6 +138,7 @@ public:
     hipStream_t stream() const { return stream_; }
     void release(uint64_t id);
     size_t numPending() const { return pending_.size(); }
+    bool busy() const { return !pending_.empty() || numActiveSlots() != 0; }
     size_t numActiveSlots() const;
     RenderStats& stats() { return stats_; }
     static double now();
@@ -147,6 +160,7 @@ private:
     void retire(uint32_t slot);
     bool admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase);
     template <typename F> void timed(int category, hipStream_t stream, F&& launch);
+    uint64_t& footprintUsed() { return cfg_.sharedFootprint ? *cfg_.sharedFootprint : footprintUsed_; }
 
     EngineConfig cfg_;
     RenderStats stats_;
@@ -161,6 +175,8 @@ private:
     DevBuf qA_, qB_, cat_, scratch_, offs_, prmTable_, live_, fetch_;
     wrk_paths paths_{};
     uint32_t deadCount_ = 0;              // entries ended last iteration: qA_[0, deadCount_)
+    bool launched_ = false;               // launch() done, finish() not yet
+    uint32_t launchedItems_ = 0, launchedScenes_ = 0;
     uint32_t* hostOffs_ = nullptr;        // pinned
     uint32_t* hostLive_ = nullptr;        // pinned: live paths per slot, read back each step
     uint32_t* hostLiveInit_ = nullptr;    // pinned staging: initial live count per slot

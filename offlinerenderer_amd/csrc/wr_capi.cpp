@@ -183,6 +183,10 @@ void* wr_c_server_create(const wr_c_server_config* cfg) {
         c.timeKernels = (cfg->flags & 1u) != 0;
         c.shortestFirst = (cfg->flags & 2u) != 0;
         c.roundRobin = (cfg->flags & 4u) != 0;
+        c.expressOverlap = (cfg->flags & 8u) != 0;
+        c.expressMaxPaths = cfg->expressMaxPaths;
+        if (cfg->expressPathsInFlight) c.expressPathsInFlight = cfg->expressPathsInFlight;
+        if (cfg->expressIterations) c.expressIterations = cfg->expressIterations;
         out = new wr::RenderServer(c);
     });
     return out;

@@ -542,9 +542,29 @@ void Renderer::renderWithGuides(Scene* scene, const Camera& camera, const Render
 }
 
 // ----------------------------------------------------------- RenderServer
+// Two lanes: the main pool, and (ServerConfig::expressMaxPaths > 0) a small
+// express pool for requests of at most that many paths, so that a short
+// request takes its few bounce iterations at the express pool's pace instead
+// of one large-pool iteration per bounce. A step runs up to
+// expressIterations express iterations and then one main iteration
+// (time-sliced), or, with expressOverlap, launches the main iteration and
+// runs express iterations on the express engine's own streams while it is
+// on the device (at least one, at most expressIterations). Both lanes charge
+// one memory budget. Server request ids map to (lane, lane request id).
 struct ServerImpl {
     ServerConfig cfg;
-    std::unique_ptr<detail::Engine> eng;
+    uint64_t footprint = 0;  // accumulation + image bytes admitted, both lanes
+    std::unique_ptr<detail::Engine> eng, express;
+    struct Route {
+        detail::Engine* lane;
+        uint64_t id;
+    };
+    std::map<uint64_t, Route> routes;
+    std::map<std::pair<detail::Engine*, uint64_t>, uint64_t> back;
+    std::deque<uint64_t> completed;
+    uint64_t nextId = 1;
+    RenderStats stats;
+
     explicit ServerImpl(const ServerConfig& c) : cfg(c) {
         ensure_context();
         detail::EngineConfig e;
@@ -554,9 +574,29 @@ struct ServerImpl {
         e.timeKernels = c.timeKernels;
         e.shortestFirst = c.shortestFirst;
         e.roundRobin = c.roundRobin;
+        e.sharedFootprint = &footprint;
         eng = std::make_unique<detail::Engine>(e);
         eng->ensureCapacity(e.capacity);
+        if (c.expressMaxPaths) {
+            if (c.expressIterations == 0) throw std::runtime_error("RenderServer: expressIterations must be > 0");
+            detail::EngineConfig x = e;
+            x.capacity = std::max(c.expressPathsInFlight, 1024u);
+            x.maxChunkPaths = std::min(c.maxChunkPaths ? c.maxChunkPaths : x.capacity, x.capacity);
+            express = std::make_unique<detail::Engine>(x);
+            express->ensureCapacity(x.capacity);
+        }
     }
+    const Route& route(uint64_t id) const {
+        auto it = routes.find(id);
+        if (it == routes.end()) throw std::runtime_error("unknown request id");
+        return it->second;
+    }
+    void collect(detail::Engine& lane) {
+        uint64_t ids[64];
+        for (size_t n; (n = lane.takeCompleted(ids, 64)) != 0;)
+            for (size_t i = 0; i < n; ++i) completed.push_back(back.at({&lane, ids[i]}));
+    }
+    bool busy() const { return eng->busy() || (express && express->busy()); }
 };
 
 RenderServer::RenderServer(const ServerConfig& config) : m_impl(new ServerImpl(config)) {}
@@ -564,23 +604,55 @@ RenderServer::~RenderServer() { delete m_impl; }
 
 uint64_t RenderServer::submit(Scene* scene, const Camera& camera, const RenderParams& params) {
     if (params.width == 0 || params.height == 0 || params.spp == 0) throw std::runtime_error("render: empty image");
+    ServerImpl& s = *m_impl;
     wrk_render_params k;
     std::memset(&k, 0, sizeof(k));
     k.width = params.width; k.height = params.height; k.useNEE = params.useNEE ? 1u : 0u;
     k.maxBounces = params.maxBounces; k.rrStartDepth = params.russianRouletteDepth;
-    k.fireflyClamp = params.fireflyClamp; k.iterationCap = m_impl->cfg.iterationCap;
+    k.fireflyClamp = params.fireflyClamp; k.iterationCap = s.cfg.iterationCap;
     uint64_t total = (uint64_t)params.width * params.height * params.spp;
     const uint64_t first = (uint64_t)params.firstSample * params.width * params.height;
-    return m_impl->eng->submit(scene, detail::camera_record(camera), k, params.spp, first, total,
-                               detail::Engine::Mode::Accumulate, nullptr);
+    detail::Engine* lane = s.express && total <= s.cfg.expressMaxPaths ? s.express.get() : s.eng.get();
+    const uint64_t laneId = lane->submit(scene, detail::camera_record(camera), k, params.spp, first, total,
+                                         detail::Engine::Mode::Accumulate, nullptr);
+    const uint64_t id = s.nextId++;
+    s.routes[id] = {lane, laneId};
+    s.back[{lane, laneId}] = id;
+    return id;
 }
-bool RenderServer::step() { return m_impl->eng->step(); }
-bool RenderServer::isDone(uint64_t id) const { return m_impl->eng->isDone(id); }
+bool RenderServer::step() {
+    ServerImpl& s = *m_impl;
+    const bool main = s.eng->busy();
+    if (s.express && !s.cfg.expressOverlap) {
+        for (uint32_t i = 0; i < s.cfg.expressIterations && s.express->busy(); ++i) s.express->step();
+        s.collect(*s.express);
+    }
+    if (main) s.eng->launch();
+    if (s.express && s.cfg.expressOverlap) {
+        for (uint32_t i = 0; i < s.cfg.expressIterations && s.express->busy(); ++i) {
+            if (i > 0 && main && s.eng->iterationDone()) break;
+            s.express->step();
+        }
+        s.collect(*s.express);
+    }
+    if (main) s.eng->finish();
+    s.collect(*s.eng);
+    return s.busy();
+}
+bool RenderServer::isDone(uint64_t id) const {
+    const auto& r = m_impl->route(id);
+    return r.lane->isDone(r.id);
+}
 uint32_t RenderServer::takeCompleted(uint64_t* ids, uint32_t maxIds) {
-    return (uint32_t)m_impl->eng->takeCompleted(ids, maxIds);
+    auto& c = m_impl->completed;
+    const size_t n = std::min<size_t>(maxIds, c.size());
+    std::copy(c.begin(), c.begin() + n, ids);
+    c.erase(c.begin(), c.begin() + n);
+    return (uint32_t)n;
 }
 RequestStatus RenderServer::status(uint64_t id) const {
-    const auto& r = m_impl->eng->request(id);
+    const auto& rt = m_impl->route(id);
+    const auto& r = rt.lane->request(rt.id);
     RequestStatus s;
     s.done = r.done;
     s.admitted = r.admitted;
@@ -590,10 +662,38 @@ RequestStatus RenderServer::status(uint64_t id) const {
     s.paths = r.endPath - r.firstPath;
     return s;
 }
-void RenderServer::fetch(uint64_t id, Vec3* out) { m_impl->eng->fetch(id, reinterpret_cast<float*>(out)); }
-uint32_t RenderServer::numPending() const { return (uint32_t)m_impl->eng->numPending(); }
-uint32_t RenderServer::numInFlight() const { return (uint32_t)m_impl->eng->numActiveSlots(); }
-const RenderStats& RenderServer::getStats() const { return m_impl->eng->stats(); }
+void RenderServer::fetch(uint64_t id, Vec3* out) {
+    ServerImpl& s = *m_impl;
+    const ServerImpl::Route rt = s.route(id);
+    rt.lane->fetch(rt.id, reinterpret_cast<float*>(out));
+    s.back.erase({rt.lane, rt.id});
+    s.routes.erase(id);
+}
+uint32_t RenderServer::numPending() const {
+    const ServerImpl& s = *m_impl;
+    return (uint32_t)(s.eng->numPending() + (s.express ? s.express->numPending() : 0));
+}
+uint32_t RenderServer::numInFlight() const {
+    const ServerImpl& s = *m_impl;
+    return (uint32_t)(s.eng->numActiveSlots() + (s.express ? s.express->numActiveSlots() : 0));
+}
+const RenderStats& RenderServer::getStats() const {
+    ServerImpl& s = *m_impl;
+    s.stats = s.eng->stats();
+    if (s.express) {
+        const RenderStats& x = s.express->stats();
+        RenderStats& t = s.stats;
+        t.seconds += x.seconds; t.paths += x.paths; t.closestRays += x.closestRays; t.shadowRays += x.shadowRays;
+        t.iterations += x.iterations; t.batches += x.batches;
+        t.msGenerate += x.msGenerate; t.msTraceClosest += x.msTraceClosest; t.msTraceAny += x.msTraceAny;
+        t.msClassify += x.msClassify; t.msPartition += x.msPartition; t.msShade += x.msShade;
+        t.msAccumulate += x.msAccumulate;
+        t.launchesTraceClosest += x.launchesTraceClosest; t.launchesTraceAny += x.launchesTraceAny;
+        t.closestNodeFetches += x.closestNodeFetches; t.closestTriFetches += x.closestTriFetches;
+        t.anyNodeFetches += x.anyNodeFetches; t.anyTriFetches += x.anyTriFetches;
+    }
+    return s.stats;
+}
 double RenderServer::clock() { return detail::Engine::now(); }
 
 }  // namespace wr

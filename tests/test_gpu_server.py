@@ -154,3 +154,75 @@ def test_renderer_small_chunks_match_one_batch(world):
     b = many.render(sm, cam, prm)
     assert _same(a, b)
     assert many.getStats()["batches"] >= (dm.width * dm.height * dm.spp) // 1500
+
+
+@pytest.mark.parametrize("stagger,overlap", [(False, False), (True, False), (False, True), (True, True)])
+def test_express_lane_matches_alone(world, stagger, overlap):
+    """Requests of at most expressMaxPaths paths run in the express pool
+    (here split into chunks, a few iterations per step, before or during
+    the main iteration) and still equal their stand-alone renders."""
+    reqs = _requests(world)
+    ref = _alone(reqs, cap=0)
+    cfg = wr.ServerConfig(maxPathsInFlight=1 << 20, expressMaxPaths=5000, expressPathsInFlight=4096,
+                          expressIterations=3, expressOverlap=overlap)
+    got, srv = _serve(reqs, cfg, stagger=stagger)
+    for i, (a, b) in enumerate(zip(got, ref)):
+        assert _same(a, b), f"request {i} differs (express lane, stagger {stagger}, overlap {overlap})"
+    assert srv.numPending() == 0 and srv.numInFlight() == 0
+    st = srv.getStats()
+    assert st["closestRays"] > 0 and st["iterations"] > 0
+
+
+def test_express_request_passes_long_one(world):
+    """A short request submitted behind a long one that fills the pool
+    finishes first when it takes the express lane, and after the long one
+    without it (in-order admission)."""
+    dc, sc = world["cornell"]
+    dm, sm = world["material"]
+    long_req = (sc, wr.Camera(**dc.camera), wr.RenderParams(width=64, height=64, spp=16, maxBounces=8))
+    short_req = (sm, wr.Camera(**dm.camera), wr.RenderParams(width=32, height=32, spp=1, maxBounces=8))
+    ref = _alone([long_req, short_req], cap=0)
+    order = {}
+    for express in (0, 2048):
+        srv = wr.RenderServer(wr.ServerConfig(maxPathsInFlight=1 << 16, expressMaxPaths=express))
+        ids = [srv.submit(*long_req), srv.submit(*short_req)]
+        done, imgs = [], {}
+        for _ in range(10000):
+            more = srv.step()
+            for rid in srv.takeCompleted():
+                done.append(rid)
+                imgs[rid] = srv.fetch(rid)
+            if not more:
+                break
+        assert len(done) == 2
+        order[express] = [ids.index(r) for r in done]
+        for k, rid in enumerate(ids):
+            assert _same(imgs[rid], ref[k])
+    assert order[2048] == [1, 0]
+    assert order[0] == [0, 1]
+
+
+def test_express_lane_shares_memory_budget(world):
+    """Both lanes charge one budget: with room for one request's buffers, a
+    request is never admitted while another holds its buffers."""
+    dc, sc = world["cornell"]
+    dm, sm = world["material"]
+    reqs = [(sc, wr.Camera(**dc.camera), wr.RenderParams(width=64, height=64, spp=2, maxBounces=8)),
+            (sm, wr.Camera(**dm.camera), wr.RenderParams(width=32, height=48, spp=1, maxBounces=8)),
+            (sc, wr.Camera(**dc.camera), wr.RenderParams(width=64, height=64, spp=1, maxBounces=8))]
+    ref = _alone(reqs, cap=0)
+    budget = 64 * 64 * 24
+    srv = wr.RenderServer(wr.ServerConfig(maxPathsInFlight=1 << 16, memoryBudgetBytes=budget,
+                                          expressMaxPaths=2000))
+    ids = [srv.submit(*r) for r in reqs]
+    fetched = {}
+    for _ in range(10000):
+        more = srv.step()
+        holding = [rid for rid in ids if rid not in fetched and srv.status(rid)["admitted"]]
+        assert len(holding) <= 1, "two requests hold buffers beyond the budget"
+        for rid in srv.takeCompleted():
+            fetched[rid] = srv.fetch(rid)
+        if not more:
+            break
+    for k, rid in enumerate(ids):
+        assert _same(fetched[rid], ref[k])
