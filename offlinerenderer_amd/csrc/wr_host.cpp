@@ -629,9 +629,15 @@ bool RenderServer::step() {
     }
     if (main) s.eng->launch();
     if (s.express && s.cfg.expressOverlap) {
-        for (uint32_t i = 0; i < s.cfg.expressIterations && s.express->busy(); ++i) {
-            if (i > 0 && main && s.eng->iterationDone()) break;
-            s.express->step();
+        try {
+            for (uint32_t i = 0; i < s.cfg.expressIterations && s.express->busy(); ++i) {
+                if (i > 0 && main && s.eng->iterationDone()) break;
+                s.express->step();
+            }
+        } catch (...) {
+            // the launched main iteration still completes and is accounted
+            if (main) s.eng->finish();
+            throw;
         }
         s.collect(*s.express);
     }
