@@ -1,7 +1,7 @@
 /*
  * wr_c.h — C ABI over the wr:: C++ API, for foreign-function callers
  * (Python ctypes, Go cgo, JNI, N-API). Each entry point forwards to the
- * wr:: method named in its comment (libWR/include/wr/*.h signatures) and
+ * wr:: method named in its comment (signatures of the libWR/include/wr/ headers) and
  * turns C++ exceptions into a negative return code, with the exception's
  * message available from wr_c_last_error().
  */
