@@ -67,17 +67,20 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
         hip_check(hipEventCreateWithFlags(&sceneDone_[s], hipEventDisableTiming), "hipEventCreate");
     }
     hip_check(hipEventCreateWithFlags(&stepStart_, hipEventDisableTiming), "hipEventCreate");
-    hip_check(hipHostMalloc((void**)&hostOffs_, 64 * sizeof(uint32_t), hipHostMallocDefault), "hipHostMalloc");
-    hip_check(hipHostMalloc((void**)&hostLive_, WRK_MAX_SLOTS * sizeof(uint32_t), hipHostMallocDefault),
+    // one pinned block read back each step: the 16 words at offs_ + 48 (the
+    // queue offsets) and the per-slot live counts behind them
+    hip_check(hipHostMalloc((void**)&hostOffs_, (16 + WRK_MAX_SLOTS) * sizeof(uint32_t), hipHostMallocDefault),
               "hipHostMalloc");
+    hostLive_ = hostOffs_ + 16;
     hip_check(hipHostMalloc((void**)&hostPrm_, WRK_MAX_SLOTS * sizeof(wrk_render_params), hipHostMallocDefault),
               "hipHostMalloc");
     hip_check(hipHostMalloc((void**)&hostLiveInit_, WRK_MAX_SLOTS * sizeof(uint32_t), hipHostMallocDefault),
               "hipHostMalloc");
     std::memset(hostPrm_, 0, WRK_MAX_SLOTS * sizeof(wrk_render_params));
-    offs_.alloc(64 * sizeof(uint32_t));
+    // offs_: per-scene partition offsets at 8 * scene, the queue offsets at
+    // 48, the per-slot live counts at 64
+    offs_.alloc((64 + WRK_MAX_SLOTS) * sizeof(uint32_t));
     prmTable_.alloc(WRK_MAX_SLOTS * sizeof(wrk_render_params));
-    live_.alloc(WRK_MAX_SLOTS * sizeof(uint32_t));
     fetch_.alloc(64);
     hip_check(hipMemsetAsync(fetch_.ptr, 0, 64, stream_), "hipMemsetAsync");
     for (uint32_t s = WRK_MAX_SLOTS; s-- > 0;) freeSlots_.push_back(s);
@@ -86,7 +89,6 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
 Engine::~Engine() {
     for (hipEvent_t e : eventPool_) (void)hipEventDestroy(e);
     if (hostOffs_) (void)hipHostFree(hostOffs_);
-    if (hostLive_) (void)hipHostFree(hostLive_);
     if (hostPrm_) (void)hipHostFree(hostPrm_);
     if (hostLiveInit_) (void)hipHostFree(hostLiveInit_);
     for (uint32_t s = 0; s < kMaxScenes; ++s) {
@@ -393,8 +395,8 @@ bool Engine::admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase) {
     hip_check(hipMemcpyAsync(prmTable_.as<wrk_render_params>() + s, hostPrm_ + s, sizeof(wrk_render_params),
                              hipMemcpyHostToDevice, stream_),
               "hipMemcpyAsync");
-    hip_check(hipMemcpyAsync(live_.as<uint32_t>() + s, hostLiveInit_ + s, sizeof(uint32_t), hipMemcpyHostToDevice,
-                             stream_),
+    hip_check(hipMemcpyAsync(offs_.as<uint32_t>() + 64 + s, hostLiveInit_ + s, sizeof(uint32_t),
+                             hipMemcpyHostToDevice, stream_),
               "hipMemcpyAsync");
     const uint32_t at = queueBase + *added;
     timed(T_GENERATE, stream_, [&] {
@@ -418,7 +420,7 @@ void Engine::launch() {
     uint8_t* cat = cat_.as<uint8_t>();
     uint32_t* off = offs_.as<uint32_t>();
     const wrk_render_params* prmTable = prmTable_.as<wrk_render_params>();
-    uint32_t* live = live_.as<uint32_t>();
+    uint32_t* live = off + 64;
     unsigned long long* fc = fetch_.as<unsigned long long>();
     eventNext_ = 0;
     marks_.clear();
@@ -539,15 +541,15 @@ void Engine::launch() {
             wrk_check(wrk_partition(cat, qb, items, K, qa, off + 48, scratch_.ptr, scratch_.bytes, stream_),
                       "wrk_partition");
         });
-        hip_check(hipMemcpyAsync(hostOffs_, off + 48, (K + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
-                  "hipMemcpyAsync");
         // live paths per slot: subtract the paths that ended
         timed(T_PARTITION, stream_, [&] {
             wrk_check(wrk_slot_finish(qa, items, off + 48 + 1, &paths_, live, stream_), "wrk_slot_finish");
         });
     }
-    if (numActiveSlots())
-        hip_check(hipMemcpyAsync(hostLive_, live, WRK_MAX_SLOTS * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+    // queue offsets and live counts in one read-back
+    if (items || numActiveSlots())
+        hip_check(hipMemcpyAsync(hostOffs_, off + 48, (16 + WRK_MAX_SLOTS) * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, stream_),
                   "hipMemcpyAsync");
     launchedItems_ = items;
     launchedScenes_ = S2;

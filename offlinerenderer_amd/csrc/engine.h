@@ -172,13 +172,13 @@ private:
     DevBuf sceneScratch_[4];
     uint32_t capacity_ = 0;
     DevBuf org_, dir_, thr_, rad_, st_, contrib_, savedOrg_, savedDir_, hit_, occl_, result_, guides_;
-    DevBuf qA_, qB_, cat_, scratch_, offs_, prmTable_, live_, fetch_;
+    DevBuf qA_, qB_, cat_, scratch_, offs_, prmTable_, fetch_;
     wrk_paths paths_{};
     uint32_t deadCount_ = 0;              // entries ended last iteration: qA_[0, deadCount_)
     bool launched_ = false;               // launch() done, finish() not yet
     uint32_t launchedItems_ = 0, launchedScenes_ = 0;
-    uint32_t* hostOffs_ = nullptr;        // pinned
-    uint32_t* hostLive_ = nullptr;        // pinned: live paths per slot, read back each step
+    uint32_t* hostOffs_ = nullptr;        // pinned: queue offsets, then the live counts
+    uint32_t* hostLive_ = nullptr;        // hostOffs_ + 16: live paths per slot, read back each step
     uint32_t* hostLiveInit_ = nullptr;    // pinned staging: initial live count per slot
     wrk_render_params* hostPrm_ = nullptr;  // pinned staging for the slot table
     std::vector<Scene*> scenes_;

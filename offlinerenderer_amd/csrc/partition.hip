@@ -5,8 +5,9 @@
 //   1. k_part_count  — each 1024-item tile counts its items per category
 //                      with wave ballots (one popcount per wave per category);
 //   2. k_scan_*      — a three-launch scan of the K x tiles count matrix
-//                      (category-major) gives every (category, tile) its
-//                      output offset;
+//                      (category-major; one launch when it fits one
+//                      segment) gives every (category, tile) its output
+//                      offset;
 //   3. k_part_scatter— each tile ranks its items stably (ballot prefix inside
 //                      the wave, LDS prefix across the 4 waves, running total
 //                      across the 4 rounds) and writes them.
@@ -137,6 +138,34 @@ __global__ __launch_bounds__(1024) void k_scan_apply(uint32_t* __restrict__ coun
     }
 }
 
+// The whole scan in one block when the count matrix fits one segment
+// (m <= kScanSeg: up to 8192 / K tiles), which saves two launches on the
+// small queues of short requests.
+__global__ __launch_bounds__(1024) void k_scan_single(uint32_t* __restrict__ counts, uint32_t m, uint32_t numTiles,
+                                                      uint32_t K, uint32_t* __restrict__ offsets) {
+    __shared__ uint32_t waveSum[16];
+    __shared__ uint32_t total;
+    const uint32_t base = threadIdx.x * 8;
+    uint32_t v[8];
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        v[i] = base + i < m ? counts[base + i] : 0u;
+        local += v[i];
+    }
+    uint32_t run = block_exclusive_scan_1024(local, waveSum, &total);
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        uint32_t e = base + i;
+        if (e < m) {
+            counts[e] = run;
+            if (e % numTiles == 0) offsets[e / numTiles] = run;
+        }
+        run += v[i];
+    }
+    if (threadIdx.x == 0) offsets[K] = total;
+}
+
 __global__ void k_zero_offsets(uint32_t* offsets, uint32_t n) {
     if (threadIdx.x < n) offsets[threadIdx.x] = 0;
 }
@@ -196,9 +225,14 @@ static int partition_impl(const uint8_t* category, const uint32_t* in, uint32_t 
     uint32_t* counts = static_cast<uint32_t*>(scratch);
     uint32_t* segTotals = counts + m;
     hipLaunchKernelGGL(k_part_count<K>, dim3(numTiles), dim3(256), 0, s, category, n, numTiles, counts);
-    hipLaunchKernelGGL(k_scan_segments, dim3(numSeg), dim3(1024), 0, s, counts, m, segTotals);
-    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, s, segTotals, numSeg, K, offsetsOut);
-    hipLaunchKernelGGL(k_scan_apply, dim3(numSeg), dim3(1024), 0, s, counts, m, segTotals, numTiles, K, offsetsOut);
+    if (numSeg == 1) {
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s, counts, m, numTiles, K, offsetsOut);
+    } else {
+        hipLaunchKernelGGL(k_scan_segments, dim3(numSeg), dim3(1024), 0, s, counts, m, segTotals);
+        hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, s, segTotals, numSeg, K, offsetsOut);
+        hipLaunchKernelGGL(k_scan_apply, dim3(numSeg), dim3(1024), 0, s, counts, m, segTotals, numTiles, K,
+                           offsetsOut);
+    }
     hipLaunchKernelGGL(k_part_scatter<K>, dim3(numTiles), dim3(256), 0, s, category, in, n, numTiles, counts, out);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
