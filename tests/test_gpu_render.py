@@ -99,3 +99,35 @@ def test_first_sample_offset_matches_oracle(gpu, oracle):
     acc = oracle.render_image(os_, d.camera, oracle.params_in(24, 16, 8, 3.0, iterationCap=8), 3, s0=5, use_bvh=False)
     ref = acc * np.float32(1.0 / 3)
     assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+
+
+def test_converged_image_within_3_sigma_of_fp64_oracle(gpu, oracle):
+    """A 4096-spp GPU render of the Cornell box, over samples disjoint from
+    the reference's, against the FP64 oracle's 4096-spp image. The image mean
+    of each channel lies within 3 sigma, and at least 90% of the 8x8-pixel
+    block means do (per-block tails are heavy: Russian roulette divides
+    throughput by survival probabilities down to 0.05). Sigma comes from the
+    per-path variance of the oracle's paths; both images carry it."""
+    N = 4096
+    d = scenes.cornell(32, 32, 1)
+    os_ = oracle.build_oracle_scene(d, bvh=True)
+    prm = oracle.params_in(32, 32, d.maxBounces, d.rrDepth, iterationCap=0)
+    ref = oracle.render_image(os_, d.camera, prm, N, s0=0, fp64=True) / N
+    npix = 32 * 32
+    idx = np.arange(npix * N)
+    L, _ = oracle.render_paths(os_, d.camera, prm, idx % npix, idx // npix)
+    per_path = L.reshape(N, 32, 32, 3).astype(np.float64)
+    se_img = per_path.mean(axis=(1, 2)).std(0, ddof=1) / np.sqrt(N)
+    se_blk = per_path.reshape(N, 4, 8, 4, 8, 3).mean(axis=(2, 4)).std(0, ddof=1) / np.sqrt(N)
+    r = wr.Renderer()
+    r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=1 << 22))
+    img = r.render(G.product_scene(d), G.camera(d),
+                   wr.RenderParams(width=32, height=32, spp=N, maxBounces=d.maxBounces,
+                                   russianRouletteDepth=d.rrDepth, firstSample=1 << 16)).astype(np.float64)
+    z_img = (img.mean(axis=(0, 1)) - ref.mean(axis=(0, 1))) / (np.sqrt(2) * se_img)
+    blocks = lambda a: a.reshape(4, 8, 4, 8, 3).mean(axis=(1, 3))
+    z_blk = (blocks(img) - blocks(ref)) / (np.sqrt(2) * se_blk)
+    print(f"image z {np.round(z_img, 2)}, blocks within 3 sigma {np.mean(np.abs(z_blk) <= 3):.3f}, "
+          f"max |z| {np.abs(z_blk).max():.2f}")
+    assert np.all(np.abs(z_img) <= 3), z_img
+    assert np.mean(np.abs(z_blk) <= 3) >= 0.9
