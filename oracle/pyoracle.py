@@ -9,13 +9,15 @@ replayed onto it exactly as onto the product Scene.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / "build" / "liboracle.so"
+# WR_ORACLE_LIB selects another build of the same sources (the sanitized one)
+LIB = Path(os.environ.get("WR_ORACLE_LIB", HERE / "build" / "liboracle.so"))
 _lib = None
 
 WRK_MAT_LAMBERTIAN, WRK_MAT_EMISSIVE, WRK_MAT_GLASS, WRK_MAT_GGX_REFLECTION, WRK_MAT_GGX_TRANSMISSION = range(5)
