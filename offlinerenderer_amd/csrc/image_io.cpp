@@ -14,6 +14,13 @@
 
 namespace wrhost {
 
+// Largest image accepted: 2^24 texels per side (stb_image's default limit)
+// and 2^28 texels in all, so sizes never overflow and a corrupt header
+// cannot ask for an absurd allocation.
+static bool dims_ok(uint64_t w, uint64_t h) {
+    return w > 0 && h > 0 && w <= (1u << 24) && h <= (1u << 24) && w * h <= (1ull << 28);
+}
+
 static uint32_t be32(const uint8_t* p) { return (uint32_t(p[0]) << 24) | (uint32_t(p[1]) << 16) | (uint32_t(p[2]) << 8) | p[3]; }
 
 static bool read_file(const std::string& path, std::vector<uint8_t>& out) {
@@ -59,7 +66,7 @@ bool load_png_rgba8(const std::string& path, Image8& img) {
         }
         pos += 12 + len;
     }
-    if (w == 0 || h == 0 || interlace != 0 || depth != 8) return false;
+    if (!dims_ok(w, h) || interlace != 0 || depth != 8) return false;
     int channels;
     switch (ctype) {
         case 0: channels = 1; break;
@@ -132,7 +139,9 @@ bool load_hdr_rgba32f(const std::string& path, ImageF& img) {
     line(s);
     int w = 0, h = 0;
     char ya[3] = {0}, xa[3] = {0};
-    if (std::sscanf(s.c_str(), "%2s %d %2s %d", ya, &h, xa, &w) != 4 || w <= 0 || h <= 0) return false;
+    if (std::sscanf(s.c_str(), "%2s %d %2s %d", ya, &h, xa, &w) != 4 || w <= 0 || h <= 0 ||
+        !dims_ok((uint64_t)w, (uint64_t)h))
+        return false;
     std::vector<uint8_t> rgbe((size_t)w * h * 4);
     for (int y = 0; y < h; ++y) {
         uint8_t* row = &rgbe[(size_t)y * w * 4];
