@@ -66,7 +66,7 @@ def main():
     ap.add_argument("--max-chunk", type=int, default=None, help="paths per admitted chunk (default: whole pool; "
                     "8M for serving)")
     ap.add_argument("--requests", type=int, default=256, help="serving: number of requests")
-    ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrival rate, requests/s")
+    ap.add_argument("--rate", type=float, default=40.0, help="serving: Poisson arrival rate, requests/s")
     ap.add_argument("--round-robin", action="store_true",
                     help="serving: one chunk per pending request per round (default: in-order admission)")
     ap.add_argument("--express-max-paths", type=int, default=1 << 22,
@@ -77,7 +77,7 @@ def main():
     ap.add_argument("--express-overlap", action="store_true",
                     help="serving: express iterations run while the main iteration is on the device "
                     "(default: before it)")
-    ap.add_argument("--policy", choices=["fifo", "shortest"], default="shortest",
+    ap.add_argument("--policy", choices=["fifo", "shortest"], default="fifo",
                     help="serving: admission order of pending requests")
     ap.add_argument("--dn-scenes", type=int, default=16, help="denoise: training scenes")
     ap.add_argument("--dn-cams", type=int, default=48, help="denoise: cameras per training scene")
