@@ -79,10 +79,10 @@ def main():
                     "(default: before it)")
     ap.add_argument("--policy", choices=["fifo", "shortest"], default="fifo",
                     help="serving: admission order of pending requests")
-    ap.add_argument("--dn-scenes", type=int, default=16, help="denoise: training scenes")
+    ap.add_argument("--dn-scenes", type=int, default=24, help="denoise: training scenes")
     ap.add_argument("--dn-cams", type=int, default=48, help="denoise: cameras per training scene")
     ap.add_argument("--dn-target-spp", type=int, default=2048, help="denoise: spp of the training targets")
-    ap.add_argument("--dn-steps", type=int, default=40000, help="denoise: training steps (16 crops of 128^2)")
+    ap.add_argument("--dn-steps", type=int, default=60000, help="denoise: training steps (16 crops of 128^2)")
     ap.add_argument("--dn-loss", choices=["relmse_split", "relmse", "relmse_pred", "log"], default="relmse_split",
                     help="denoise: training loss")
     ap.add_argument("--dn-lr", type=float, default=1e-3, help="denoise: peak learning rate")
