@@ -89,6 +89,10 @@ def main():
     ap.add_argument("--dn-eval-size", type=str, default="1920x1080", help="denoise: evaluation resolution")
     ap.add_argument("--dn-ref-spp", type=int, default=4096, help="denoise: spp of the evaluation reference")
     ap.add_argument("--dn-save", type=str, default=None, help="denoise: write the trained weights here")
+    ap.add_argument("--dn-checkpoint", type=str, default=None,
+                    help="denoise: training checkpoint (.npz: weights, Adam state, crop generator, next step)")
+    ap.add_argument("--dn-checkpoint-every", type=int, default=5000, help="denoise: steps between checkpoints")
+    ap.add_argument("--dn-resume", type=str, default=None, help="denoise: continue training from this checkpoint")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target wall time of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -410,7 +414,9 @@ def denoise(args):
     t_data = time.perf_counter() - t0
     net = denoiser.UNet(seed=1)
     t0 = time.perf_counter()
-    losses = dt.train(net, data, steps=args.dn_steps, lr=args.dn_lr, loss=args.dn_loss, log=log)
+    losses = dt.train(net, data, steps=args.dn_steps, lr=args.dn_lr, loss=args.dn_loss, log=log,
+                      checkpoint=args.dn_checkpoint, checkpoint_every=args.dn_checkpoint_every,
+                      resume=args.dn_resume)
     t_train = time.perf_counter() - t0
     del data
     if args.dn_save:

@@ -83,6 +83,12 @@ uint64_t wrk_unet_num_params(void* net);
 int wrk_unet_init(void* net, uint64_t seed);
 int wrk_unet_get_params(void* net, float* hostOut);
 int wrk_unet_set_params(void* net, const float* hostIn);
+/* Adam state, for training checkpoints: first and second moments (layout
+ * of the parameters, NULL skips one) and the number of steps taken. No
+ * counterpart in the reference, which runs OptiX's pretrained denoiser and
+ * trains nothing; a resumed run continues bit for bit. */
+int wrk_unet_get_state(void* net, float* mOut, float* vOut, uint32_t* stepOut);
+int wrk_unet_set_state(void* net, const float* m, const float* v, uint32_t step);
 /* gradient of the last wrk_unet_train_step (layout of the parameters) */
 int wrk_unet_get_grads(void* net, float* hostOut);
 /* Sample counts of the source images (host array, count entries) that the

@@ -159,6 +159,8 @@ _SIGS = {
     "wrk_unet_get_params": (C.c_int, [_P, _P]),
     "wrk_unet_set_params": (C.c_int, [_P, _P]),
     "wrk_unet_get_grads": (C.c_int, [_P, _P]),
+    "wrk_unet_get_state": (C.c_int, [_P, _P, _P, _P]),
+    "wrk_unet_set_state": (C.c_int, [_P, _P, _P, C.c_uint32]),
     "wrk_unet_set_sample_counts": (C.c_int, [_P, _P, C.c_uint32]),
     "wrk_unet_forward": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                                    _P, _P]),

@@ -135,6 +135,21 @@ public:
         }
         featCount_ = count;
     }
+    // Adam state for training checkpoints: both moment vectors (layout of
+    // the parameters) and the step count
+    void getState(float* m, float* v, uint32_t* step) const {
+        hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        if (m) hip_ok(hipMemcpy(m, M_.p, nParams_ * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        if (v) hip_ok(hipMemcpy(v, V_.p, nParams_ * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        if (step) *step = step_;
+    }
+    void setState(const float* m, const float* v, uint32_t step) {
+        if (!m || !v) throw std::runtime_error("wrk_unet_set_state: null moments");
+        hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        hip_ok(hipMemcpy(M_.p, m, nParams_ * 4, hipMemcpyHostToDevice), "hipMemcpy H2D");
+        hip_ok(hipMemcpy(V_.p, v, nParams_ * 4, hipMemcpyHostToDevice), "hipMemcpy H2D");
+        step_ = step;
+    }
     void getGrads(float* host) const {
         hip_ok(hipMemcpy(host, G_.p, nParams_ * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
     }
@@ -386,6 +401,12 @@ int wrk_unet_get_params(void* net, float* out) {
 }
 int wrk_unet_set_params(void* net, const float* in) {
     return guard([&] { static_cast<UNet*>(net)->setParams(in); });
+}
+int wrk_unet_get_state(void* net, float* m, float* v, uint32_t* step) {
+    return guard([&] { static_cast<UNet*>(net)->getState(m, v, step); });
+}
+int wrk_unet_set_state(void* net, const float* m, const float* v, uint32_t step) {
+    return guard([&] { static_cast<UNet*>(net)->setState(m, v, step); });
 }
 int wrk_unet_get_grads(void* net, float* out) {
     return guard([&] { static_cast<UNet*>(net)->getGrads(out); });

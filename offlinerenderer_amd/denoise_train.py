@@ -93,18 +93,30 @@ def build_dataset(num_scenes=8, cams_per_scene=48, res=256, noisy_spps=(4, 16, 6
 
 
 def train(net: denoiser.UNet, data: dict, steps=3000, batch=16, crop=128, lr=1e-3, seed=7, loss="relmse",
-          log=print):
+          log=print, checkpoint=None, checkpoint_every=0, resume=None, until=None):
     """Adam on random crops; cosine learning-rate decay. Images are drawn
     with probability proportional to log2(spp), so the cleaner inputs, whose
     losses are small, are not drowned out by the noisy ones (weighting their
-    loss by sqrt(spp / 4) as well measured no better). Returns the losses."""
+    loss by sqrt(spp / 4) as well measured no better). Returns the losses.
+
+    checkpoint / checkpoint_every: write the network, the Adam state, the
+    crop generator and the next step there every so many steps and at the
+    end. resume: continue from such a file; the run then equals an
+    uninterrupted one bit for bit. until: stop before that step (the
+    schedule still spans `steps`)."""
     n, res = data["color"].shape[0], data["color"].shape[1]
     net.set_sample_counts(data["spp"])  # input channel 9 per source image
     g = torch.Generator(device="cpu").manual_seed(seed)
+    first = 0
+    if resume:
+        extra = net.load_checkpoint(resume)
+        g.set_state(torch.from_numpy(np.ascontiguousarray(extra["crop_rng"], dtype=np.uint8)))
+        first = int(extra["next_step"])
+    last = min(steps, until) if until is not None else steps
     weight = torch.from_numpy(np.log2(np.maximum(data["spp"], 2)).astype(np.float64))
     losses = []
     t0 = time.perf_counter()
-    for it in range(steps):
+    for it in range(first, last):
         img = torch.multinomial(weight, batch, replacement=True, generator=g)
         xy = torch.randint(0, res - crop + 1, (batch, 2), generator=g)
         origins = torch.stack([img, xy[:, 0], xy[:, 1]], 1).to(torch.int32).reshape(-1).cuda()
@@ -113,9 +125,11 @@ def train(net: denoiser.UNet, data: dict, steps=3000, batch=16, crop=128, lr=1e-
         ta, tb = (data["target"], data["target2"]) if it % 2 == 0 else (data["target2"], data["target"])
         losses.append(net.train_step(data["color"], data["albedo"], data["normal"], ta, step_lr,
                                      origins=origins, crop=(batch, crop, crop), loss=loss, target2=tb))
-        if (it + 1) % 500 == 0 or it + 1 == steps:
+        if (it + 1) % 500 == 0 or it + 1 == last:
             log(f"[denoise] train: step {it + 1}/{steps}, loss {np.mean(losses[-100:]):.5f}, "
                 f"{time.perf_counter() - t0:.1f} s")
+        if checkpoint and ((checkpoint_every and (it + 1) % checkpoint_every == 0) or it + 1 == last):
+            net.save_checkpoint(checkpoint, crop_rng=g.get_state().numpy(), next_step=np.uint32(it + 1))
     return losses
 
 

@@ -83,6 +83,29 @@ class UNet:
     def save(self, path):
         self.get_params().tofile(path)
 
+    def save_checkpoint(self, path, **extra):
+        """Parameters, Adam moments and step, and any extra arrays (the
+        training loop adds its crop generator's state and next step) in one
+        .npz of plain arrays."""
+        m = np.zeros(self.num_params, np.float32)
+        v = np.zeros(self.num_params, np.float32)
+        step = C.c_uint32(0)
+        self._check(lib().wrk_unet_get_state(self._h, m.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p),
+                                             C.byref(step)), "get_state")
+        with open(path, "wb") as f:
+            np.savez(f, params=self.get_params(), adam_m=m, adam_v=v, adam_step=np.uint32(step.value), **extra)
+
+    def load_checkpoint(self, path) -> dict:
+        """Restores what save_checkpoint wrote; returns the extra arrays."""
+        with np.load(path, allow_pickle=False) as z:
+            self.set_params(z["params"])
+            m = np.ascontiguousarray(z["adam_m"], dtype=np.float32)
+            v = np.ascontiguousarray(z["adam_v"], dtype=np.float32)
+            assert m.size == self.num_params and v.size == self.num_params
+            self._check(lib().wrk_unet_set_state(self._h, m.ctypes.data_as(C.c_void_p),
+                                                 v.ctypes.data_as(C.c_void_p), int(z["adam_step"])), "set_state")
+            return {k: z[k] for k in z.files if k not in ("params", "adam_m", "adam_v", "adam_step")}
+
     def load(self, path):
         self.set_params(np.fromfile(path, dtype=np.float32))
 

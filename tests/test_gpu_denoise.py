@@ -263,3 +263,29 @@ def test_renderer_denoises_with_shipped_weights(gpu, tmp_path):
     np.zeros(10, np.float32).tofile(bad)
     with pytest.raises(RuntimeError):
         r.loadDenoiser(bad)
+
+
+def test_training_checkpoint_resume_is_bit_exact(gpu, tmp_path):
+    """Six training steps in one run equal three steps, a checkpoint, and
+    three more steps in a fresh network restored from it (weights, Adam
+    moments and step, crop generator), bit for bit."""
+    from offlinerenderer_amd import denoise_train as dt
+
+    rng = np.random.default_rng(21)
+    n, res = 3, 64
+    color, albedo, nrm = _images(rng, n, res, res)
+    cuda = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+    data = {"color": cuda(color), "albedo": cuda(albedo), "normal": cuda(nrm),
+            "target": cuda(color * 0.5 + 0.1), "target2": cuda(color * 0.5 + 0.12),
+            "spp": np.array([4, 16, 64], dtype=np.uint32)}
+    kw = dict(steps=6, batch=2, crop=32, lr=1e-3, seed=5, loss="relmse_split", log=lambda *a: None)
+    whole = denoiser.UNet(seed=4)
+    dt.train(whole, data, **kw)
+    ck = tmp_path / "ck.npz"
+    first = denoiser.UNet(seed=4)
+    dt.train(first, data, checkpoint=str(ck), until=3, **kw)
+    resumed = denoiser.UNet(seed=99)  # different start: everything must come from the file
+    dt.train(resumed, data, resume=str(ck), **kw)
+    a, b = whole.get_params(), resumed.get_params()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), np.abs(a - b).max()
+    assert not np.array_equal(a, first.get_params())  # the last three steps did change the weights
