@@ -94,7 +94,7 @@ constexpr float kBoxPadRel = 9.5367431640625e-07f;  // 2^-20 outward box padding
 #define WRK_STACK_LDS 8
 #endif
 #ifndef WRK_RAYS_PER_LANE
-#define WRK_RAYS_PER_LANE 8
+#define WRK_RAYS_PER_LANE 12
 #endif
 constexpr uint32_t kStackLds = WRK_STACK_LDS;        // traversal stack entries per lane in LDS (power of 2)
 constexpr uint32_t kStackSpill = 192 - WRK_STACK_LDS; // further entries per lane in scratch

@@ -100,6 +100,22 @@ def test_many_rays_per_lane_match_one_per_lane(gpu, oracle):
     assert np.array_equal(occ_all[sl], oracle.trace_any(os_, org4[sl], dir4[sl], use_bvh=True))
 
 
+def test_rays_per_lane_cap_matches_one_per_lane(gpu):
+    """A 28M-ray launch (three copies of 9.4M rays) reaches the cap of 12
+    rays per lane on a 256-CU device; its hits equal those of 1M-ray launches
+    (one ray per lane) bit for bit."""
+    d = scenes.material_scene(96, 54, 1, tex_size=64)
+    bvh = G.product_scene(d).getBVH()
+    org1, dir1 = make_rays(d, 9 << 20, 13)
+    org4, dir4 = np.concatenate([org1] * 3), np.concatenate([dir1] * 3)
+    n, m = org4.shape[0], 1 << 20
+    t_all, p_all = G.trace_closest(bvh, org4, dir4)
+    for k in range(0, n, m):
+        t_k, p_k = G.trace_closest(bvh, org4[k:k + m], dir4[k:k + m])
+        _compare(t_all[k:k + m], p_all[k:k + m], t_k, p_k)
+    assert np.count_nonzero(p_all != 0xFFFFFFFF) > n // 10
+
+
 def test_closest_hit_categories(gpu):
     """wrk_trace_closest_categorized: hits equal wrk_trace_closest's, and the
     category at each queue position is the material kind of the hit
