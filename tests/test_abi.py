@@ -52,3 +52,31 @@ def test_cpp_api_symbols_exported():
                 "wr::Scene::addTriangleMesh(std::span<float const, 18446744073709551615ul>, "
                 "std::span<unsigned int const, 18446744073709551615ul>, unsigned int)"]:
         assert sig in out, sig
+
+
+def test_null_handles_are_errors_not_crashes():
+    """Every wr_c_* entry point taking a handle or a required pointer rejects
+    NULL with an error code and a message (no device work is reached)."""
+    import ctypes as C
+
+    L = wr.lib()
+    cam, prm = wr.wr_c_camera(), wr.wr_c_render_params()
+    bad = 0xFFFFFFFF
+    assert L.wr_c_scene_add_lambertian(None, 1.0, 1.0, 1.0, 0) == bad
+    assert b"null" in L.wr_c_last_error()
+    assert L.wr_c_scene_add_emissive(None, 1.0, 1.0, 1.0) == bad
+    assert L.wr_c_scene_finalize(None) == -1
+    assert L.wr_c_scene_build_seconds(None) == -1.0
+    assert L.wr_c_texture_create(None, b"x.png") == 0
+    assert L.wr_c_render(None, None, C.byref(cam), C.byref(prm), None) == -1
+    assert L.wr_c_renderer_stats(None, None) == -1
+    assert L.wr_c_server_create(None) is None
+    assert b"server config" in L.wr_c_last_error()
+    assert L.wr_c_server_step(None) == -1
+    assert L.wr_c_server_submit(None, None, C.byref(cam), C.byref(prm)) == 0
+    assert L.wr_c_server_take_completed(None, None, 4) == -1
+    assert L.wr_c_server_pending(None) == 0 and L.wr_c_server_in_flight(None) == 0
+    # destroying NULL is a no-op, as delete is
+    L.wr_c_scene_destroy(None)
+    L.wr_c_renderer_destroy(None)
+    L.wr_c_server_destroy(None)
