@@ -380,6 +380,12 @@ template <typename F> int guard(F&& f) {
     }
 }
 
+// A NULL network handle is an error, not a crash.
+UNet* N(void* net) {
+    if (!net) throw std::runtime_error("wrk_unet: null network handle");
+    return static_cast<UNet*>(net);
+}
+
 }  // namespace
 
 extern "C" {
@@ -392,33 +398,37 @@ void* wrk_unet_create(void) {
     return u;
 }
 void wrk_unet_destroy(void* net) { delete static_cast<UNet*>(net); }
-uint64_t wrk_unet_num_params(void* net) { return static_cast<UNet*>(net)->numParams(); }
+uint64_t wrk_unet_num_params(void* net) {
+    uint64_t n = 0;
+    guard([&] { n = N(net)->numParams(); });
+    return n;
+}
 int wrk_unet_init(void* net, uint64_t seed) {
-    return guard([&] { static_cast<UNet*>(net)->init(seed); });
+    return guard([&] { N(net)->init(seed); });
 }
 int wrk_unet_get_params(void* net, float* out) {
-    return guard([&] { static_cast<UNet*>(net)->getParams(out); });
+    return guard([&] { N(net)->getParams(out); });
 }
 int wrk_unet_set_params(void* net, const float* in) {
-    return guard([&] { static_cast<UNet*>(net)->setParams(in); });
+    return guard([&] { N(net)->setParams(in); });
 }
 int wrk_unet_get_state(void* net, float* m, float* v, uint32_t* step) {
-    return guard([&] { static_cast<UNet*>(net)->getState(m, v, step); });
+    return guard([&] { N(net)->getState(m, v, step); });
 }
 int wrk_unet_set_state(void* net, const float* m, const float* v, uint32_t step) {
-    return guard([&] { static_cast<UNet*>(net)->setState(m, v, step); });
+    return guard([&] { N(net)->setState(m, v, step); });
 }
 int wrk_unet_get_grads(void* net, float* out) {
-    return guard([&] { static_cast<UNet*>(net)->getGrads(out); });
+    return guard([&] { N(net)->getGrads(out); });
 }
 int wrk_unet_set_sample_counts(void* net, const uint32_t* spp, uint32_t count) {
-    return guard([&] { static_cast<UNet*>(net)->setSampleCounts(spp, count); });
+    return guard([&] { N(net)->setSampleCounts(spp, count); });
 }
 int wrk_unet_forward(void* net, const float* color, const float* albedo, const float* normal, uint32_t srcH,
                      uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w, float* out,
                      void* stream) {
     return guard([&] {
-        static_cast<UNet*>(net)->forward(color, albedo, normal, srcH, srcW, origins, n, h, w, out,
+        N(net)->forward(color, albedo, normal, srcH, srcW, origins, n, h, w, out,
                                          static_cast<hipStream_t>(stream));
     });
 }
@@ -426,7 +436,7 @@ int wrk_unet_train_step(void* net, const float* color, const float* albedo, cons
                         const float* target2, uint32_t srcH, uint32_t srcW, const uint32_t* origins, uint32_t n,
                         uint32_t h, uint32_t w, float lr, uint32_t lossMode, float* loss, void* stream) {
     return guard([&] {
-        float l = static_cast<UNet*>(net)->trainStep(color, albedo, normal, target, target2, srcH, srcW, origins, n, h,
+        float l = N(net)->trainStep(color, albedo, normal, target, target2, srcH, srcW, origins, n, h,
                                                      w, lr, lossMode, static_cast<hipStream_t>(stream));
         if (loss) *loss = l;
     });

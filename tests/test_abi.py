@@ -76,7 +76,10 @@ def test_null_handles_are_errors_not_crashes():
     assert L.wr_c_server_submit(None, None, C.byref(cam), C.byref(prm)) == 0
     assert L.wr_c_server_take_completed(None, None, 4) == -1
     assert L.wr_c_server_pending(None) == 0 and L.wr_c_server_in_flight(None) == 0
+    assert L.wrk_unet_init(None, 1) != 0 and b"null" in L.wrk_unet_last_error()
+    assert L.wrk_unet_num_params(None) == 0
     # destroying NULL is a no-op, as delete is
+    L.wrk_unet_destroy(None)
     L.wr_c_scene_destroy(None)
     L.wr_c_renderer_destroy(None)
     L.wr_c_server_destroy(None)
