@@ -3,7 +3,8 @@
 // Replaces libWR/kernels/compact.cu:6-17 (one global atomic per survivor,
 // nondeterministic order) with an ordered, atomic-free reduce-then-scan:
 //   1. k_part_count  — each 1024-item tile counts its items per category
-//                      with wave ballots (one popcount per wave per category);
+//                      with wave ballots (one popcount per wave per category;
+//                      one 4-byte category load per lane when aligned);
 //   2. k_scan_*      — a three-launch scan of the K x tiles count matrix
 //                      (category-major; one launch when it fits one
 //                      segment) gives every (category, tile) its output
@@ -40,6 +41,44 @@ __global__ __launch_bounds__(256) void k_part_count(const uint8_t* __restrict__ 
     for (uint32_t r = 0; r < kRounds; ++r) {
         uint32_t j = tile * kTile + r * 256 + threadIdx.x;
         uint32_t c = j < n ? cat[j] : 0xFFu;
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k) mine[k] += __popcll(__ballot(c == k));
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k) waveCnt[wave][k] = mine[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        uint32_t k = threadIdx.x;
+        counts[k * numTiles + tile] = waveCnt[0][k] + waveCnt[1][k] + waveCnt[2][k] + waveCnt[3][k];
+    }
+}
+
+// The same counts with one 4-byte load per lane: lane t of a tile holds its
+// items 4t .. 4t+3 (counts do not depend on which lane holds an item). For a
+// 4-byte-aligned category array; the last partial word is read bytewise so
+// nothing past n is touched.
+template <uint32_t K>
+__global__ __launch_bounds__(256) void k_part_count_vec(const uint8_t* __restrict__ cat, uint32_t n,
+                                                        uint32_t numTiles, uint32_t* __restrict__ counts) {
+    __shared__ uint32_t waveCnt[4][K];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t j0 = tile * kTile + threadIdx.x * 4u;
+    uint32_t word = 0xFFFFFFFFu;
+    if (j0 + 3u < n) {
+        word = reinterpret_cast<const uint32_t*>(cat)[j0 >> 2];
+    } else {
+        for (uint32_t b = 0; b < 4u; ++b)
+            if (j0 + b < n) word = (word & ~(0xFFu << (8u * b))) | ((uint32_t)cat[j0 + b] << (8u * b));
+    }
+    uint32_t mine[K];
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) mine[k] = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < 4u; ++b) {
+        const uint32_t c = (word >> (8u * b)) & 0xFFu;
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k) mine[k] += __popcll(__ballot(c == k));
     }
@@ -210,6 +249,59 @@ __global__ __launch_bounds__(256) void k_part_scatter(const uint8_t* __restrict_
     }
 }
 
+// Scatter with the layout of k_part_count_vec: lane t holds the tile's items
+// 4t .. 4t+3, so item order within the tile is (lane, byte). An item of
+// category k lands after the tile's earlier items of k: those of earlier
+// waves (LDS totals), of earlier lanes of its wave (ballot prefixes over all
+// four bytes) and of earlier bytes of its own lane. One barrier per tile.
+template <uint32_t K>
+__global__ __launch_bounds__(256) void k_part_scatter_vec(const uint8_t* __restrict__ cat,
+                                                          const uint32_t* __restrict__ in, uint32_t n,
+                                                          uint32_t numTiles, const uint32_t* __restrict__ scanned,
+                                                          uint32_t* __restrict__ out) {
+    __shared__ uint32_t waveTot[4][K];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t lt = lanemask_lt();
+    const uint32_t j0 = tile * kTile + threadIdx.x * 4u;
+    uint32_t word = 0xFFFFFFFFu;
+    if (j0 + 3u < n) {
+        word = reinterpret_cast<const uint32_t*>(cat)[j0 >> 2];
+    } else {
+        for (uint32_t b = 0; b < 4u; ++b)
+            if (j0 + b < n) word = (word & ~(0xFFu << (8u * b))) | ((uint32_t)cat[j0 + b] << (8u * b));
+    }
+    uint32_t c[4];
+#pragma unroll
+    for (uint32_t b = 0; b < 4u; ++b) c[b] = (word >> (8u * b)) & 0xFFu;
+    uint32_t before[K];  // items of category k in earlier lanes of this wave
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        uint32_t tot = 0, bef = 0;
+#pragma unroll
+        for (uint32_t b = 0; b < 4u; ++b) {
+            const uint64_t m = __ballot(c[b] == k);
+            tot += __popcll(m);
+            bef += __popcll(m & lt);
+        }
+        before[k] = bef;
+        if (lane == 0) waveTot[wave][k] = tot;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t b = 0; b < 4u; ++b) {
+        const uint32_t k = c[b];
+        if (k >= K) continue;
+        uint32_t bef = 0;  // select, not a dynamically indexed register array
+#pragma unroll
+        for (uint32_t kk = 0; kk < K; ++kk) bef = kk == k ? before[kk] : bef;
+        uint32_t pos = scanned[k * numTiles + tile] + bef;
+        for (uint32_t w = 0; w < wave; ++w) pos += waveTot[w][k];
+        for (uint32_t b2 = 0; b2 < b; ++b2) pos += c[b2] == k ? 1u : 0u;
+        out[pos] = in ? in[j0 + b] : j0 + b;
+    }
+}
+
 template <uint32_t K>
 static int partition_impl(const uint8_t* category, const uint32_t* in, uint32_t n, uint32_t* out,
                           uint32_t* offsetsOut, void* scratch, uint64_t scratchBytes, hipStream_t s) {
@@ -224,7 +316,10 @@ static int partition_impl(const uint8_t* category, const uint32_t* in, uint32_t 
     }
     uint32_t* counts = static_cast<uint32_t*>(scratch);
     uint32_t* segTotals = counts + m;
-    hipLaunchKernelGGL(k_part_count<K>, dim3(numTiles), dim3(256), 0, s, category, n, numTiles, counts);
+    if ((reinterpret_cast<uintptr_t>(category) & 3u) == 0)
+        hipLaunchKernelGGL(k_part_count_vec<K>, dim3(numTiles), dim3(256), 0, s, category, n, numTiles, counts);
+    else
+        hipLaunchKernelGGL(k_part_count<K>, dim3(numTiles), dim3(256), 0, s, category, n, numTiles, counts);
     if (numSeg == 1) {
         hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s, counts, m, numTiles, K, offsetsOut);
     } else {
@@ -233,7 +328,12 @@ static int partition_impl(const uint8_t* category, const uint32_t* in, uint32_t 
         hipLaunchKernelGGL(k_scan_apply, dim3(numSeg), dim3(1024), 0, s, counts, m, segTotals, numTiles, K,
                            offsetsOut);
     }
-    hipLaunchKernelGGL(k_part_scatter<K>, dim3(numTiles), dim3(256), 0, s, category, in, n, numTiles, counts, out);
+    if ((reinterpret_cast<uintptr_t>(category) & 3u) == 0)
+        hipLaunchKernelGGL(k_part_scatter_vec<K>, dim3(numTiles), dim3(256), 0, s, category, in, n, numTiles, counts,
+                           out);
+    else
+        hipLaunchKernelGGL(k_part_scatter<K>, dim3(numTiles), dim3(256), 0, s, category, in, n, numTiles, counts,
+                           out);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }
