@@ -275,6 +275,12 @@ int wrk_shade(const wrk_scene_view* scene, const wrk_render_params* paramsTable,
  * state carries that slot. */
 int wrk_slot_finish(const uint32_t* finished, uint32_t maxItems, const uint32_t* countDev, const wrk_paths* paths,
                     uint32_t* live, void* stream);
+/* The same count with the slot of pool entry p taken from slotMap[p / 256]
+ * (one byte per 256 pool entries; chunks start and end on such granules),
+ * so the scattered state words are not read. Every entry of `finished` must
+ * lie below 256 times the map's length. */
+int wrk_slot_finish_map(const uint32_t* finished, uint32_t maxItems, const uint32_t* countDev,
+                        const uint8_t* slotMap, uint32_t* live, void* stream);
 /* accum[pixel] += result[poolBase + k] for the chunk's paths in sample order
  * (renderer.cpp:190 serial samples; per-pixel accumulation order is fixed).
  * With paths->guides and non-NULL accumAlbedo/accumNormal the first-hit

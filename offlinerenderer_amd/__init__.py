@@ -187,6 +187,7 @@ _SIGS = {
     "wrk_trace_closest_categorized": (C.c_int, [C.POINTER(wrk_bvh), _P, _P, _P, C.c_uint32, _P, _P, _P, _P, _P]),
     "wrk_shade": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_slot_finish": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
+    "wrk_slot_finish_map": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
     "wrk_accumulate": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P, _P, _P]),
     "wrk_average": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_tonemap_aces": (C.c_int, [_P, C.c_uint32, _P, _P]),

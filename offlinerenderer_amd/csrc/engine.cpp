@@ -108,6 +108,8 @@ void Engine::setConfig(const EngineConfig& cfg) {
 size_t Engine::numActiveSlots() const { return WRK_MAX_SLOTS - freeSlots_.size(); }
 
 void Engine::reserve(uint32_t cap) {
+    if (cap > 0xFFFFFF00u) throw std::runtime_error("path pool too large");
+    cap = granuleRound(cap);
     if (cap <= capacity_) return;
     if (numActiveSlots() != 0) throw std::runtime_error("path pool cannot grow while paths are in flight");
     size_t c = cap;
@@ -115,6 +117,7 @@ void Engine::reserve(uint32_t cap) {
     contrib_.alloc(c * 16); savedOrg_.alloc(c * 16); savedDir_.alloc(c * 16); hit_.alloc(c * 8);
     occl_.alloc(c); result_.alloc(c * 16);
     qA_.alloc(c * 4); qB_.alloc(c * 4); cat_.alloc(c);
+    slotMap_.alloc(c / 256);
     scratch_.alloc(wrk_partition_scratch_bytes(cap, kFinalCategories));
     for (uint32_t s = 0; s < kMaxScenes; ++s) sceneScratch_[s].alloc(wrk_partition_scratch_bytes(cap, 6));
     capacity_ = cap;
@@ -285,7 +288,7 @@ template <typename F> void Engine::timed(int category, hipStream_t stream, F&& l
 // Frees a chunk's pool range and slot.
 void Engine::retire(uint32_t s) {
     Slot& sl = slots_[s];
-    freeRange(sl.poolBase, sl.count);
+    freeRange(sl.poolBase, granuleRound(sl.count));
     sl.active = false;
     sl.finished = false;
     freeSlots_.push_back(s);
@@ -350,9 +353,12 @@ bool Engine::admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase) {
     const uint64_t floor = cfg_.minChunkPaths ? cfg_.minChunkPaths
                                               : std::max<uint64_t>(capacity_ / 32, std::min<uint64_t>(65536, capacity_ / 8));
     const uint64_t minChunk = numActiveSlots() == 0 ? 1 : std::min<uint64_t>(full, floor);
+    // ranges are whole granules: a chunk of n paths holds granuleRound(n)
+    // entries; free ranges are whole granules too, so when the rounded chunk
+    // does not fit, the largest free range is smaller than n itself
     uint32_t n = (uint32_t)full;
     uint32_t base = 0;
-    if (!allocRange(n, &base)) {
+    if (!allocRange(granuleRound(n), &base)) {
         const uint32_t m = largestFree();
         if (m == 0 || m < minChunk) return false;
         n = m;
@@ -381,6 +387,8 @@ bool Engine::admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase) {
     }
     const uint32_t s = freeSlots_.back();
     freeSlots_.pop_back();
+    hip_check(hipMemsetAsync(slotMap_.as<uint8_t>() + base / 256, (int)s, granuleRound(n) / 256, stream_),
+              "hipMemsetAsync");
     Slot& sl = slots_[s];
     sl.active = true;
     sl.finished = false;
@@ -543,7 +551,8 @@ void Engine::launch() {
         });
         // live paths per slot: subtract the paths that ended
         timed(T_PARTITION, stream_, [&] {
-            wrk_check(wrk_slot_finish(qa, items, off + 48 + 1, &paths_, live, stream_), "wrk_slot_finish");
+            wrk_check(wrk_slot_finish_map(qa, items, off + 48 + 1, slotMap_.as<uint8_t>(), live, stream_),
+                      "wrk_slot_finish_map");
         });
     }
     // queue offsets and live counts in one read-back

@@ -161,6 +161,9 @@ private:
     bool admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase);
     template <typename F> void timed(int category, hipStream_t stream, F&& launch);
     uint64_t& footprintUsed() { return cfg_.sharedFootprint ? *cfg_.sharedFootprint : footprintUsed_; }
+    // pool ranges are whole granules of 256 entries, so one byte per granule
+    // (slotMap_) names the slot that owns an entry
+    static uint32_t granuleRound(uint32_t n) { return (n + 255u) & ~255u; }
 
     EngineConfig cfg_;
     RenderStats stats_;
@@ -172,7 +175,7 @@ private:
     DevBuf sceneScratch_[4];
     uint32_t capacity_ = 0;
     DevBuf org_, dir_, thr_, rad_, st_, contrib_, savedOrg_, savedDir_, hit_, occl_, result_, guides_;
-    DevBuf qA_, qB_, cat_, scratch_, offs_, prmTable_, fetch_;
+    DevBuf qA_, qB_, cat_, scratch_, offs_, prmTable_, fetch_, slotMap_;
     wrk_paths paths_{};
     uint32_t deadCount_ = 0;              // entries ended last iteration: qA_[0, deadCount_)
     bool launched_ = false;               // launch() done, finish() not yet

@@ -618,9 +618,13 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_para
 // iteration. A block takes 256 * perThread entries; each lane merges runs of
 // one slot before adding them to an LDS histogram, and the block subtracts
 // its nonzero bins from live[], so device atomics stay near one per block.
+// MAP: the slot of pool entry p is slotMap[p >> 8] (a byte per 256 pool
+// entries, L2-resident) instead of the flags word of its scattered state.
+template <bool MAP>
 __global__ __launch_bounds__(256) void k_slot_finish(const uint32_t* __restrict__ finished, uint32_t maxItems,
                                                      const uint32_t* __restrict__ countDev,
-                                                     const uint4* __restrict__ st, uint32_t perThread,
+                                                     const uint4* __restrict__ st,
+                                                     const uint8_t* __restrict__ slotMap, uint32_t perThread,
                                                      uint32_t* live) {
     __shared__ uint32_t hist[kMaxSlots];
     const uint32_t t = threadIdx.x;
@@ -640,7 +644,11 @@ __global__ __launch_bounds__(256) void k_slot_finish(const uint32_t* __restrict_
         }
         uint32_t sl[kBatch];
 #pragma unroll
-        for (uint32_t u = 0; u < kBatch; ++u) sl[u] = p[u] != 0xFFFFFFFFu ? (st[p[u]].z >> FLAG_SLOT_SHIFT) : 0xFFFFFFFFu;
+        for (uint32_t u = 0; u < kBatch; ++u) {
+            if (p[u] == 0xFFFFFFFFu) sl[u] = 0xFFFFFFFFu;
+            else if constexpr (MAP) sl[u] = slotMap[p[u] >> 8];
+            else sl[u] = st[p[u]].z >> FLAG_SLOT_SHIFT;
+        }
 #pragma unroll
         for (uint32_t u = 0; u < kBatch; ++u) {
             if (sl[u] == 0xFFFFFFFFu) continue;
@@ -759,8 +767,22 @@ extern "C" int wrk_slot_finish(const uint32_t* finished, uint32_t maxItems, cons
     uint32_t perThread = ceil_div(maxItems, 256ull * 2048ull);
     perThread = perThread < 1u ? 1u : (perThread > 256u ? 256u : perThread);
     uint32_t blocks = ceil_div(maxItems, 256ull * perThread);
-    hipLaunchKernelGGL(k_slot_finish, dim3(blocks), dim3(256), 0, as_stream(stream), finished, maxItems, countDev,
-                       (const uint4*)paths->state, perThread, live);
+    hipLaunchKernelGGL(k_slot_finish<false>, dim3(blocks), dim3(256), 0, as_stream(stream), finished, maxItems,
+                       countDev, (const uint4*)paths->state, nullptr, perThread, live);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_slot_finish_map(const uint32_t* finished, uint32_t maxItems, const uint32_t* countDev,
+                                   const uint8_t* slotMap, uint32_t* live, void* stream) {
+    WRK_REQUIRE(slotMap && live, "wrk_slot_finish_map: null argument");
+    if (maxItems == 0) return WRK_OK;
+    WRK_REQUIRE(finished, "wrk_slot_finish_map: null argument");
+    uint32_t perThread = ceil_div(maxItems, 256ull * 2048ull);
+    perThread = perThread < 1u ? 1u : (perThread > 256u ? 256u : perThread);
+    uint32_t blocks = ceil_div(maxItems, 256ull * perThread);
+    hipLaunchKernelGGL(k_slot_finish<true>, dim3(blocks), dim3(256), 0, as_stream(stream), finished, maxItems,
+                       countDev, nullptr, slotMap, perThread, live);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }

@@ -81,3 +81,26 @@ def test_slot_finish_matches_histogram(gpu, n):
     got = live.cpu().numpy().view(np.uint32)
     want = live0 - np.bincount(slot_of[finished[:m]], minlength=256).astype(np.uint32)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n", [0, 1, 1000, 3000017])
+def test_slot_finish_map_matches_histogram(gpu, n):
+    """wrk_slot_finish_map: the slot of pool entry p is slotMap[p >> 8]."""
+    rng = np.random.default_rng(4321 + n)
+    granules = max((2 * n + 255) // 256, 1)
+    slot_map = rng.integers(0, 256, size=granules).astype(np.uint8)
+    slot_map[: granules // 2] = np.sort(slot_map[: granules // 2])
+    finished = rng.permutation(granules * 256)[:n].astype(np.uint32)
+    finished[: n // 2] = np.sort(finished[: n // 2])
+    m = max(n - 5, 0)  # device-side count shorter than the launch size
+    live0 = rng.integers(1 << 20, 1 << 24, size=256).astype(np.uint32)
+    q = G.dev(finished if n else np.zeros(1, np.uint32))
+    cnt = G.dev(np.array([m], np.uint32))
+    live = G.dev(live0)
+    smap = G.dev(slot_map)
+    G.check(wr.lib().wrk_slot_finish_map(G.vp(q), n, G.vp(cnt), G.vp(smap), G.vp(live), G.stream()))
+    torch.cuda.synchronize()
+    got = live.cpu().numpy().view(np.uint32)
+    want = live0 - np.bincount(slot_map[finished[:m] >> 8], minlength=256).astype(np.uint32)
+    assert np.array_equal(got, want)
+
