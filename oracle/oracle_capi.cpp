@@ -278,6 +278,30 @@ void orc_aces(const float* img, uint32_t npix, uint8_t* out) {
     }
 }
 
+// --- single-step hooks (see wr_oracle.hpp)
+void orc_shade_event(void* h, const orc_params_in* pi, EventIO* e) {
+    Params p{pi->width, pi->height, pi->useNEE, pi->maxBounces, pi->rrStartDepth, pi->fireflyClamp, pi->iterationCap};
+    shade_event(static_cast<OScene*>(h)->s, p, e);
+}
+void orc_hit_attributes(void* h, const float* o, const float* d, float t, uint32_t prim, float* out11) {
+    hit_attributes(static_cast<OScene*>(h)->s, o, d, t, prim, out11);
+}
+void orc_camera_ray(const orc_camera_in* ci, const orc_params_in* pi, uint32_t pixel, uint32_t sample,
+                    float* origin, float* dir) {
+    Camera cam = make_camera(ci->position, ci->target, ci->up, ci->fovY, ci->aspect, ci->focalDistance, ci->lensRadius);
+    Params p{pi->width, pi->height, pi->useNEE, pi->maxBounces, pi->rrStartDepth, pi->fireflyClamp, pi->iterationCap};
+    camera_ray(cam, p, pixel, sample, origin, dir);
+}
+void orc_sample_texture(void* h, uint32_t texId, float u, float v, float* out) {
+    sample_texture(static_cast<OScene*>(h)->s, texId, u, v, out);
+}
+int orc_effective_normal(void* h, uint32_t mat, const float* n, float u, float v, float* out) {
+    const Scene& s = static_cast<OScene*>(h)->s;
+    if (mat >= s.materials.size()) return -1;
+    effective_normal(s, mat, n, u, v, out);
+    return 0;
+}
+
 double orc_wall_seconds(void) {
     using namespace std::chrono;
     return duration<double>(steady_clock::now().time_since_epoch()).count();

@@ -39,6 +39,19 @@ class orc_params_in(C.Structure):
                 ("rrStartDepth", C.c_float), ("fireflyClamp", C.c_float), ("iterationCap", C.c_uint32)]
 
 
+class EventIO(C.Structure):
+    """orc::EventIO (wr_oracle.hpp): one shade() visit's ray state and hit."""
+    _fields_ = [("origin", C.c_float * 3), ("direction", C.c_float * 3), ("throughput", C.c_float * 3),
+                ("radiance", C.c_float * 3), ("tMin", C.c_float), ("tMax", C.c_float), ("prevPdf", C.c_float),
+                ("seed", C.c_uint32), ("rngDimension", C.c_uint32), ("depth", C.c_uint32), ("stage", C.c_uint32),
+                ("neeDone", C.c_uint32), ("prevWasDelta", C.c_uint32), ("shadowContribution", C.c_float * 3),
+                ("savedDirection", C.c_float * 3), ("position", C.c_float * 3), ("normal", C.c_float * 3),
+                ("geometricNormal", C.c_float * 3), ("u", C.c_float), ("v", C.c_float), ("materialId", C.c_uint32),
+                ("flushed", C.c_uint32), ("flushL", C.c_float * 3)]
+
+
+STAGE_TRACE, STAGE_SHADE, STAGE_SHADOW, STAGE_TERMINATED = range(4)
+
 _P = C.c_void_p
 
 
@@ -91,6 +104,16 @@ def lib():
             "orc_frame": (None, [_P, _P, _P]),
             "orc_power_heuristic": (C.c_float, [C.c_float, C.c_float]),
             "orc_cdf_search": (C.c_uint32, [_P, C.c_uint32, C.c_float]),
+            "orc_shade_event": (None, [_P, C.POINTER(orc_params_in), C.POINTER(EventIO)]),
+            "orc_hit_attributes": (None, [_P, _P, _P, C.c_float, C.c_uint32, _P]),
+            "orc_camera_ray": (None, [C.POINTER(orc_camera_in), C.POINTER(orc_params_in), C.c_uint32, C.c_uint32,
+                                      _P, _P]),
+            "orc_sample_texture": (None, [_P, C.c_uint32, C.c_float, C.c_float, _P]),
+            "orc_effective_normal": (C.c_int, [_P, C.c_uint32, _P, C.c_float, C.c_float, _P]),
+            "orc_sample_triangle": (None, [C.c_float, C.c_float, _P, _P, _P, _P, _P, _P]),
+            "orc_offset_origin": (None, [_P, _P, _P, _P]),
+            "orc_asym_correction": (C.c_float, [_P, _P, _P]),
+            "orc_intersect_triangle": (C.c_int, [_P, _P, _P, _P, _P, C.c_float, C.c_float, _P]),
             "orc_wall_seconds": (C.c_double, []),
             "orc_max_threads": (C.c_int, []),
         }
@@ -278,3 +301,86 @@ def aces(img):
     out = np.zeros(a.shape, dtype=np.uint8)
     lib().orc_aces(_p(a), a.size // 3, _p(out))
     return out
+
+
+# ------------------------------------------------------- single-step hooks
+def _f32(v, n=3):
+    a = np.ascontiguousarray(v, dtype=np.float32).ravel()
+    assert a.size == n
+    return a
+
+
+def shade_event(scene: OracleScene, prm: orc_params_in, *, direction, position, normal, geometric_normal=None,
+                material=0, seed=0, dim=0, depth=0, throughput=(1, 1, 1), radiance=(0, 0, 0), nee_done=0,
+                prev_was_delta=1, prev_pdf=1.0, u=0.0, v=0.0) -> EventIO:
+    """Run one shade() visit (shade.cu:24-85) and return the resulting state."""
+    e = EventIO()
+    e.direction[:] = [float(x) for x in direction]
+    e.position[:] = [float(x) for x in position]
+    e.origin[:] = [float(x) for x in position]
+    e.normal[:] = [float(x) for x in normal]
+    e.geometricNormal[:] = [float(x) for x in (normal if geometric_normal is None else geometric_normal)]
+    e.throughput[:] = [float(x) for x in throughput]
+    e.radiance[:] = [float(x) for x in radiance]
+    e.tMin, e.tMax, e.prevPdf = 0.0, 1e20, float(prev_pdf)
+    e.seed, e.rngDimension, e.depth = int(seed), int(dim), int(depth)
+    e.stage, e.neeDone, e.prevWasDelta = STAGE_SHADE, int(nee_done), int(prev_was_delta)
+    e.u, e.v, e.materialId = float(u), float(v), int(material)
+    lib().orc_shade_event(scene.h, C.byref(prm), C.byref(e))
+    return e
+
+
+def hit_attributes(scene: OracleScene, o, d, t, prim):
+    """(position, shading normal, geometric normal, u, v) of trace.cu:167-264."""
+    out = np.zeros(11, np.float32)
+    o, d = _f32(o), _f32(d)
+    lib().orc_hit_attributes(scene.h, _p(o), _p(d), float(t), int(prim), _p(out))
+    return out[0:3], out[3:6], out[6:9], float(out[9]), float(out[10])
+
+
+def camera_ray(cam: dict, prm: orc_params_in, pixel, sample):
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    ci = camera_in(cam)
+    lib().orc_camera_ray(C.byref(ci), C.byref(prm), int(pixel), int(sample), _p(o), _p(d))
+    return o, d
+
+
+def sample_texture(scene: OracleScene, tex_id, u, v):
+    out = np.zeros(3, np.float32)
+    lib().orc_sample_texture(scene.h, int(tex_id), float(u), float(v), _p(out))
+    return out
+
+
+def effective_normal(scene: OracleScene, mat, n, u=0.0, v=0.0):
+    out = np.zeros(3, np.float32)
+    n = _f32(n)
+    if lib().orc_effective_normal(scene.h, int(mat), _p(n), float(u), float(v), _p(out)) != 0:
+        raise ValueError("material id out of range")
+    return out
+
+
+def sample_triangle(u1, u2, v0, v1, v2):
+    pt, nrm, area = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(1, np.float32)
+    a, b, c = _f32(v0), _f32(v1), _f32(v2)
+    lib().orc_sample_triangle(float(u1), float(u2), _p(a), _p(b), _p(c), _p(pt), _p(nrm), _p(area))
+    return pt, nrm, float(area[0])
+
+
+def offset_origin(pos, n, d):
+    out = np.zeros(3, np.float32)
+    a, b, c = _f32(pos), _f32(n), _f32(d)
+    lib().orc_offset_origin(_p(a), _p(b), _p(c), _p(out))
+    return out
+
+
+def asym_correction(wi, sn, gn):
+    a, b, c = _f32(wi), _f32(sn), _f32(gn)
+    return float(lib().orc_asym_correction(_p(a), _p(b), _p(c)))
+
+
+def intersect_triangle(o, d, v0, v1, v2, tmin=0.0, tmax=1e20):
+    t = np.zeros(1, np.float32)
+    args = [_f32(x) for x in (o, d, v0, v1, v2)]
+    hit = lib().orc_intersect_triangle(*[_p(a) for a in args], float(tmin), float(tmax), _p(t))
+    return bool(hit), float(t[0])

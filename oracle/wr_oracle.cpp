@@ -358,7 +358,7 @@ struct Tracer {
         return mk<T>(x, y, z);
     }
     // sampling.cuh:85-110
-    V3<T> sampleTriangle(T u1, T u2, V3<T> v0, V3<T> v1, V3<T> v2, V3<T>& normal, T& area) const {
+    static V3<T> sampleTriangle(T u1, T u2, V3<T> v0, V3<T> v1, V3<T> v2, V3<T>& normal, T& area) {
         T su = sqrtT(u1);
         T b0 = T(1) - su;
         T b1 = su * (T(1) - u2);
@@ -917,6 +917,80 @@ void first_hit_guides(const Scene& s, const Camera& cam, const Params& p, uint32
 }
 template void trace_path<double>(const Scene&, const Camera&, const Params&, uint32_t, uint32_t, bool, PathOut*, double*);
 
+// ------------------------------------------------ single-step test hooks
+static V3<float> v3(const float* a) { return mk<float>(a[0], a[1], a[2]); }
+static void put(float* o, V3<float> v) { o[0] = v.x; o[1] = v.y; o[2] = v.z; }
+
+void shade_event(const Scene& s, const Params& p, EventIO* e) {
+    static const Camera noCam{};
+    Tracer<float> tr(s, noCam, p, false, 0);
+    Ray<float> ray;
+    std::memset(&ray, 0, sizeof(ray));
+    ray.origin = v3(e->origin); ray.direction = v3(e->direction);
+    ray.throughput = v3(e->throughput); ray.radiance = v3(e->radiance);
+    ray.tMin = e->tMin; ray.tMax = e->tMax; ray.prevPdf = e->prevPdf;
+    ray.seed = e->seed; ray.rngDimension = e->rngDimension; ray.depth = e->depth;
+    ray.stage = Shade; ray.neeDone = e->neeDone; ray.prevWasDelta = e->prevWasDelta != 0;
+    ray.shadowContribution = v3(e->shadowContribution); ray.savedDirection = v3(e->savedDirection);
+    HitInfo<float> hit;
+    std::memset(&hit, 0, sizeof(hit));
+    hit.position = v3(e->position); hit.normal = v3(e->normal); hit.geometricNormal = v3(e->geometricNormal);
+    hit.u = e->u; hit.v = e->v; hit.materialId = e->materialId;
+    tr.shade(ray, hit);
+    put(e->origin, ray.origin); put(e->direction, ray.direction);
+    put(e->throughput, ray.throughput); put(e->radiance, ray.radiance);
+    e->tMin = ray.tMin; e->tMax = ray.tMax; e->prevPdf = ray.prevPdf;
+    e->seed = ray.seed; e->rngDimension = ray.rngDimension; e->depth = ray.depth; e->stage = ray.stage;
+    e->neeDone = ray.neeDone; e->prevWasDelta = ray.prevWasDelta ? 1u : 0u;
+    put(e->shadowContribution, ray.shadowContribution); put(e->savedDirection, ray.savedDirection);
+    e->flushed = tr.flushed ? 1u : 0u;
+    put(e->flushL, tr.flushed ? tr.flushL : mk<float>(0.0f, 0.0f, 0.0f));
+}
+
+void hit_attributes(const Scene& s, const float o[3], const float d[3], float t, uint32_t prim, float out[11]) {
+    static const Camera noCam{};
+    static const Params noParams{1, 1, 0, 1, 0.0f, 0.0f, 0};
+    Tracer<float> tr(s, noCam, noParams, false, 0);
+    Ray<float> ray;
+    std::memset(&ray, 0, sizeof(ray));
+    ray.origin = v3(o); ray.direction = v3(d);
+    HitInfo<float> hit;
+    std::memset(&hit, 0, sizeof(hit));
+    tr.closestHit(ray, hit, t, prim);
+    put(out, hit.position); put(out + 3, hit.normal); put(out + 6, hit.geometricNormal);
+    out[9] = hit.u; out[10] = hit.v;
+}
+
+void camera_ray(const Camera& cam, const Params& p, uint32_t pixel, uint32_t sample, float origin[3],
+                float dir[3]) {
+    static const Scene noScene{};
+    Tracer<float> tr(noScene, cam, p, false, sample);
+    Ray<float> ray;
+    std::memset(&ray, 0, sizeof(ray));
+    tr.cameraRay(ray, pixel);
+    put(origin, ray.origin); put(dir, ray.direction);
+}
+
+void sample_texture(const Scene& s, uint32_t texId, float u, float v, float out[3]) {
+    static const Camera noCam{};
+    static const Params noParams{1, 1, 0, 1, 0.0f, 0.0f, 0};
+    Tracer<float> tr(s, noCam, noParams, false, 0);
+    wrk_material m{};
+    m.albedo[0] = m.albedo[1] = m.albedo[2] = -1.0f;  // marks "constant albedo taken"
+    m.albedoTextureId = texId;
+    put(out, tr.materialAlbedo(m, u, v));
+}
+
+void effective_normal(const Scene& s, uint32_t mat, const float n[3], float u, float v, float out[3]) {
+    static const Camera noCam{};
+    static const Params noParams{1, 1, 0, 1, 0.0f, 0.0f, 0};
+    Tracer<float> tr(s, noCam, noParams, false, 0);
+    HitInfo<float> hit;
+    std::memset(&hit, 0, sizeof(hit));
+    hit.normal = v3(n); hit.u = u; hit.v = v;
+    put(out, tr.effectiveNormal(s.materials[mat], hit));
+}
+
 }  // namespace orc
 
 // ------------------------------------------------ math exports for tests
@@ -958,4 +1032,36 @@ void orc_frame(const float* n, float* t, float* b) {
 }
 float orc_power_heuristic(float a, float b) { return TF::powerHeuristic(a, b); }
 uint32_t orc_cdf_search(const float* cdf, uint32_t n, float u) { return orc::cdf_search<float>(cdf, n, u); }
+}  // extern "C"
+
+// ------------------------------------------- more single-function exports
+extern "C" {
+using orc::V3;
+// sampling.cuh:85-110: point (3), normal (3), area (1)
+void orc_sample_triangle(float u1, float u2, const float* v0, const float* v1, const float* v2, float* pt,
+                         float* normal, float* area) {
+    V3<float> n;
+    V3<float> p = orc::Tracer<float>::sampleTriangle(u1, u2, orc::mk(v0[0], v0[1], v0[2]),
+                                                      orc::mk(v1[0], v1[1], v1[2]),
+                                                      orc::mk(v2[0], v2[1], v2[2]), n, *area);
+    pt[0] = p.x; pt[1] = p.y; pt[2] = p.z;
+    normal[0] = n.x; normal[1] = n.y; normal[2] = n.z;
+}
+// utils/ray_offset.cuh:9-21
+void orc_offset_origin(const float* pos, const float* n, const float* dir, float* out) {
+    V3<float> o = orc::Tracer<float>::offsetOrigin(orc::mk(pos[0], pos[1], pos[2]), orc::mk(n[0], n[1], n[2]),
+                                                   orc::mk(dir[0], dir[1], dir[2]));
+    out[0] = o.x; out[1] = o.y; out[2] = o.z;
+}
+// material_common.cuh:14-29
+float orc_asym_correction(const float* wi, const float* sn, const float* gn) {
+    return orc::Tracer<float>::asymCorrection(orc::mk(wi[0], wi[1], wi[2]), orc::mk(sn[0], sn[1], sn[2]),
+                                              orc::mk(gn[0], gn[1], gn[2]));
+}
+// Watertight test of one triangle; returns 1 and t on a hit in [tmin, tmax].
+int orc_intersect_triangle(const float* o, const float* d, const float* v0, const float* v1, const float* v2,
+                           float tmin, float tmax, float* t) {
+    orc::RayPre r = orc::ray_pre(o, d);
+    return orc::intersect_tri(r, v0, v1, v2, tmin, tmax, t) ? 1 : 0;
+}
 }  // extern "C"

@@ -265,4 +265,35 @@ void trace_path(const Scene& s, const Camera& cam, const Params& p, uint32_t pix
 void first_hit_guides(const Scene& s, const Camera& cam, const Params& p, uint32_t pixel, uint32_t sample,
                       bool useBvh, float albedo[3], float normal[3]);
 
+// ------------------------------------------------ single-step test hooks
+// These run one piece of the FP32 restatement on explicit inputs so that the
+// tests can pin each piece against closed forms and invariants.
+
+// One shade() visit (shade.cu:24-85) on an explicit ray state and hit record.
+// Field meanings follow RayState / HitInfo (gpu_types.h:48-86); `flushed`
+// and `flushL` report a terminateRay() flush (material_common.cuh:33-37).
+struct EventIO {
+    float origin[3], direction[3], throughput[3], radiance[3];
+    float tMin, tMax, prevPdf;
+    uint32_t seed, rngDimension, depth, stage, neeDone, prevWasDelta;
+    float shadowContribution[3], savedDirection[3];
+    float position[3], normal[3], geometricNormal[3], u, v;
+    uint32_t materialId;
+    uint32_t flushed;
+    float flushL[3];
+};
+void shade_event(const Scene& s, const Params& p, EventIO* e);
+// Closest-hit attribute reconstruction (trace.cu:167-264) for the ray (o, d)
+// at distance t on triangle prim: out = position[3], shading normal[3],
+// geometric normal[3] (both facing the ray), u, v.
+void hit_attributes(const Scene& s, const float o[3], const float d[3], float t, uint32_t prim, float out[11]);
+// Camera ray of (pixel, sample) (trace.cu:53-125).
+void camera_ray(const Camera& cam, const Params& p, uint32_t pixel, uint32_t sample, float origin[3],
+                float dir[3]);
+// Albedo texture fetch of texture texId (1-based) at (u, v) (materials.cuh:9-16).
+void sample_texture(const Scene& s, uint32_t texId, float u, float v, float out[3]);
+// getEffectiveNormal (normal_map.cuh:36-60) of material mat for shading
+// normal n at (u, v).
+void effective_normal(const Scene& s, uint32_t mat, const float n[3], float u, float v, float out[3]);
+
 }  // namespace orc
