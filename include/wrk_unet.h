@@ -51,8 +51,9 @@ int wrk_up2_bwd(const uint16_t* dy, uint32_t dyPitch, uint32_t c, uint16_t* dx, 
  * a per-image feature (channel 9: feature[image] for image < numFeature,
  * else 0; the network's is log2(spp) / 16), zeros, from FP32
  * (img, srcH, srcW, 3) layers; origins (device, 3 per crop: image, x0, y0)
- * select n crops of h x w, or NULL for whole images. feature is a device
- * array or NULL. */
+ * select n crops of h x w, or NULL for whole images, in which case h >= srcH
+ * and w >= srcW: positions past the last source row or column repeat it
+ * (edge padding). feature is a device array or NULL. */
 int wrk_unet_pack(const float* color, const float* albedo, const float* normal, uint32_t srcH, uint32_t srcW,
                   const uint32_t* origins, const float* feature, uint32_t numFeature, uint16_t* out, uint32_t n,
                   uint32_t h, uint32_t w, void* stream);
@@ -95,7 +96,8 @@ int wrk_unet_get_grads(void* net, float* hostOut);
  * next forward / training calls see: input channel 9 = log2(spp) / 16 of
  * each pixel's image (0 for images beyond count, or before any call). */
 int wrk_unet_set_sample_counts(void* net, const uint32_t* spp, uint32_t count);
-/* denoised colour (n, h, w, 3) FP32 into out; h, w multiples of 8 */
+/* denoised colour (n, h, w, 3) FP32 into out; h, w multiples of 8. Without
+ * origins, h x w may exceed srcH x srcW (edge padding, see wrk_unet_pack). */
 int wrk_unet_forward(void* net, const float* color, const float* albedo, const float* normal, uint32_t srcH,
                      uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w, float* out,
                      void* stream);

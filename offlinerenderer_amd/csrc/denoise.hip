@@ -357,6 +357,8 @@ __global__ __launch_bounds__(256) void k_up2_bwd(const bf16* __restrict__ dy, ui
 // 9 guide channels into a 32-channel bf16 tensor: log(1 + max(colour, 0)),
 // albedo, normal, zeros. Inputs are FP32 (n, H, W, 3), rows of srcW pixels
 // starting at (sx0, sy0) of an image of srcH x srcW (crops for training).
+// Positions past the source's last row or column repeat it (edge padding of
+// images whose sides are not multiples of 8).
 __global__ __launch_bounds__(256) void k_unet_pack(const float* __restrict__ color, const float* __restrict__ albedo,
                                                    const float* __restrict__ normal, uint32_t srcH, uint32_t srcW,
                                                    const uint32_t* __restrict__ origins,
@@ -371,6 +373,8 @@ __global__ __launch_bounds__(256) void k_unet_pack(const float* __restrict__ col
         sx += origins[3 * n + 1];
         sy += origins[3 * n + 2];
     }
+    sx = min(sx, srcW - 1u);
+    sy = min(sy, srcH - 1u);
     const uint64_t s = ((uint64_t)img * srcH + sy) * srcW + sx;
     float v[32];
 #pragma unroll
@@ -641,7 +645,8 @@ extern "C" int wrk_unet_pack(const float* color, const float* albedo, const floa
                              uint32_t srcW, const uint32_t* origins, const float* feature, uint32_t numFeature,
                              uint16_t* out, uint32_t n, uint32_t h, uint32_t w, void* stream) {
     WRK_REQUIRE(color && albedo && normal && out, "wrk_unet_pack: null argument");
-    WRK_REQUIRE(origins || (srcH == h && srcW == w), "wrk_unet_pack: crops need origins");
+    WRK_REQUIRE(origins || (h >= srcH && w >= srcW), "wrk_unet_pack: crops need origins");
+    WRK_REQUIRE(srcH > 0 && srcW > 0, "wrk_unet_pack: empty source");
     const uint64_t np = (uint64_t)n * h * w;
     if (np == 0) return WRK_OK;
     hipLaunchKernelGGL(k_unet_pack, dim3(ceil_div(np, 256)), dim3(256), 0, as_stream(stream), color, albedo, normal,

@@ -439,18 +439,17 @@ struct RendererImpl {
         const uint64_t total = (uint64_t)rp.width * rp.height * rp.spp;
         detail::Engine& e = engine(total);
         auto t0 = std::chrono::steady_clock::now();
-        // denoiser.cpp:77-195 takes colour + albedo + normal; the learned
-        // network needs image sides divisible by 8
+        // denoiser.cpp:77-195 takes colour + albedo + normal of any size;
+        // the learned network runs on sides padded up to multiples of 8
         if (rp.denoiser.enabled && !unet && !triedDefault) {
             triedDefault = true;
             const std::string path = default_denoiser_path();
             if (!path.empty() && std::ifstream(path).good()) load_denoiser(path);
         }
-        const bool denoise = rp.denoiser.enabled && unet && rp.width % 8 == 0 && rp.height % 8 == 0;
-        if (rp.denoiser.enabled && !denoise && g_logLevel >= 2)
-            std::cout << "[Renderer] denoiser requested but no network is loaded or the image sides are not "
-                         "multiples of 8: returning the averaged image"
-                      << std::endl;
+        if (rp.denoiser.enabled && !unet)
+            throw std::runtime_error("render: denoiser requested but no denoiser network is loaded "
+                                     "(Renderer::loadDenoiser)");
+        const bool denoise = rp.denoiser.enabled;
         const bool guides = albedo || normal || denoise;
         const uint64_t first = (uint64_t)rp.firstSample * rp.width * rp.height;
         uint64_t id = e.submit(scene, detail::camera_record(camera), params_of(rp), rp.spp, first, total,
@@ -464,15 +463,20 @@ struct RendererImpl {
             dst = tmp.data();
         }
         if (denoise) {
-            const size_t bytes = (size_t)rp.width * rp.height * 3 * sizeof(float);
-            denoised.alloc(bytes);
+            // sides padded to multiples of 8 by repeating the last row and
+            // column; the caller gets the top-left width x height crop
+            const uint32_t pw = (rp.width + 7u) & ~7u, ph = (rp.height + 7u) & ~7u;
+            const size_t rowBytes = (size_t)rp.width * 3 * sizeof(float);
+            denoised.alloc((size_t)pw * ph * 3 * sizeof(float));
             const uint32_t spp = rp.spp;  // the network is told how noisy its input is
             if (wrk_unet_set_sample_counts(unet, &spp, 1) != WRK_OK ||
                 wrk_unet_forward(unet, e.deviceImage(id), e.deviceAlbedo(id), e.deviceNormal(id), rp.height,
-                                 rp.width, nullptr, 1, rp.height, rp.width, denoised.as<float>(), e.stream()) != WRK_OK)
+                                 rp.width, nullptr, 1, ph, pw, denoised.as<float>(), e.stream()) != WRK_OK)
                 throw std::runtime_error(std::string("denoiser: ") + wrk_unet_last_error());
             hip_check(hipStreamSynchronize(e.stream()), "hipStreamSynchronize");
-            hip_check(hipMemcpy(dst, denoised.ptr, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+            hip_check(hipMemcpy2D(dst, rowBytes, denoised.ptr, (size_t)pw * 3 * sizeof(float), rowBytes, rp.height,
+                                  hipMemcpyDeviceToHost),
+                      "hipMemcpy2D D2H");
             e.fetch(id, nullptr, albedo, normal);
         } else {
             e.fetch(id, dst, albedo, normal);

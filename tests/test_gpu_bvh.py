@@ -159,3 +159,69 @@ def test_bvh_degenerate_inputs_build_fast(gpu):
         ps = G.product_scene(d)
         assert ps.getBuildSeconds() < 1.0
         check_bvh(d)
+
+
+def check_bvh_fast(desc, ps):
+    """The invariants of check_bvh, vectorised over nodes (for 10M-triangle
+    builds): primitive ids a permutation and records equal to the input
+    triangles; every node but the root referenced exactly once; leaves cover
+    each triangle exactly once; every entry's decoded box holds the exact
+    bounds of all triangles beneath it; the depth is as reported."""
+    b = ps.getBVH()
+    lo, hi, child = decode_nodes(b)
+    N, width = child.shape
+    T = b.numTris
+    tris = d2h(b.tris, T * 48).view(np.float32).reshape(T, 3, 4)
+    prim = tris[:, 0, 3].view(np.uint32)
+    V, F, _ = flatten(desc)
+    assert np.array_equal(np.sort(prim), np.arange(T, dtype=np.uint32))
+    assert np.array_equal(tris[:, :, :3], V[F[prim]])
+    used = child != 0xFFFFFFFF
+    leaf = used & ((child & 0x80000000) != 0)
+    inner = used & ~leaf
+    refs = np.bincount(child[inner].astype(np.int64), minlength=N)
+    assert refs[0] == 0 and np.all(refs[1:] == 1)
+    codes = child[leaf]
+    first = (codes & 0x07FFFFFF).astype(np.int64)
+    count = (((codes >> 27) & 0xF) + 1).astype(np.int64)
+    cov = np.zeros(T + 1, np.int64)
+    np.add.at(cov, first, 1)
+    np.add.at(cov, first + count, -1)
+    assert np.all(np.cumsum(cov)[:T] == 1)
+    tlo, thi = tris[:, :, :3].min(1), tris[:, :, :3].max(1)
+    elo = np.full((codes.size, 3), np.inf, np.float32)
+    ehi = np.full((codes.size, 3), -np.inf, np.float32)
+    for j in range(int(count.max())):
+        m = j < count
+        elo[m] = np.minimum(elo[m], tlo[first[m] + j])
+        ehi[m] = np.maximum(ehi[m], thi[first[m] + j])
+    assert np.all(elo >= lo[leaf]) and np.all(ehi <= hi[leaf])
+    # breadth-first levels, then exact subtree bounds bottom-up
+    levels, frontier = [], np.array([0], np.int64)
+    while frontier.size:
+        levels.append(frontier)
+        frontier = child[frontier][inner[frontier]].astype(np.int64)
+    assert sum(l.size for l in levels) == N
+    assert len(levels) == b.maxDepth
+    sub_lo = np.full((N, 3), np.inf, np.float32)
+    sub_hi = np.full((N, 3), -np.inf, np.float32)
+    rows = np.nonzero(leaf)[0]
+    np.minimum.at(sub_lo, rows, elo)
+    np.maximum.at(sub_hi, rows, ehi)
+    for lvl in reversed(levels):
+        r, c = np.nonzero(inner[lvl])
+        parent = lvl[r]
+        kid = child[parent, c].astype(np.int64)
+        assert np.all(sub_lo[kid] >= lo[parent, c]) and np.all(sub_hi[kid] <= hi[parent, c])
+        np.minimum.at(sub_lo, parent, sub_lo[kid])
+        np.maximum.at(sub_hi, parent, sub_hi[kid])
+    assert np.all(sub_lo[0] == tlo.min(0)) and np.all(sub_hi[0] == thi.max(0))
+    return b
+
+
+def test_fast_checker_agrees_with_reference_checker(gpu):
+    d = scenes.large_scene(8, 8, 1, num_spheres=60, num_emitters=4)
+    ps = G.product_scene(d)
+    b1 = check_bvh(d)
+    b2 = check_bvh_fast(d, ps)
+    assert (b1.numNodes, b1.maxDepth) == (b2.numNodes, b2.maxDepth)

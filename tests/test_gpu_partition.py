@@ -104,3 +104,43 @@ def test_slot_finish_map_matches_histogram(gpu, n):
     want = live0 - np.bincount(slot_map[finished[:m] >> 8], minlength=256).astype(np.uint32)
     assert np.array_equal(got, want)
 
+
+
+@pytest.mark.parametrize("K", [6, 13])
+@pytest.mark.parametrize("off", [0, 1, 2, 3])
+@pytest.mark.parametrize("n", [1023, 1027, 5000, 777_777])
+def test_partition_production_widths_at_queue_offsets(gpu, K, off, n):
+    """The engine partitions each scene's part of the queue with K = 6 from
+    `category + a`, `queue + a` (any a), and the whole queue with K = 13.
+    A category pointer that is not 4-byte aligned takes the bytewise kernels;
+    an aligned one the 4-per-lane kernels. Both must give numpy's stable
+    partition, with n around tile (1024) and word edges and over several
+    scan segments."""
+    rng = np.random.default_rng(K * 7919 + off * 131 + n)
+    cat = rng.integers(0, K + 1, size=n).astype(np.uint8)   # category K (and above) is dropped
+    cat[rng.uniform(size=n) < 0.3] = K - 1                    # a heavy last category, like "finished"
+    items = rng.permutation(n).astype(np.uint32)
+    L = wr.lib()
+    # buffers padded on both sides: the kernels see only [off, off + n)
+    cat_big = np.full(n + 8, 0xEE, np.uint8)
+    cat_big[off:off + n] = cat
+    items_big = np.full(n + 8, 0xDEADBEEF, np.uint32)
+    items_big[off:off + n] = items
+    cat_d, items_d = G.dev(cat_big), G.dev(items_big)
+    out_d = torch.full((n + 8,), -1, dtype=torch.int32, device="cuda")
+    offs = torch.zeros(K + 1, dtype=torch.int32, device="cuda")
+    sb = L.wrk_partition_scratch_bytes(n, K)
+    scratch = torch.empty(sb, dtype=torch.uint8, device="cuda")
+    import ctypes as C
+
+    G.check(L.wrk_partition(C.c_void_p(cat_d.data_ptr() + off), C.c_void_p(items_d.data_ptr() + 4 * off), n, K,
+                            C.c_void_p(out_d.data_ptr() + 4 * off), G.vp(offs), G.vp(scratch), sb, G.stream()))
+    torch.cuda.synchronize()
+    o = offs.cpu().numpy()
+    got = out_d.cpu().numpy().view(np.uint32)
+    expect_offs = np.concatenate([[0], np.cumsum([np.count_nonzero(cat == k) for k in range(K)])])
+    assert np.array_equal(o, expect_offs)
+    for k in range(K):
+        assert np.array_equal(got[off + o[k]:off + o[k + 1]], items[cat == k]), k
+    # nothing written outside the partitioned range
+    assert np.all(got[:off] == 0xFFFFFFFF) and np.all(got[off + o[K]:] == 0xFFFFFFFF)
