@@ -226,6 +226,19 @@ int wrk_compact(const uint8_t* keep, const uint32_t* in, uint32_t n, uint32_t* o
 int wrk_partition(const uint8_t* category, const uint32_t* in, uint32_t n, uint32_t numCategories,
                   uint32_t* out, uint32_t* offsetsOut, void* scratch, uint64_t scratchBytes,
                   void* stream);
+/* wrk_partition without a scratch argument, allocating nothing: no
+ * counterpart in libWR. For callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops that must not call hipMalloc,
+ * harnesses that forbid it). The count matrix lives in a 4,194,816-byte
+ * device array of the library, one per GPU, held until the process ends;
+ * n may be at most wrk_partition_noalloc_max_items() = 2^26 (larger n is
+ * rejected with WRK_ERR_INVALID) and numCategories at most 16. Results equal
+ * wrk_partition's. Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (two streams, two threads) race and must be serialised
+ * by the caller. */
+int wrk_partition_noalloc(const uint8_t* category, const uint32_t* in, uint32_t n, uint32_t numCategories,
+                          uint32_t* out, uint32_t* offsetsOut, void* stream);
+uint32_t wrk_partition_noalloc_max_items(void);
 
 /* ---------------------------------------------------------- wavefront --- */
 /* Path state flags (uint4 state.z): depth bits 0..15, isFirstHit 16,

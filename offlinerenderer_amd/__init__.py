@@ -180,6 +180,8 @@ _SIGS = {
     "wrk_compact_scratch_bytes": (C.c_uint64, [C.c_uint32]),
     "wrk_compact": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P, C.c_uint64, _P]),
     "wrk_partition": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint64, _P]),
+    "wrk_partition_noalloc": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, _P, _P]),
+    "wrk_partition_noalloc_max_items": (C.c_uint32, []),
     "wrk_generate": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32,
                                _P]),
     "wrk_classify": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_uint32, _P]),

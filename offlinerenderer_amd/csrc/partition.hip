@@ -15,6 +15,9 @@
 // The output equals std::stable_partition / std::copy_if on the host.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
 #include "wrk_common.h"
 
 namespace wrk {
@@ -338,23 +341,16 @@ static int partition_impl(const uint8_t* category, const uint32_t* in, uint32_t 
     return WRK_OK;
 }
 
-}  // namespace wrk
-
-using namespace wrk;
-
-extern "C" uint64_t wrk_partition_scratch_bytes(uint32_t n, uint32_t numCategories) {
-    uint64_t tiles = (n + kTile - 1) / kTile;
-    uint32_t k = numCategories < 1 ? 1 : numCategories;
-    uint64_t m = tiles * k;
+// Scratch bytes of a partition of n items into k categories: the k x tiles
+// count matrix, one total per scan segment, and a small tail.
+constexpr uint64_t partition_scratch_bytes(uint64_t n, uint32_t k) {
+    const uint64_t m = (n + kTile - 1) / kTile * (k < 1 ? 1 : k);
     return (m + (m + kScanSeg - 1) / kScanSeg + 64) * 4;
 }
 
-extern "C" int wrk_partition(const uint8_t* category, const uint32_t* in, uint32_t n, uint32_t numCategories,
-                             uint32_t* out, uint32_t* offsetsOut, void* scratch, uint64_t scratchBytes,
-                             void* stream) {
-    WRK_REQUIRE(offsetsOut && scratch, "wrk_partition: null argument");
-    WRK_REQUIRE(n == 0 || (category && out), "wrk_partition: null argument");
-    hipStream_t s = as_stream(stream);
+static int partition_dispatch(const uint8_t* category, const uint32_t* in, uint32_t n, uint32_t numCategories,
+                              uint32_t* out, uint32_t* offsetsOut, void* scratch, uint64_t scratchBytes,
+                              hipStream_t s) {
     switch (numCategories) {
         case 1: return partition_impl<1>(category, in, n, out, offsetsOut, scratch, scratchBytes, s);
         case 2: return partition_impl<2>(category, in, n, out, offsetsOut, scratch, scratchBytes, s);
@@ -374,6 +370,80 @@ extern "C" int wrk_partition(const uint8_t* category, const uint32_t* in, uint32
         case 16: return partition_impl<16>(category, in, n, out, offsetsOut, scratch, scratchBytes, s);
         default: set_error("wrk_partition: numCategories must be in [1, 16]"); return WRK_ERR_INVALID;
     }
+}
+
+// ------------------------------------------------ allocation-free variant
+// The count matrix of wrk_partition_noalloc lives in this array, sized for
+// the largest queue that entry point accepts (2^26 items, 16 categories:
+// 4,194,816 bytes). Each GPU holds its own copy for the life of the process.
+// Host code never passes the array's name to a kernel (on the host it is a
+// stand-in, not a device address): it uses the per-device address from
+// hipGetSymbolAddress.
+constexpr uint32_t kNoAllocMaxItems = 1u << 26;
+constexpr uint32_t kNoAllocMaxCategories = 16;
+constexpr uint64_t kNoAllocScratchBytes =
+    (partition_scratch_bytes(kNoAllocMaxItems, kNoAllocMaxCategories) + 255) & ~uint64_t(255);
+namespace noalloc {
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+}
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_noallocMutex;
+static void* g_noallocScratch[kMaxDevices];  // device address of noalloc::scratch per device, once resolved
+
+// Device address of the array on the current device, resolved on the first
+// call made with that device current.
+static void* noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_noallocMutex);
+    if (!g_noallocScratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc::scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("wrk_partition_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        g_noallocScratch[device] = p;
+    }
+    return g_noallocScratch[device];
+}
+
+}  // namespace wrk
+
+using namespace wrk;
+
+extern "C" uint64_t wrk_partition_scratch_bytes(uint32_t n, uint32_t numCategories) {
+    return partition_scratch_bytes(n, numCategories);
+}
+
+extern "C" int wrk_partition(const uint8_t* category, const uint32_t* in, uint32_t n, uint32_t numCategories,
+                             uint32_t* out, uint32_t* offsetsOut, void* scratch, uint64_t scratchBytes,
+                             void* stream) {
+    WRK_REQUIRE(offsetsOut && scratch, "wrk_partition: null argument");
+    WRK_REQUIRE(n == 0 || (category && out), "wrk_partition: null argument");
+    return partition_dispatch(category, in, n, numCategories, out, offsetsOut, scratch, scratchBytes,
+                              as_stream(stream));
+}
+
+extern "C" uint32_t wrk_partition_noalloc_max_items(void) { return kNoAllocMaxItems; }
+
+extern "C" int wrk_partition_noalloc(const uint8_t* category, const uint32_t* in, uint32_t n,
+                                     uint32_t numCategories, uint32_t* out, uint32_t* offsetsOut, void* stream) {
+    WRK_REQUIRE(offsetsOut, "wrk_partition_noalloc: null argument");
+    WRK_REQUIRE(n == 0 || (category && out), "wrk_partition_noalloc: null argument");
+    if (n > kNoAllocMaxItems) {
+        set_error("wrk_partition_noalloc: n = " + std::to_string(n) + " exceeds the maximum of " +
+                  std::to_string(kNoAllocMaxItems) + " items (use wrk_partition with a scratch buffer)");
+        return WRK_ERR_INVALID;
+    }
+    WRK_REQUIRE(numCategories >= 1 && numCategories <= kNoAllocMaxCategories,
+                "wrk_partition_noalloc: numCategories must be in [1, 16]");
+    int device = -1;
+    WRK_HIP_TRY(hipGetDevice(&device));
+    WRK_REQUIRE(device >= 0 && device < kMaxDevices, "wrk_partition_noalloc: device index out of range");
+    void* scratch = noalloc_scratch(device);
+    if (!scratch) return WRK_ERR_HIP;
+    return partition_dispatch(category, in, n, numCategories, out, offsetsOut, scratch, kNoAllocScratchBytes,
+                              as_stream(stream));
 }
 
 // keep[i] != 0 -> category 0 (kept), else category 1 (dropped: >= K).
