@@ -89,24 +89,24 @@ typedef struct wrk_scene_view {
     uint32_t _pad2[3];
 } wrk_scene_view;
 
-/* BVH over leaf-ordered triangles; nodeBytes gives the node format.
- * 96 (default): compressed 8-wide node = float origin[3] and the three
- * biased float exponents e_x, e_y, e_z in bytes 12..14 (16 B), child[8]
- * (32 B), then 8-bit planes qlo.x[8] qlo.y[8] qlo.z[8] qhi.x[8] qhi.y[8]
- * qhi.z[8] (48 B); plane = fma(q, 2^(e - 127), origin), rounded outward at
- * build time. 128: 4-wide node = child boxes as lo.x[4] hi.x[4] lo.y[4]
- * hi.y[4] lo.z[4] hi.z[4], then child[4], then pad[4].
+/* BVH over leaf-ordered triangles; nodeBytes gives the node format (96).
+ * Compressed 8-wide node = float origin[3] and the three biased float
+ * exponents e_x, e_y, e_z in bytes 12..14 (16 B), child[8] (32 B), then
+ * 8-bit planes qlo.x[8] qlo.y[8] qlo.z[8] qhi.x[8] qhi.y[8] qhi.z[8] (48 B);
+ * plane = fma(q, 2^(e - 127), origin), rounded outward at build time.
  * child: bit31 clear -> inner node index; bit31 set -> leaf with
- * count-1 in bits 27..30 and first leaf-ordered triangle in bits 0..26;
- * 0xFFFFFFFF -> empty slot (its box is inverted). Triangles: 3 float4 each,
- * (v0.xyz, original primitive id as bits), (v1.xyz, 0), (v2.xyz, 0). */
+ * count-1 in bits 27..30 and first leaf-ordered triangle in bits 0..26
+ * (at most 2 triangles per leaf); 0xFFFFFFFF -> empty slot (its box is
+ * inverted; used slots come first). Triangles: 3 float4 each, (v0.xyz,
+ * original primitive id as bits), (v1.xyz, 0 or 1 + the triangle's shade
+ * category after wrk_bvh_tag_categories), (v2.xyz, 0). */
 typedef struct wrk_bvh {
     uint64_t nodes;
     uint64_t tris;
     uint32_t numNodes;
     uint32_t numTris;
     uint32_t maxDepth;
-    uint32_t nodeBytes; /* 96: compressed 8-wide nodes, 128: 4-wide nodes */
+    uint32_t nodeBytes; /* 96: compressed 8-wide nodes */
     float sceneLo[4];
     float sceneHi[4];
 } wrk_bvh;
@@ -164,10 +164,15 @@ int wrk_device_count(int* count);
 /* Sizes of the caller-provided buffers for wrk_bvh_build. */
 int wrk_bvh_query_sizes(uint32_t numTris, uint64_t* nodeBytes, uint64_t* triBytes,
                         uint64_t* scratchBytes);
-/* GPU LBVH: 30-bit Morton codes of triangle centroids, LDS radix sort,
- * Karras 2012 hierarchy, bottom-up boxes, collapse to 4-wide nodes. Replaces
- * optixAccelComputeMemoryUsage + optixAccelBuild (scene.cpp:493-550).
- * Synchronizes `stream` (reads level counts back to the host). */
+/* GPU LBVH: 63-bit Morton codes of triangle centroids (21 bits per axis),
+ * the LDS-staged radix sort of wrk_radix_sort_pairs_u64, a binary tree by
+ * agglomerative clustering of the Morton-ordered triangles (PLOC, Meister
+ * and Bittner 2018, neighbour radius 8), bottom-up boxes and SAH cost rows
+ * (Ylitie, Karras and Laine 2017), the SAH-optimal collapse to compressed
+ * 8-wide nodes (format above; leaves of at most 2 triangles) and the
+ * triangle records in leaf order. Replaces optixAccelComputeMemoryUsage +
+ * optixAccelBuild (scene.cpp:493-550). Synchronizes `stream` (reads level
+ * and cluster counts back to the host). */
 int wrk_bvh_build(const float* vertices, const uint32_t* indices, uint32_t numTris,
                   void* nodes, void* tris, void* scratch, uint64_t scratchBytes,
                   wrk_bvh* out, void* stream);
@@ -210,6 +215,25 @@ int wrk_trace_closest_counted(const wrk_bvh* bvh, const void* org4, const void* 
 int wrk_trace_any_counted(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue,
                           uint32_t n, const uint32_t* countDev, uint8_t* occluded, unsigned long long* fetches,
                           void* stream);
+
+/* ---------------------------------------------------- sort and scan --- */
+/* Building blocks of wrk_bvh_build (inside optixAccelBuild in the
+ * reference, scene.cpp:525-550), exported for callers and tests.
+ * Stable LSD radix sort of n (key, value) pairs by key bits
+ * [beginBit, endBit), 8 bits per pass: per-tile LDS histograms, a scan of
+ * the digit x tile counts, and a scatter that ranks each tile's keys with
+ * wave ballots and stages them in LDS in digit order. The sorted pairs end
+ * in keys/vals (equal to std::stable_sort by key); keysTmp/valsTmp are
+ * n-entry ping-pong buffers. */
+uint64_t wrk_radix_sort_scratch_bytes(uint32_t n);
+int wrk_radix_sort_pairs_u64(uint64_t* keys, uint32_t* vals, uint64_t* keysTmp, uint32_t* valsTmp, uint32_t n,
+                             uint32_t beginBit, uint32_t endBit, void* scratch, uint64_t scratchBytes,
+                             void* stream);
+/* Exclusive scan: out[i] = in[0] + ... + in[i-1] (in == out allowed);
+ * *total (device, may be NULL) = the sum of all n values. */
+uint64_t wrk_exclusive_scan_scratch_bytes(uint32_t n);
+int wrk_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, void* scratch,
+                           uint64_t scratchBytes, void* stream);
 
 /* -------------------------------------------------------- compaction --- */
 /* Scratch bytes for wrk_partition / wrk_compact over n items. */

@@ -13,16 +13,13 @@ from __future__ import annotations
 
 import ctypes as C
 import math
-import os
 from dataclasses import dataclass, field
 from pathlib import Path
 
 import numpy as np
 
 PKG = Path(__file__).resolve().parent
-# OFFLINERENDERER_AMD_LIB selects another build of the same library (used to
-# compare tuning variants of the kernels); default is the in-tree build.
-LIB_PATH = Path(os.environ.get("OFFLINERENDERER_AMD_LIB", PKG / "lib" / "libwr_amd.so"))
+LIB_PATH = PKG / "lib" / "libwr_amd.so"
 
 _lib = None
 
@@ -181,6 +178,11 @@ _SIGS = {
     "wrk_compact": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P, C.c_uint64, _P]),
     "wrk_partition": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint64, _P]),
     "wrk_partition_noalloc": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, _P, _P]),
+    "wrk_radix_sort_scratch_bytes": (C.c_uint64, [C.c_uint32]),
+    "wrk_radix_sort_pairs_u64": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_uint64,
+                                           _P]),
+    "wrk_exclusive_scan_scratch_bytes": (C.c_uint64, [C.c_uint32]),
+    "wrk_exclusive_scan_u32": (C.c_int, [_P, _P, C.c_uint32, _P, _P, C.c_uint64, _P]),
     "wrk_partition_noalloc_max_items": (C.c_uint32, []),
     "wrk_generate": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32,
                                _P]),

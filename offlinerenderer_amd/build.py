@@ -23,6 +23,7 @@ LIB = PKG / "lib" / "libwr_amd.so"
 
 SOURCES = [
     "bvh.hip",
+    "sort.hip",
     "partition.hip",
     "wavefront.hip",
     "denoise.hip",

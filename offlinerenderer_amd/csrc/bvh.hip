@@ -1,12 +1,11 @@
-// bvh.hip — GPU LBVH build (Morton codes, radix sort, Karras 2012 hierarchy,
-// bottom-up boxes, collapse to 4-wide nodes) and watertight FP32
-// closest-hit / any-hit traversal for gfx950.
+// bvh.hip — GPU LBVH build (63-bit Morton codes, LDS-staged radix sort,
+// agglomerative binary tree, bottom-up boxes and SAH cost rows, SAH-optimal
+// collapse to compressed 8-wide nodes) and watertight FP32 closest-hit /
+// any-hit traversal for gfx950.
 //
 // Replaces the OptiX GAS build (libWR/src/internal/scene.cpp:493-550) and
 // optixTrace (libWR/kernels/trace.cu:27-51,127-151).
 #include <hip/hip_runtime.h>
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cstring>
@@ -143,24 +142,8 @@ __global__ __launch_bounds__(1024) void k_reduce_bounds(const float* __restrict_
     }
 }
 
-#ifndef WRK_MORTON64
-#define WRK_MORTON64 1
-#endif
-#if WRK_MORTON64
 typedef uint64_t MortonKey;
 constexpr unsigned kMortonBits = 63;
-#else
-typedef uint32_t MortonKey;
-constexpr unsigned kMortonBits = 30;
-#endif
-
-__device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
-    v = (v * 0x00010001u) & 0xFF0000FFu;
-    v = (v * 0x00000101u) & 0x0F00F00Fu;
-    v = (v * 0x00000011u) & 0xC30C30C3u;
-    v = (v * 0x00000005u) & 0x49249249u;
-    return v;
-}
 
 // 21 bits per axis -> every third bit of a 63-bit code
 __device__ __forceinline__ uint64_t expand_bits21(uint64_t v) {
@@ -190,62 +173,11 @@ __global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ triLo
         float mx = unordered(counters[C_CMAX + a]);
         float ext = mx - mn;
         float u = ext > 0.0f ? (c[a] - mn) / ext : 0.0f;
-#if WRK_MORTON64
         float s = fminf(fmaxf(u * 2097152.0f, 0.0f), 2097151.0f);
-#else
-        float s = fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
-#endif
         q[a] = (uint32_t)s;
     }
-#if WRK_MORTON64
     keys[t] = (expand_bits21(q[0]) << 2) | (expand_bits21(q[1]) << 1) | expand_bits21(q[2]);
-#else
-    keys[t] = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
-#endif
     vals[t] = t;
-}
-
-// Karras 2012: common prefix length of sorted keys i and j, with the index
-// appended to break ties between equal codes.
-__device__ __forceinline__ int delta(const MortonKey* keys, int n, int i, int j) {
-    if (j < 0 || j >= n) return -1;
-    MortonKey ki = keys[i], kj = keys[j];
-#if WRK_MORTON64
-    if (ki == kj) return 64 + __clz((uint32_t)i ^ (uint32_t)j);
-    return __clzll((unsigned long long)(ki ^ kj));
-#else
-    if (ki == kj) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
-    return __clz(ki ^ kj);
-#endif
-}
-
-__global__ __launch_bounds__(256) void k_karras(const MortonKey* __restrict__ keys, int n, BinNode* bnodes) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n - 1) return;
-    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) > 0 ? 1 : -1;
-    int dmin = delta(keys, n, i, i - d);
-    int lmax = 2;
-    while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
-    int l = 0;
-    for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
-    int j = i + l * d;
-    int dnode = delta(keys, n, i, j);
-    int s = 0;
-    int t = l;
-    do {
-        t = (t + 1) >> 1;
-        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
-    } while (t > 1);
-    int gamma = i + s * d + (d < 0 ? -1 : 0);
-    int first = i < j ? i : j, last = i < j ? j : i;
-    uint32_t left = (first == gamma) ? (kLeafBit | (uint32_t)gamma) : (uint32_t)gamma;
-    uint32_t right = (last == gamma + 1) ? (kLeafBit | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
-    BinNode& b = bnodes[i];
-    b.lo.w = __uint_as_float(left);
-    b.hi.w = __uint_as_float(right);
-    b.first = (uint32_t)first;
-    b.last = (uint32_t)last;
 }
 
 __device__ __forceinline__ void child_box(const BinNode* bnodes, const float4* triLo, const float4* triHi,
@@ -268,11 +200,8 @@ __device__ __forceinline__ void child_box(const BinNode* bnodes, const float4* t
 // finds the neighbour within WRK_PLOC_RADIUS positions whose merged box has
 // the least surface area, mutual nearest neighbours merge into a new binary
 // node, and the survivors are compacted in order. The result is a binary
-// tree of SAH quality close to a top-down sweep, unlike the Morton-split
-// hierarchy whose splits ignore triangle sizes.
-#ifndef WRK_PLOC
-#define WRK_PLOC 1
-#endif
+// tree of SAH quality close to a top-down sweep, unlike a Morton-split
+// (Karras 2012) hierarchy whose splits ignore triangle sizes.
 #ifndef WRK_PLOC_RADIUS
 #define WRK_PLOC_RADIUS 8
 #endif
@@ -457,9 +386,6 @@ __device__ __forceinline__ float half_area(const float lo[3], const float hi[3])
 // between n's two children), C(n, i > 1) = min(C(n, i - 1), best split of
 // i slots). A is the surface area. The collapse follows the argmins top-down,
 // recomputing the same float sums, so its choices match the table exactly.
-#ifndef WRK_SAH_COLLAPSE
-#define WRK_SAH_COLLAPSE 1
-#endif
 #ifndef WRK_SAH_CPRIM
 #define WRK_SAH_CPRIM 0.3f
 #endif
@@ -520,23 +446,20 @@ __global__ __launch_bounds__(256) void k_refit_level(const uint32_t* __restrict_
     float hi[3] = {fmaxf(lhi[0], rhi[0]), fmaxf(lhi[1], rhi[1]), fmaxf(lhi[2], rhi[2])};
     b->lo = make_float4(lo[0], lo[1], lo[2], __uint_as_float(l));
     b->hi = make_float4(hi[0], hi[1], hi[2], __uint_as_float(r));
-    if (WRK_SAH_COLLAPSE) {
-        float cl[W], cr[W], row[W];
-        sah_row<W>(cost, triLo, triHi, vals, l, cl);
-        sah_row<W>(cost, triLo, triHi, vals, r, cr);
-        const float area = half_area(lo, hi);
-        int k;
-        const float inner = sah_split<W>(cl, cr, W, &k) + area * kSahNode;
-        row[0] = fminf(sah_leaf_cost(area, b->last - b->first + 1u), inner);
-        for (int j = 2; j <= W; ++j) row[j - 1] = fminf(row[j - 2], sah_split<W>(cl, cr, j, &k));
+    float cl[W], cr[W], row[W];
+    sah_row<W>(cost, triLo, triHi, vals, l, cl);
+    sah_row<W>(cost, triLo, triHi, vals, r, cr);
+    const float area = half_area(lo, hi);
+    int k;
+    const float inner = sah_split<W>(cl, cr, W, &k) + area * kSahNode;
+    row[0] = fminf(sah_leaf_cost(area, b->last - b->first + 1u), inner);
+    for (int j = 2; j <= W; ++j) row[j - 1] = fminf(row[j - 2], sah_split<W>(cl, cr, j, &k));
 #pragma unroll
-        for (int j = 0; j < W; ++j) cost[(size_t)node * W + j] = row[j];
-    }
+    for (int j = 0; j < W; ++j) cost[(size_t)node * W + j] = row[j];
 }
 
-// Collapse one binary subtree root into a W-wide node: repeatedly open the
-// child with the largest surface area until W children, leaving subtrees of
-// at most kLeafMax triangles as leaves. W = 8 writes the compressed Node8.
+// Collapse one binary subtree root into a W-wide node following the SAH
+// cost rows, and write it as a compressed Node8.
 template <int W>
 __global__ __launch_bounds__(256) void k_collapse(const uint4* __restrict__ tasksIn, uint32_t numTasks,
                                                   const BinNode* __restrict__ bnodes, const float4* __restrict__ triLo,
@@ -550,14 +473,12 @@ __global__ __launch_bounds__(256) void k_collapse(const uint4* __restrict__ task
 #pragma unroll
     for (int a = 0; a < 3; ++a)
         sceneExtent = fmaxf(sceneExtent, unordered(counters[C_BMAX + a]) - unordered(counters[C_BMIN + a]));
+    static_assert(W == 8, "the node format is the compressed 8-wide Node8");
     uint32_t ref[W];
     uint32_t m = 0;
-    auto count_of = [&](uint32_t r) -> uint32_t {
-        return (r & kLeafBit) ? 1u : (bnodes[r].last - bnodes[r].first + 1u);
-    };
     if (task.x & kLeafBit) {  // single-triangle scene
         ref[m++] = task.x;
-    } else if (WRK_SAH_COLLAPSE) {
+    } else {
         // follow the SAH table: split W slots between the root's children,
         // then each (subtree, slots) pair either into one child (a leaf or
         // an inner node, as its cost row decided) or on into its children
@@ -591,25 +512,6 @@ __global__ __launch_bounds__(256) void k_collapse(const uint4* __restrict__ task
             sr[top] = cr_; sj[top++] = jj - (uint32_t)k;
             sr[top] = cl_; sj[top++] = (uint32_t)k;
         }
-    } else {
-        ref[m++] = __float_as_uint(bnodes[task.x].lo.w);
-        ref[m++] = __float_as_uint(bnodes[task.x].hi.w);
-    }
-    while (!WRK_SAH_COLLAPSE && m < (uint32_t)W) {
-        int best = -1;
-        float bestA = -1.0f;
-        for (uint32_t c = 0; c < m; ++c) {
-            if (!(ref[c] & kLeafBit) && count_of(ref[c]) > kLeafMax) {
-                float lo[3] = {bnodes[ref[c]].lo.x, bnodes[ref[c]].lo.y, bnodes[ref[c]].lo.z};
-                float hi[3] = {bnodes[ref[c]].hi.x, bnodes[ref[c]].hi.y, bnodes[ref[c]].hi.z};
-                float a = half_area(lo, hi);
-                if (a > bestA) { bestA = a; best = (int)c; }
-            }
-        }
-        if (best < 0) break;
-        uint32_t r = ref[best];
-        ref[best] = __float_as_uint(bnodes[r].lo.w);
-        ref[m++] = __float_as_uint(bnodes[r].hi.w);
     }
     float blo[W][3], bhi[W][3];
     uint32_t child[W];
@@ -632,7 +534,7 @@ __global__ __launch_bounds__(256) void k_collapse(const uint4* __restrict__ task
             lo[0] = b.lo.x; lo[1] = b.lo.y; lo[2] = b.lo.z; hi[0] = b.hi.x; hi[1] = b.hi.y; hi[2] = b.hi.z;
             uint32_t cnt = b.last - b.first + 1u;
             bool asLeaf = cnt <= kLeafMax;
-            if (WRK_SAH_COLLAPSE && asLeaf) {
+            if (asLeaf) {
                 // a leaf where its cost row chose the leaf cost
                 float row[W];
                 sah_row<W>(cost, triLo, triHi, vals, r, row);
@@ -658,18 +560,7 @@ __global__ __launch_bounds__(256) void k_collapse(const uint4* __restrict__ task
             bhi[c][a] = hi[a] + pad;
         }
     }
-    if constexpr (W == 4) {
-        Node4 out;
-        out.lox = make_float4(blo[0][0], blo[1][0], blo[2][0], blo[3][0]);
-        out.hix = make_float4(bhi[0][0], bhi[1][0], bhi[2][0], bhi[3][0]);
-        out.loy = make_float4(blo[0][1], blo[1][1], blo[2][1], blo[3][1]);
-        out.hiy = make_float4(bhi[0][1], bhi[1][1], bhi[2][1], bhi[3][1]);
-        out.loz = make_float4(blo[0][2], blo[1][2], blo[2][2], blo[3][2]);
-        out.hiz = make_float4(bhi[0][2], bhi[1][2], bhi[2][2], bhi[3][2]);
-        out.child = make_uint4(child[0], child[1], child[2], child[3]);
-        out.pad = make_uint4(0, 0, 0, 0);
-        static_cast<Node4*>(nodesOut)[task.y] = out;
-    } else {
+    {
         // quantise: per axis the smallest power-of-two step with which 255
         // steps cover the union, then each plane rounded outward and checked
         // with the decoding expression the traversal uses, fma(q, s, origin)
@@ -744,24 +635,20 @@ __global__ __launch_bounds__(256) void k_reorder_tris(const uint32_t* __restrict
 // traversal stack keeps its top kStackLds entries in LDS (a per-lane ring,
 // column-major so a wave touches distinct banks) and spills older entries
 // to a per-lane scratch array. Closest-hit rays visit children near to far,
-// shadow (any-hit) rays far to near. W = 8 (compressed nodes): each child
-// box is decoded and tested in turn, the hit children (leaves included) are
-// ordered by a branch-free sorting network and stacked, and a leaf is
-// intersected when it is popped, so the node decode and the triangle test
-// never hold registers at the same time (64 VGPRs, 8 waves per SIMD, no
-// spills). W = 4: leaf children are intersected as soon as their node is
-// visited. Box test: slabs (lo - o) * (1/d) on boxes padded outward by
-// 2^-20 * (|coordinate| + scene extent) (and, for W = 8, rounded outward
-// to the quantisation grid), far distance (clamped to the best hit) scaled
+// shadow (any-hit) rays far to near. Each child box of a compressed node is
+// decoded and tested in turn, the hit children (leaves included) are
+// ordered by a branch-free sorting network and stacked, and leaves wait for
+// a wave-wide leaf step, so the node decode and the triangle test never
+// hold registers at the same time (64 VGPRs, 8 waves per SIMD, no spills).
+// Box test: slabs (lo - o) * (1/d) on boxes padded outward by
+// 2^-20 * (|coordinate| + scene extent) and rounded outward to the
+// quantisation grid, far distance (clamped to the best hit) scaled
 // by 1 + 2*gamma_3 (Ize 2013), so a box is never culled while it may hold
 // a triangle the watertight test reports at t <= best (ties included). Ties
 // in t resolve to the smaller primitive id, so the result does not depend
 // on the traversal order.
 #ifndef WRK_TRACE_WAVES
 #define WRK_TRACE_WAVES 8
-#endif
-#ifndef WRK_LEAF_BATCH
-#define WRK_LEAF_BATCH 1
 #endif
 #ifndef WRK_LEAF_BATCH_NUM
 #define WRK_LEAF_BATCH_NUM 1
@@ -791,7 +678,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
     const uint32_t chunkEnd = min(chunkBegin + 64u * raysPerLane, cnt);
     uint32_t cursor = chunkBegin;
     const float kRobust = 1.0000005f;
-    constexpr bool kLeafOnStack = W == 8;  // leaves go through the stack
+    static_assert(W == 8, "traversal of compressed 8-wide nodes");
 
     uint32_t spillN[kStackSpill];
     float spillT[kStackSpill];
@@ -836,7 +723,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
 
         // one traversal step: a wave-wide leaf step or node step, then pop
         bool leafStep = false;
-        if constexpr (kLeafOnStack && WRK_LEAF_BATCH) {
+        {
             // Leaves found while descending wait in `leaf` (one per lane)
             // while the lane keeps visiting inner nodes; the wave tests its
             // pending leaves together once half of its active lanes hold
@@ -851,8 +738,6 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
             const uint32_t nPend = __popcll(__ballot(leaf != kEmptyChild));
             const bool canStep = __ballot(cur != kEmptyChild && !(cur & kLeafBit)) != 0;
             leafStep = nPend != 0 && (!canStep || nPend * WRK_LEAF_BATCH_DEN >= nAct * WRK_LEAF_BATCH_NUM);
-        } else if constexpr (kLeafOnStack) {
-            if (cur != kEmptyChild && (cur & kLeafBit)) { leaf = cur; cur = kEmptyChild; leafStep = true; }
         }
         if (leafStep) {
             if (leaf != kEmptyChild) {
@@ -877,7 +762,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
             leaf = kEmptyChild;
             if (ANY && found) { cur = kEmptyChild; sp = 0; }
             }
-        } else if (cur != kEmptyChild && !(kLeafOnStack && (cur & kLeafBit))) {
+        } else if (cur != kEmptyChild && !(cur & kLeafBit)) {
             if (COUNT) ++nodeFetches;
             float key[W];
             uint32_t code[W];
@@ -890,16 +775,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                 float tf = fminf(fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)), tbest) * kRobust;
                 return (cc != kEmptyChild && tn <= tf) ? tn : INFINITY;
             };
-            if constexpr (W == 4) {
-                const Node4* nd = static_cast<const Node4*>(nodesIn) + cur;
-                const float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
-                const uint4 ch = nd->child;
-                code[0] = ch.x; code[1] = ch.y; code[2] = ch.z; code[3] = ch.w;
-                key[0] = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, code[0]);
-                key[1] = slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, code[1]);
-                key[2] = slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, code[2]);
-                key[3] = slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, code[3]);
-            } else {
+            {
                 // six 16-byte loads: origin | child 0-3 | child 4-7 | 48 quantised planes
                 const uint4* raw = reinterpret_cast<const uint4*>(static_cast<const Node8*>(nodesIn) + cur);
                 const uint4 r0 = raw[0], r1 = raw[1], r2 = raw[2], r3 = raw[3], r4 = raw[4], r5 = raw[5];
@@ -921,41 +797,12 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
 
                 }
             }
-            // leaf children first (unless leaves go through the stack)
-#pragma unroll
-            for (int c = 0; c < W; ++c) {
-                if (!kLeafOnStack && key[c] != INFINITY && (code[c] & kLeafBit)) {
-                    key[c] = INFINITY;
-                    uint32_t first = code[c] & 0x07FFFFFFu;
-                    uint32_t count = ((code[c] >> 27) & 0xFu) + 1u;
-                    for (uint32_t k = 0; k < count; ++k) {
-                        const float4* tp = tris + (size_t)(first + k) * 3;
-                        if (COUNT) ++triFetches;
-                        float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
-                        float th;
-                        if (intersect_tri(rp, v0, v1, v2, tmin, tbest, &th)) {
-                            uint32_t prim = __float_as_uint(v0.w);
-                            if (ANY) { found = true; break; }
-                            if (th < tbest || prim < (bestPrim & kPrimMask)) {
-                                tbest = th;
-                                bestPrim = prim | ((__float_as_uint(v1.w) & 7u) << 28);
-                            }
-                        }
-                    }
-                }
-            }
             if (ANY && found) {
                 cur = kEmptyChild;
                 sp = 0;
             } else {
-                // children in visiting order (sorting network on W keys)
-                // (leaves just tested may have shortened the ray; with leaves
-                // on the stack the keys are already clamped to the ray)
-                if constexpr (!kLeafOnStack) {
-#pragma unroll
-                    for (int c = 0; c < W; ++c)
-                        if (key[c] > tbest * kRobust) key[c] = INFINITY;
-                }
+                // children in visiting order (sorting network on W keys,
+                // already clamped to the ray)
 // branch-free compare-exchange (selects, no divergent blocks)
 #define WRK_CSWAP(a, b)                                                             \
     {                                                                               \
@@ -976,9 +823,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                     for (int c = 0; c < W; ++c)
                         if (key[c] != INFINITY) key[c] = -key[c];
                 }
-                if constexpr (W == 4) {
-                    WRK_CSWAP(0, 1) WRK_CSWAP(2, 3) WRK_CSWAP(0, 2) WRK_CSWAP(1, 3) WRK_CSWAP(1, 2)
-                } else if (ANY) {
+                if (ANY) {
                     // shadow rays: a tournament puts the farthest child first;
                     // the order of the others barely matters to an any-hit
                     // query (measured: 21.3 -> 20.5 ms per step; for closest-
@@ -1012,7 +857,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                 cur = (key[0] != INFINITY) ? code[0] : kEmptyChild;
             }
         }
-        if (kLeafOnStack && WRK_LEAF_BATCH && cur != kEmptyChild && (cur & kLeafBit) && leaf == kEmptyChild) {
+        if (cur != kEmptyChild && (cur & kLeafBit) && leaf == kEmptyChild) {
             leaf = cur;  // the nearest child is a leaf: it waits for a leaf step
             cur = kEmptyChild;
         }
@@ -1028,7 +873,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                     sT[slot][lane] = spillT[sp - kStackLds];
                 }
                 if (tt <= tbest * kRobust) {
-                    if (kLeafOnStack && WRK_LEAF_BATCH && (nn & kLeafBit) && leaf == kEmptyChild) {
+                    if ((nn & kLeafBit) && leaf == kEmptyChild) {
                         leaf = nn;  // park it and look for an inner node
                         continue;
                     }
@@ -1037,7 +882,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                     break;
                 }
             }
-            if (kLeafOnStack && leaf != kEmptyChild) done = false;
+            if (leaf != kEmptyChild) done = false;
             if (done) {
                 if (ANY) occl[ray] = found ? 1 : 0;
                 else {
@@ -1090,28 +935,20 @@ BuildLayout layout_for(uint32_t n) {
     L.tasksA = take(nn * 16);
     L.tasksB = take(nn * 16);
     L.counters = take(C_COUNT * 4);
-    L.costs = take(WRK_SAH_COLLAPSE ? nn * kBvhWidth * 4 : 4);
-    size_t tmp = 0;
-    (void)rocprim::radix_sort_pairs(nullptr, tmp, (MortonKey*)nullptr, (MortonKey*)nullptr, (uint32_t*)nullptr,
-                                    (uint32_t*)nullptr, (size_t)nn, 0u, kMortonBits);
-    L.sortTempBytes = tmp;
-    L.sortTemp = take(tmp);
-    if (WRK_PLOC) {
-        L.cidA = take(nn * 4);
-        L.cidB = take(nn * 4);
-        L.cloA = take(nn * 16);
-        L.cloB = take(nn * 16);
-        L.chiA = take(nn * 16);
-        L.chiB = take(nn * 16);
-        L.nn = take(nn * 4);
-        L.keep = take(nn * 4);
-        L.pos = take(nn * 4);
-        size_t st = 0;
-        (void)rocprim::exclusive_scan(nullptr, st, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u, (size_t)nn,
-                                      rocprim::plus<uint32_t>());
-        L.scanTempBytes = st;
-        L.scanTemp = take(st);
-    }
+    L.costs = take(nn * kBvhWidth * 4);
+    L.sortTempBytes = radix_sort_scratch_bytes((uint32_t)nn);
+    L.sortTemp = take(L.sortTempBytes);
+    L.cidA = take(nn * 4);
+    L.cidB = take(nn * 4);
+    L.cloA = take(nn * 16);
+    L.cloB = take(nn * 16);
+    L.chiA = take(nn * 16);
+    L.chiB = take(nn * 16);
+    L.nn = take(nn * 4);
+    L.keep = take(nn * 4);
+    L.pos = take(nn * 4);
+    L.scanTempBytes = exclusive_scan_scratch_bytes((uint32_t)nn);
+    L.scanTemp = take(L.scanTempBytes);
     L.total = off;
     return L;
 }
@@ -1121,7 +958,7 @@ extern "C" int wrk_bvh_query_sizes(uint32_t numTris, uint64_t* nodeBytes, uint64
                                    uint64_t* scratchBytes) {
     WRK_REQUIRE(nodeBytes && triBytes && scratchBytes, "wrk_bvh_query_sizes: null output");
     WRK_REQUIRE(numTris > 0 && numTris < (1u << 27), "wrk_bvh_query_sizes: numTris must be in [1, 2^27)");
-    *nodeBytes = (uint64_t)std::max<uint32_t>(numTris, 1u) * (kBvhWidth == 8 ? sizeof(Node8) : sizeof(Node4));
+    *nodeBytes = (uint64_t)std::max<uint32_t>(numTris, 1u) * sizeof(Node8);
     *triBytes = (uint64_t)numTris * 48u;
     *scratchBytes = layout_for(numTris).total;
     return WRK_OK;
@@ -1156,21 +993,23 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
     hipLaunchKernelGGL(k_reduce_bounds, dim3(1), dim3(1024), 0, s, partials, g, counters);
     hipLaunchKernelGGL(k_morton, dim3(g), dim3(256), 0, s, triLo, triHi, n, counters, keysA, valsA);
     WRK_CHECK_LAUNCH();
-    size_t tmp = L.sortTempBytes;
-    hipError_t sortErr = rocprim::radix_sort_pairs(base + L.sortTemp, tmp, keysA, keysB, valsA, valsB, (size_t)n, 0u,
-                                                   kMortonBits, s);
-    WRK_HIP_TRY(sortErr);
+    // Morton order (stable: equal codes keep triangle order)
+    const int sortRc = radix_sort_pairs_u64(keysA, valsA, keysB, valsB, n, 0u, kMortonBits, base + L.sortTemp,
+                                            L.sortTempBytes, s);
+    if (sortRc != WRK_OK) return sortRc;
+    const uint32_t* sorted = valsA;  // triangles in Morton order
+    uint32_t* leafOrder = valsB;     // triangles in leaf order (written below)
     uint32_t root = 0;
-    const uint32_t* valsF = valsB;  // triangles in leaf order
+    const uint32_t* valsF = sorted;
     if (n > 1) {
-        if (WRK_PLOC) {
+        {
             uint32_t* cid[2] = {(uint32_t*)(base + L.cidA), (uint32_t*)(base + L.cidB)};
             float4* clo[2] = {(float4*)(base + L.cloA), (float4*)(base + L.cloB)};
             float4* chi[2] = {(float4*)(base + L.chiA), (float4*)(base + L.chiB)};
             uint32_t* nnb = (uint32_t*)(base + L.nn);
             uint32_t* keep = (uint32_t*)(base + L.keep);
             uint32_t* pos = (uint32_t*)(base + L.pos);
-            hipLaunchKernelGGL(k_ploc_init, dim3(g), dim3(256), 0, s, triLo, triHi, valsB, n, cid[0], clo[0], chi[0]);
+            hipLaunchKernelGGL(k_ploc_init, dim3(g), dim3(256), 0, s, triLo, triHi, sorted, n, cid[0], clo[0], chi[0]);
             WRK_CHECK_LAUNCH();
             uint32_t c = n;
             int cur = 0;
@@ -1184,9 +1023,9 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
                 hipLaunchKernelGGL(k_ploc_merge, dim3(gc), dim3(256), 0, s, c, nnb, cid[cur], clo[cur], chi[cur],
                                    bnodes, counters, keep);
                 WRK_CHECK_LAUNCH();
-                size_t st = L.scanTempBytes;
-                WRK_HIP_TRY(rocprim::exclusive_scan(base + L.scanTemp, st, keep, pos, 0u, (size_t)c,
-                                                    rocprim::plus<uint32_t>(), s));
+                const int scanRc = exclusive_scan_u32(keep, pos, c, (uint32_t*)(base + L.scanTemp), L.scanTempBytes,
+                                                      nullptr, s);
+                if (scanRc != WRK_OK) return scanRc;
                 hipLaunchKernelGGL(k_ploc_compact, dim3(gc), dim3(256), 0, s, c, keep, pos, cid[cur], clo[cur],
                                    chi[cur], cid[cur ^ 1], clo[cur ^ 1], chi[cur ^ 1], counters);
                 WRK_CHECK_LAUNCH();
@@ -1206,9 +1045,6 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
                 set_error("wrk_bvh_build: clustering ended on a leaf");
                 return WRK_ERR_INVALID;
             }
-        } else {
-            hipLaunchKernelGGL(k_karras, dim3(ceil_div(n - 1, 256)), dim3(256), 0, s, keysB, (int)n, bnodes);
-            WRK_CHECK_LAUNCH();
         }
         // binary levels top-down (each internal node lands in exactly one level) ...
         WRK_HIP_TRY(hipMemcpyAsync(levels, &root, 4, hipMemcpyHostToDevice, s));
@@ -1230,9 +1066,9 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
             }
             off.push_back(off.back() + next);
         }
-        if (WRK_PLOC) {
+        {
             // contiguous triangle ranges per subtree: sizes bottom-up, then
-            // ranges top-down; valsA receives the triangles in leaf order
+            // ranges top-down; leafOrder receives the triangles in that order
             for (size_t L0 = off.size() - 1; L0-- > 0;) {
                 uint32_t cnt = off[L0 + 1] - off[L0];
                 hipLaunchKernelGGL(k_ploc_sizes, dim3(ceil_div(cnt, 256)), dim3(256), 0, s, levels + off[L0], cnt,
@@ -1241,10 +1077,10 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
             for (size_t L0 = 0; L0 + 1 < off.size(); ++L0) {
                 uint32_t cnt = off[L0 + 1] - off[L0];
                 hipLaunchKernelGGL(k_ploc_ranges, dim3(ceil_div(cnt, 256)), dim3(256), 0, s, levels + off[L0], cnt,
-                                   L0 == 0, bnodes, valsB, valsA);
+                                   L0 == 0, bnodes, sorted, leafOrder);
             }
             WRK_CHECK_LAUNCH();
-            valsF = valsA;
+            valsF = leafOrder;
         }
         // ... then boxes bottom-up, one launch per level
         for (size_t L0 = off.size() - 1; L0-- > 0;) {
@@ -1282,7 +1118,7 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
     out->numNodes = hc[C_NODES];
     out->numTris = n;
     out->maxDepth = hc[C_MAXDEPTH];
-    out->nodeBytes = kBvhWidth == 8 ? (uint32_t)sizeof(Node8) : (uint32_t)sizeof(Node4);
+    out->nodeBytes = (uint32_t)sizeof(Node8);
     for (int a = 0; a < 3; ++a) {
         out->sceneLo[a] = unordered(hc[C_BMIN + a]);
         out->sceneHi[a] = unordered(hc[C_BMAX + a]);
@@ -1322,35 +1158,22 @@ static int launch_trace(const wrk_bvh* bvh, const void* org4, const void* dir4, 
                         const uint32_t* countDev, uint32_t* hits, uint8_t* occl, unsigned long long* fetches,
                         void* stream) {
     WRK_REQUIRE(bvh && bvh->nodes && bvh->tris && org4 && dir4, "wrk_trace: null argument");
-    WRK_REQUIRE(bvh->nodeBytes == sizeof(Node8) || bvh->nodeBytes == sizeof(Node4), "wrk_trace: unknown node format");
-    WRK_REQUIRE((bvh->nodeBytes == sizeof(Node8) ? 7u : 3u) * bvh->maxDepth + 8u <= kStackTotal,
-                "wrk_trace: BVH deeper than the traversal stack");
+    WRK_REQUIRE(bvh->nodeBytes == sizeof(Node8), "wrk_trace: unknown node format");
+    WRK_REQUIRE((kBvhWidth - 1u) * bvh->maxDepth + 8u <= kStackTotal, "wrk_trace: BVH deeper than the traversal stack");
     if (n == 0) return WRK_OK;
     const uint32_t rpl = rays_per_lane(n);
     WRK_REQUIRE((uint64_t)n + 256ull * rpl <= 0xFFFFFFFFull, "wrk_trace: too many rays for one launch");
     dim3 grid(ceil_div(n, 256u * rpl)), block(256);
     hipStream_t s = as_stream(stream);
     const void* nd = (const void*)bvh->nodes;
-    const bool w8 = bvh->nodeBytes == sizeof(Node8);
-    if (fetches) {
-        if (w8)
-            hipLaunchKernelGGL((k_trace<ANY, true, 8>), grid, block, 0, s, nd, (const float4*)bvh->tris,
-                               (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl,
-                               fetches, rpl);
-        else
-            hipLaunchKernelGGL((k_trace<ANY, true, 4>), grid, block, 0, s, nd, (const float4*)bvh->tris,
-                               (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl,
-                               fetches, rpl);
-    } else {
-        if (w8)
-            hipLaunchKernelGGL((k_trace<ANY, false, 8>), grid, block, 0, s, nd, (const float4*)bvh->tris,
-                               (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl,
-                               nullptr, rpl);
-        else
-            hipLaunchKernelGGL((k_trace<ANY, false, 4>), grid, block, 0, s, nd, (const float4*)bvh->tris,
-                               (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl,
-                               nullptr, rpl);
-    }
+    if (fetches)
+        hipLaunchKernelGGL((k_trace<ANY, true, 8>), grid, block, 0, s, nd, (const float4*)bvh->tris,
+                           (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl, fetches,
+                           rpl);
+    else
+        hipLaunchKernelGGL((k_trace<ANY, false, 8>), grid, block, 0, s, nd, (const float4*)bvh->tris,
+                           (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl, nullptr,
+                           rpl);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }

@@ -208,6 +208,74 @@ __global__ __launch_bounds__(1024) void k_scan_single(uint32_t* __restrict__ cou
     if (threadIdx.x == 0) offsets[K] = total;
 }
 
+// ----------------------------------------------- general exclusive scan
+// out[i] = sum of in[0 .. i) over m values (in == out allowed), with the
+// same segment structure as the count-matrix scan: one block per 8,192
+// values, one block over the segment totals, one pass applying them; a
+// single launch when m fits one segment.
+__global__ __launch_bounds__(1024) void k_gscan_single(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                       uint32_t m, uint32_t* __restrict__ total) {
+    __shared__ uint32_t waveSum[16];
+    __shared__ uint32_t tot;
+    const uint32_t base = threadIdx.x * 8;
+    uint32_t v[8];
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        v[i] = base + i < m ? in[base + i] : 0u;
+        local += v[i];
+    }
+    uint32_t run = block_exclusive_scan_1024(local, waveSum, &tot);
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        if (base + i < m) out[base + i] = run;
+        run += v[i];
+    }
+    if (total && threadIdx.x == 0) *total = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_gscan_apply(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                      uint32_t m, const uint32_t* __restrict__ segOffsets) {
+    __shared__ uint32_t waveSum[16];
+    __shared__ uint32_t total;
+    const uint32_t base = blockIdx.x * kScanSeg + threadIdx.x * 8;
+    uint32_t v[8];
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        v[i] = base + i < m ? in[base + i] : 0u;
+        local += v[i];
+    }
+    uint32_t run = segOffsets[blockIdx.x] + block_exclusive_scan_1024(local, waveSum, &total);
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        if (base + i < m) out[base + i] = run;
+        run += v[i];
+    }
+}
+
+uint64_t exclusive_scan_scratch_bytes(uint32_t m) { return ((uint64_t)ceil_div(m, kScanSeg) + 16) * 4; }
+
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t m, uint32_t* scratch, uint64_t scratchBytes,
+                       uint32_t* total, hipStream_t s) {
+    if (m == 0) return WRK_OK;
+    const uint32_t numSeg = ceil_div(m, kScanSeg);
+    WRK_REQUIRE(scratchBytes >= exclusive_scan_scratch_bytes(m), "exclusive scan: scratch too small");
+    if (numSeg == 1) {
+        hipLaunchKernelGGL(k_gscan_single, dim3(1), dim3(1024), 0, s, in, out, m, total);
+    } else {
+        // segment totals, their scan (the grand total lands in scratch[numSeg]),
+        // then each segment with its carry
+        uint32_t* seg = scratch;
+        hipLaunchKernelGGL(k_scan_segments, dim3(numSeg), dim3(1024), 0, s, in, m, seg);
+        hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, s, seg, numSeg, 0u, seg + numSeg);
+        hipLaunchKernelGGL(k_gscan_apply, dim3(numSeg), dim3(1024), 0, s, in, out, m, seg);
+        if (total) WRK_HIP_TRY(hipMemcpyAsync(total, seg + numSeg, 4, hipMemcpyDeviceToDevice, s));
+    }
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
 __global__ void k_zero_offsets(uint32_t* offsets, uint32_t n) {
     if (threadIdx.x < n) offsets[threadIdx.x] = 0;
 }
@@ -422,6 +490,14 @@ extern "C" int wrk_partition(const uint8_t* category, const uint32_t* in, uint32
     WRK_REQUIRE(n == 0 || (category && out), "wrk_partition: null argument");
     return partition_dispatch(category, in, n, numCategories, out, offsetsOut, scratch, scratchBytes,
                               as_stream(stream));
+}
+
+extern "C" uint64_t wrk_exclusive_scan_scratch_bytes(uint32_t n) { return exclusive_scan_scratch_bytes(n); }
+
+extern "C" int wrk_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, void* scratch,
+                                      uint64_t scratchBytes, void* stream) {
+    WRK_REQUIRE(n == 0 || (in && out && scratch), "wrk_exclusive_scan_u32: null argument");
+    return exclusive_scan_u32(in, out, n, static_cast<uint32_t*>(scratch), scratchBytes, total, as_stream(stream));
 }
 
 extern "C" uint32_t wrk_partition_noalloc_max_items(void) { return kNoAllocMaxItems; }

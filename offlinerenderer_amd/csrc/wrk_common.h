@@ -40,13 +40,19 @@ void set_error(const std::string& msg);
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-// BVH node as laid out in HBM (128 bytes, one cache line pair).
-struct alignas(16) Node4 {
-    float4 lox, hix, loy, hiy, loz, hiz;
-    uint4 child;
-    uint4 pad;
-};
-static_assert(sizeof(Node4) == 128, "Node4 must be 128 bytes");
+// Device-wide exclusive scan of m uint32 values (partition.hip): out[i] =
+// sum of in[0 .. i); in == out is allowed; *total (device, optional) gets
+// the sum of all m values.
+uint64_t exclusive_scan_scratch_bytes(uint32_t m);
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t m, uint32_t* scratch, uint64_t scratchBytes,
+                       uint32_t* total, hipStream_t s);
+
+// Stable LSD radix sort of (key, value) pairs by key bits [beginBit,
+// endBit) (sort.hip). The sorted pairs end in keys/vals; keysTmp/valsTmp
+// are ping-pong buffers of n entries.
+uint64_t radix_sort_scratch_bytes(uint32_t n);
+int radix_sort_pairs_u64(uint64_t* keys, uint32_t* vals, uint64_t* keysTmp, uint32_t* valsTmp, uint32_t n,
+                         uint32_t beginBit, uint32_t endBit, void* scratch, uint64_t scratchBytes, hipStream_t s);
 
 // Compressed 8-wide node (96 bytes): child boxes quantised to 8 bits per
 // plane on a per-axis power-of-two grid anchored at `origin` (decoded as
@@ -61,10 +67,7 @@ struct alignas(16) Node8 {
 };
 static_assert(sizeof(Node8) == 96, "Node8 must be 96 bytes");
 
-#ifndef WRK_BVH_WIDTH
-#define WRK_BVH_WIDTH 8
-#endif
-constexpr uint32_t kBvhWidth = WRK_BVH_WIDTH;
+constexpr uint32_t kBvhWidth = 8;
 
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kPrimMask = 0x0FFFFFFFu;  // primitive id bits of a hit (ids < 2^27)

@@ -1,4 +1,4 @@
-"""Invariants of the GPU LBVH (compressed 8-wide or 4-wide nodes): every
+"""Invariants of the GPU LBVH (compressed 8-wide nodes): every
 triangle appears in exactly one leaf, decoded leaf boxes contain their
 triangles, child boxes nest in their parent's entry, primitive ids form a
 permutation, depth is as reported."""
@@ -31,12 +31,6 @@ def d2h(ptr, nbytes):
 def decode_nodes(b):
     """(lo, hi) as (node, child, axis) float32 and child codes (node, child)."""
     raw = d2h(b.nodes, b.numNodes * b.nodeBytes)
-    if b.nodeBytes == 128:
-        f = raw.view(np.float32).reshape(b.numNodes, 32)
-        u = raw.view(np.uint32).reshape(b.numNodes, 32)
-        lo = np.stack([f[:, 0:4], f[:, 8:12], f[:, 16:20]], axis=2)
-        hi = np.stack([f[:, 4:8], f[:, 12:16], f[:, 20:24]], axis=2)
-        return lo, hi, u[:, 24:28]
     assert b.nodeBytes == 96
     rec = raw.reshape(b.numNodes, 96)
     origin = rec[:, 0:12].copy().view(np.float32)                 # node, axis
@@ -96,11 +90,6 @@ def check_bvh(desc):
                 clo, chi = v.min(0), v.max(0)
             else:
                 clo, chi = sub_lo[code], sub_hi[code]
-                if width == 4:
-                    # uncompressed boxes: the child's entries nest in this one
-                    used = child[code] != 0xFFFFFFFF
-                    assert np.all(lo[code][used].min(0) >= lo[node, c])
-                    assert np.all(hi[code][used].max(0) <= hi[node, c])
             # every entry holds all triangles beneath it
             assert np.all(clo >= lo[node, c]) and np.all(chi <= hi[node, c]), (node, c)
             sub_lo[node] = np.minimum(sub_lo[node], clo)
