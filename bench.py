@@ -9,14 +9,19 @@ completion (no 8-iteration cap), FP32. Scene and BVH are built before the
 timed region; inputs are resident in HBM.
 
 Prints one JSON line (bench contract) with a roofline object for the
-dominant kernel (closest-hit traversal) and a CPU-baseline object (the
-oracle, OpenMP on the host cores, on a 1-spp sample of the same frame).
+dominant kernel (closest-hit traversal: algorithmic bytes per launch over
+the HIP-event launch time, and the HBM traffic measured with rocprofv3
+counters), per-kernel times and item counts, a `large` object with the same
+figures for the incoherent 10.24M-triangle scene (2,000 displaced
+icospheres, 3840x2160, 16 spp, depth 8; --no-large skips it), and a
+CPU-baseline object (the oracle, OpenMP on the host cores granted to the
+job, on a sample of whole 1-spp frames of the same workload, with the host's
+nproc and CPU model).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--spp 256]
 
---workload large measures the incoherent 10.24M-triangle scene instead
-(2,000 displaced icospheres, 3840x2160, 16 spp, depth 8); the default
-workload is the one the headline metric is quoted on.
+--workload large measures the 10.24M-triangle scene as the main line; the
+default workload is the one the headline metric is quoted on.
 
 --workload serving measures the continuous-batching server: the Cornell,
 material and large scenes resident on the device, a seeded stream of
@@ -95,6 +100,9 @@ def main():
     ap.add_argument("--dn-resume", type=str, default=None, help="denoise: continue training from this checkpoint")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target wall time of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-large", action="store_true",
+                    help="material workload: skip the large-scene sub-measurement")
+    ap.add_argument("--large-steps", type=int, default=2, help="timed renders of the large-scene sub-measurement")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the bench, run a plain device copy of this many MiB (calibrates HBM counters)")
     args = ap.parse_args()
@@ -124,10 +132,36 @@ def main():
         args.max_paths = 1 << 28
     if args.workload == "material":
         W, H, spp = args.width or 1920, args.height or 1080, args.spp or 256
+    else:
+        W, H, spp = args.width or 3840, args.height or 2160, args.spp or 16
+    m = measure_render(args, args.workload, W, H, spp, args.steps, args.warmup)
+    line = m["line"]
+    if args.workload == "material" and not args.no_large:
+        # the 10.24M-triangle scene (the traversal target is defined on it)
+        lg = measure_render(args, "large", 3840, 2160, 16, args.large_steps, 1)
+        line["large"] = {k: lg["line"][k] for k in ("value", "unit", "ms_per_step", "steps", "config",
+                                                     "mrays_per_s", "bvh_build_ms", "kernel_ms_per_step",
+                                                     "kernel_items_per_step", "roofline")}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(m["desc"], W, H, args.cpu_seconds)
+    line["cpu_baseline"] = cpu
+    print(json.dumps(line), flush=True)
+
+
+def measure_render(args, workload_kind, W, H, spp, steps, warmup):
+    """Timed renders of one workload: Mpaths/s, per-kernel times and items,
+    and the closest-hit roofline (algorithmic and measured traffic)."""
+    import numpy as np
+    import torch
+
+    import offlinerenderer_amd as wr
+    from offlinerenderer_amd import scenes
+
+    if workload_kind == "material":
         desc = scenes.material_scene(W, H, spp)
         workload = f"material_scene_{W}x{H}_{spp}spp_depth{desc.maxBounces}"
     else:
-        W, H, spp = args.width or 3840, args.height or 2160, args.spp or 16
         desc = scenes.large_scene(W, H, spp)
         workload = f"large_scene_{desc.numTris}tris_{W}x{H}_{spp}spp_depth{desc.maxBounces}"
     log(f"[bench] {workload}")
@@ -157,28 +191,30 @@ def main():
 
     r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=args.max_paths,
                                             maxChunkPaths=args.max_chunk or 0, timeKernels=True))
-    for i in range(args.warmup):
+    for i in range(warmup):
         r.render(scene, cam, prm)
-        log(f"[bench] warmup {i + 1}/{args.warmup}: {r.getStats()['seconds']:.3f} s")
+        log(f"[bench] warmup {i + 1}/{warmup}: {r.getStats()['seconds']:.3f} s")
     totals = {k: 0.0 for k in ("msTraceClosest", "msTraceAny", "msShade", "msClassify", "msPartition",
                                 "msGenerate", "msAccumulate")}
-    rays_c = rays_s = launches_c = 0
+    items = {k: 0 for k in ("closestRays", "shadowRays", "classifyItems", "partitionItems", "shadeItems")}
+    launches_c = 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         img = r.render(scene, cam, prm)
         st = r.getStats()
         for k in totals:
             totals[k] += st[k]
-        rays_c += st["closestRays"]
-        rays_s += st["shadowRays"]
+        for k in items:
+            items[k] += st[k]
         launches_c += st["launchesTraceClosest"]
-        log(f"[bench] step {i + 1}/{args.steps}: {st['seconds']:.3f} s, "
+        log(f"[bench] step {i + 1}/{steps}: {st['seconds']:.3f} s, "
             f"{st['paths'] / st['seconds'] / 1e6:.1f} Mpaths/s")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     assert np.isfinite(img).all()
-    paths = W * H * spp * args.steps
+    rays_c, rays_s = items["closestRays"], items["shadowRays"]
+    paths = W * H * spp * steps
     mpaths = paths / elapsed / 1e6
     mrays = (rays_c + rays_s) / elapsed / 1e6
 
@@ -189,7 +225,7 @@ def main():
     any_bytes_per_ray = ANY_FIXED + any_nodes * bvh.nodeBytes + any_tris * TRI_BYTES
     any_gbs = any_bytes_per_ray * rays_s / max(totals["msTraceAny"] / 1e3, 1e-9) / 1e9
 
-    if args.copy_probe_mib:
+    if args.copy_probe_mib and workload_kind == args.workload:
         nbytes = args.copy_probe_mib << 20
         src = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
         dst = torch.empty_like(src)
@@ -200,30 +236,30 @@ def main():
 
     # HBM bytes per launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE runs
     # (profiles/hbm_traffic_pmc.md): measured bytes per ray x rays per launch
-    traffic = None
+    traffic, traffic_src = None, None
     tfile = ROOT / "profiles" / "hbm_traffic.json"
     if tfile.exists():
-        t = json.loads(tfile.read_text()).get(args.workload)
+        tj = json.loads(tfile.read_text())
+        t = tj.get(workload_kind)
         if t:
             traffic = round(t["hbm_bytes_per_ray"] * rays_c / max(launches_c, 1))
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(desc, W, H, args.cpu_seconds)
+            traffic_src = f"profiles/hbm_traffic.json ({tj.get('measured_at', 'unrecorded build')})"
+    avg_launch_s = avg_launch_ms / 1e3
+    measured_gbs = traffic / avg_launch_s / 1e9 if traffic and avg_launch_s > 0 else None
 
     line = {
         "metric": "Mpaths/s @1080p; time-to-relMSE<=1e-2 learned vs plain; p50/p99 req latency",
         "value": round(mpaths, 3),
         "unit": "Mpaths/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(elapsed / steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": f"synthetic (seeded {args.workload} scene generator)",
+        "data": f"synthetic (seeded {workload_kind} scene generator)",
         "config": {
             "workload": workload,
             "width": W, "height": H, "spp": spp, "max_depth": desc.maxBounces, "triangles": desc.numTris,
@@ -232,7 +268,8 @@ def main():
         },
         "mrays_per_s": round(mrays, 3),
         "bvh_build_ms": round(bvh_build_ms, 2),
-        "kernel_ms_per_step": {k: round(v / args.steps, 3) for k, v in totals.items()},
+        "kernel_ms_per_step": {k: round(v / steps, 3) for k, v in totals.items()},
+        "kernel_items_per_step": {k: round(v / steps) for k, v in items.items()},
         "roofline": {
             "kernel": "k_trace<closest>",
             "bound": "hbm",
@@ -240,8 +277,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "frac_basis": "algorithmic bytes (every node and triangle fetch, pre-cache)",
             "traffic": traffic,
             "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE+WRITE_SIZE, calibrated)",
+            "traffic_source": traffic_src,
+            "measured_gbs": round(measured_gbs, 2) if measured_gbs else None,
+            "measured_frac": round(measured_gbs / HBM_PEAK_GBS, 4) if measured_gbs else None,
+            "limiter": "latency of dependent node fetches and lane divergence, not HBM bandwidth: the "
+                       "measured HBM traffic is below the algorithmic bytes (cache hits) and below 8 TB/s",
             "algorithmic_bytes_per_launch": round(bytes_per_ray * rays_c / max(launches_c, 1)),
             "bytes_per_ray": round(bytes_per_ray, 2),
             "nodes_per_ray": round(nodes_per_ray, 3),
@@ -250,9 +293,8 @@ def main():
             "rays_per_launch": round(rays_c / max(launches_c, 1), 1),
             "any_hit_gbs": round(any_gbs, 2),
         },
-        "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    return {"line": line, "desc": desc}
 
 
 def serving(args):
@@ -525,12 +567,39 @@ def denoise_infer(args):
     }), flush=True)
 
 
+def host_cpu():
+    """nproc, the cores this process may run on, and the CPU model (lscpu)."""
+    import subprocess
+
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                model = ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None and Path("/proc/cpuinfo").exists():
+        for ln in Path("/proc/cpuinfo").read_text().splitlines():
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    return {"nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)), "model": model,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(desc, W, H, target_s):
-    """Oracle (the CPU restatement of libWR's kernel math) on the host cores."""
+    """Oracle (the CPU restatement of libWR's kernel math) on the host cores
+    granted to this job: OMP_NUM_THREADS when the host sets it (the GPU hosts
+    give one GPU's job 16 threads), else every core of the affinity mask."""
     from oracle import pyoracle as po
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    log(f"[bench] CPU baseline: oracle with host BVH, {threads} threads")
+    hc = host_cpu()
+    threads = hc["affinity_cores"]
+    if hc["omp_num_threads"] and hc["omp_num_threads"].isdigit():
+        threads = max(1, min(threads, int(hc["omp_num_threads"])))
+    log(f"[bench] CPU baseline: oracle with host BVH, {threads} threads on {hc['model']} "
+        f"(nproc {hc['nproc']}, affinity {hc['affinity_cores']})")
     os_ = po.build_oracle_scene(desc, bvh=True)
     prm = po.params_in(W, H, desc.maxBounces, desc.rrDepth, iterationCap=0)
     done_spp, secs = 0, 0.0
@@ -542,7 +611,9 @@ def cpu_baseline(desc, W, H, target_s):
         log(f"[bench] CPU baseline: {done_spp} spp in {secs:.2f} s")
     paths = W * H * done_spp
     return {"value": round(paths / secs / 1e6, 4), "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"{done_spp} spp of the same {W}x{H} frame (samples 0..{done_spp - 1}), full depth 16"}
+            "sample": f"{done_spp} spp of the same {W}x{H} frame (samples 0..{done_spp - 1}), full depth "
+                      f"{desc.maxBounces}, {secs:.1f} s",
+            "host": hc}
 
 
 if __name__ == "__main__":

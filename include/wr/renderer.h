@@ -66,6 +66,10 @@ struct RenderStats {
     uint32_t launchesTraceClosest = 0, launchesTraceAny = 0;
     // with WavefrontConfig::countTraversal
     uint64_t closestNodeFetches = 0, closestTriFetches = 0, anyNodeFetches = 0, anyTriFetches = 0;
+    // with WavefrontConfig::timeKernels: items the classify, partition and
+    // shade launches processed (queue entries; for shade the entries of the
+    // five material queues)
+    uint64_t classifyItems = 0, partitionItems = 0, shadeItems = 0;
 };
 
 struct RendererImpl;

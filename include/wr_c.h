@@ -36,6 +36,7 @@ typedef struct wr_c_stats {
     double msGenerate, msTraceClosest, msTraceAny, msClassify, msPartition, msShade, msAccumulate;
     uint32_t launchesTraceClosest, launchesTraceAny;
     uint64_t closestNodeFetches, closestTriFetches, anyNodeFetches, anyTriFetches;
+    uint64_t classifyItems, partitionItems, shadeItems;
 } wr_c_stats;
 
 const char* wr_c_last_error(void);

@@ -313,11 +313,12 @@ int wrk_shade(const wrk_scene_view* scene, const wrk_render_params* paramsTable,
 int wrk_slot_finish(const uint32_t* finished, uint32_t maxItems, const uint32_t* countDev, const wrk_paths* paths,
                     uint32_t* live, void* stream);
 /* The same count with the slot of pool entry p taken from slotMap[p / 256]
- * (one byte per 256 pool entries; chunks start and end on such granules),
- * so the scattered state words are not read. Every entry of `finished` must
- * lie below 256 times the map's length. */
+ * (numGranules bytes, one per 256 pool entries; chunks start and end on
+ * such granules), so the scattered state words are not read. maxItems may
+ * be at most 256 * numGranules; entries p >= 256 * numGranules are skipped
+ * (wrk_slot_finish likewise skips entries past paths->capacity). */
 int wrk_slot_finish_map(const uint32_t* finished, uint32_t maxItems, const uint32_t* countDev,
-                        const uint8_t* slotMap, uint32_t* live, void* stream);
+                        const uint8_t* slotMap, uint32_t numGranules, uint32_t* live, void* stream);
 /* accum[pixel] += result[poolBase + k] for the chunk's paths in sample order
  * (renderer.cpp:190 serial samples; per-pixel accumulation order is fixed).
  * With paths->guides and non-NULL accumAlbedo/accumNormal the first-hit

@@ -63,7 +63,8 @@ class wr_c_stats(C.Structure):
                 ("msClassify", C.c_double), ("msPartition", C.c_double), ("msShade", C.c_double),
                 ("msAccumulate", C.c_double), ("launchesTraceClosest", C.c_uint32),
                 ("launchesTraceAny", C.c_uint32), ("closestNodeFetches", C.c_uint64),
-                ("closestTriFetches", C.c_uint64), ("anyNodeFetches", C.c_uint64), ("anyTriFetches", C.c_uint64)]
+                ("closestTriFetches", C.c_uint64), ("anyNodeFetches", C.c_uint64), ("anyTriFetches", C.c_uint64),
+                ("classifyItems", C.c_uint64), ("partitionItems", C.c_uint64), ("shadeItems", C.c_uint64)]
 
 
 class wrk_paths(C.Structure):
@@ -191,7 +192,7 @@ _SIGS = {
     "wrk_trace_closest_categorized": (C.c_int, [C.POINTER(wrk_bvh), _P, _P, _P, C.c_uint32, _P, _P, _P, _P, _P]),
     "wrk_shade": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_slot_finish": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
-    "wrk_slot_finish_map": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
+    "wrk_slot_finish_map": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, _P, _P]),
     "wrk_accumulate": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P, _P, _P]),
     "wrk_average": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_tonemap_aces": (C.c_int, [_P, C.c_uint32, _P, _P]),

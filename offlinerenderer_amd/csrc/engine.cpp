@@ -72,6 +72,8 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
     hip_check(hipHostMalloc((void**)&hostOffs_, (16 + WRK_MAX_SLOTS) * sizeof(uint32_t), hipHostMallocDefault),
               "hipHostMalloc");
     hostLive_ = hostOffs_ + 16;
+    hip_check(hipHostMalloc((void**)&hostSceneOffs_, 8 * kMaxScenes * sizeof(uint32_t), hipHostMallocDefault),
+              "hipHostMalloc");
     hip_check(hipHostMalloc((void**)&hostPrm_, WRK_MAX_SLOTS * sizeof(wrk_render_params), hipHostMallocDefault),
               "hipHostMalloc");
     hip_check(hipHostMalloc((void**)&hostLiveInit_, WRK_MAX_SLOTS * sizeof(uint32_t), hipHostMallocDefault),
@@ -89,6 +91,7 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
 Engine::~Engine() {
     for (hipEvent_t e : eventPool_) (void)hipEventDestroy(e);
     if (hostOffs_) (void)hipHostFree(hostOffs_);
+    if (hostSceneOffs_) (void)hipHostFree(hostSceneOffs_);
     if (hostPrm_) (void)hipHostFree(hostPrm_);
     if (hostLiveInit_) (void)hipHostFree(hostLiveInit_);
     for (uint32_t s = 0; s < kMaxScenes; ++s) {
@@ -476,6 +479,8 @@ void Engine::launch() {
         }
         stats_.closestRays += nC;
         stats_.shadowRays += nA;
+        stats_.classifyItems += n;
+        stats_.partitionItems += n;
         timed(T_CLASSIFY, ss, [&] {
             wrk_check(wrk_classify(&view, prmTable, &paths_, qlive + a, nT, nTS, nS, cat + a, nC ? 1u : 0u, ss),
                       "wrk_classify");
@@ -551,7 +556,8 @@ void Engine::launch() {
         });
         // live paths per slot: subtract the paths that ended
         timed(T_PARTITION, stream_, [&] {
-            wrk_check(wrk_slot_finish_map(qa, items, off + 48 + 1, slotMap_.as<uint8_t>(), live, stream_),
+            wrk_check(wrk_slot_finish_map(qa, items, off + 48 + 1, slotMap_.as<uint8_t>(), capacity_ / 256, live,
+                                          stream_),
                       "wrk_slot_finish_map");
         });
     }
@@ -560,6 +566,13 @@ void Engine::launch() {
         hip_check(hipMemcpyAsync(hostOffs_, off + 48, (16 + WRK_MAX_SLOTS) * sizeof(uint32_t),
                                  hipMemcpyDeviceToHost, stream_),
                   "hipMemcpyAsync");
+    stats_.partitionItems += items;
+    // per-scene material-queue sizes (item counts for the kernel statistics)
+    if (cfg_.timeKernels && total)
+        hip_check(hipMemcpyAsync(hostSceneOffs_, off, 8 * kMaxScenes * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 stream_),
+                  "hipMemcpyAsync");
+    launchedTotal_ = total;
     launchedItems_ = items;
     launchedScenes_ = S2;
     launched_ = true;
@@ -577,6 +590,9 @@ bool Engine::finish() {
     launched_ = false;
     const uint32_t items = launchedItems_, S2 = launchedScenes_;
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    if (cfg_.timeKernels && launchedTotal_)
+        for (uint32_t s = 0; s < S2; ++s)
+            if (segT_[s] + segTS_[s] + segS_[s]) stats_.shadeItems += hostSceneOffs_[8 * s + 5];
     deadCount_ = items ? hostOffs_[1] : 0;
     for (uint32_t s = 0; s < S2; ++s) {
         const uint32_t* o = hostOffs_ + 1 + 3 * s;

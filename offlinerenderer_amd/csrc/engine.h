@@ -179,7 +179,8 @@ private:
     wrk_paths paths_{};
     uint32_t deadCount_ = 0;              // entries ended last iteration: qA_[0, deadCount_)
     bool launched_ = false;               // launch() done, finish() not yet
-    uint32_t launchedItems_ = 0, launchedScenes_ = 0;
+    uint32_t launchedItems_ = 0, launchedScenes_ = 0, launchedTotal_ = 0;
+    uint32_t* hostSceneOffs_ = nullptr;   // pinned: per-scene partition offsets (kernel statistics only)
     uint32_t* hostOffs_ = nullptr;        // pinned: queue offsets, then the live counts
     uint32_t* hostLive_ = nullptr;        // hostOffs_ + 16: live paths per slot, read back each step
     uint32_t* hostLiveInit_ = nullptr;    // pinned staging: initial live count per slot

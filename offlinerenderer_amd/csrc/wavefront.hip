@@ -624,8 +624,8 @@ template <bool MAP>
 __global__ __launch_bounds__(256) void k_slot_finish(const uint32_t* __restrict__ finished, uint32_t maxItems,
                                                      const uint32_t* __restrict__ countDev,
                                                      const uint4* __restrict__ st,
-                                                     const uint8_t* __restrict__ slotMap, uint32_t perThread,
-                                                     uint32_t* live) {
+                                                     const uint8_t* __restrict__ slotMap, uint32_t bound,
+                                                     uint32_t perThread, uint32_t* live) {
     __shared__ uint32_t hist[kMaxSlots];
     const uint32_t t = threadIdx.x;
     hist[t] = 0u;
@@ -645,9 +645,11 @@ __global__ __launch_bounds__(256) void k_slot_finish(const uint32_t* __restrict_
         uint32_t sl[kBatch];
 #pragma unroll
         for (uint32_t u = 0; u < kBatch; ++u) {
+            // entries past the map (or the pool) are skipped: a stale or
+            // corrupt queue entry must not read beyond the buffer
             if (p[u] == 0xFFFFFFFFu) sl[u] = 0xFFFFFFFFu;
-            else if constexpr (MAP) sl[u] = slotMap[p[u] >> 8];
-            else sl[u] = st[p[u]].z >> FLAG_SLOT_SHIFT;
+            else if constexpr (MAP) sl[u] = (p[u] >> 8) < bound ? (uint32_t)slotMap[p[u] >> 8] : 0xFFFFFFFFu;
+            else sl[u] = p[u] < bound ? st[p[u]].z >> FLAG_SLOT_SHIFT : 0xFFFFFFFFu;
         }
 #pragma unroll
         for (uint32_t u = 0; u < kBatch; ++u) {
@@ -768,21 +770,23 @@ extern "C" int wrk_slot_finish(const uint32_t* finished, uint32_t maxItems, cons
     perThread = perThread < 1u ? 1u : (perThread > 256u ? 256u : perThread);
     uint32_t blocks = ceil_div(maxItems, 256ull * perThread);
     hipLaunchKernelGGL(k_slot_finish<false>, dim3(blocks), dim3(256), 0, as_stream(stream), finished, maxItems,
-                       countDev, (const uint4*)paths->state, nullptr, perThread, live);
+                       countDev, (const uint4*)paths->state, nullptr, paths->capacity, perThread, live);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }
 
 extern "C" int wrk_slot_finish_map(const uint32_t* finished, uint32_t maxItems, const uint32_t* countDev,
-                                   const uint8_t* slotMap, uint32_t* live, void* stream) {
+                                   const uint8_t* slotMap, uint32_t numGranules, uint32_t* live, void* stream) {
     WRK_REQUIRE(slotMap && live, "wrk_slot_finish_map: null argument");
     if (maxItems == 0) return WRK_OK;
     WRK_REQUIRE(finished, "wrk_slot_finish_map: null argument");
+    WRK_REQUIRE((uint64_t)maxItems <= 256ull * numGranules,
+                "wrk_slot_finish_map: more finished entries than the slot map covers");
     uint32_t perThread = ceil_div(maxItems, 256ull * 2048ull);
     perThread = perThread < 1u ? 1u : (perThread > 256u ? 256u : perThread);
     uint32_t blocks = ceil_div(maxItems, 256ull * perThread);
     hipLaunchKernelGGL(k_slot_finish<true>, dim3(blocks), dim3(256), 0, as_stream(stream), finished, maxItems,
-                       countDev, nullptr, slotMap, perThread, live);
+                       countDev, nullptr, slotMap, numGranules, perThread, live);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }

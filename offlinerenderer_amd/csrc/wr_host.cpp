@@ -701,6 +701,7 @@ const RenderStats& RenderServer::getStats() const {
         t.launchesTraceClosest += x.launchesTraceClosest; t.launchesTraceAny += x.launchesTraceAny;
         t.closestNodeFetches += x.closestNodeFetches; t.closestTriFetches += x.closestTriFetches;
         t.anyNodeFetches += x.anyNodeFetches; t.anyTriFetches += x.anyTriFetches;
+        t.classifyItems += x.classifyItems; t.partitionItems += x.partitionItems; t.shadeItems += x.shadeItems;
     }
     return s.stats;
 }

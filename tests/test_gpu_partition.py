@@ -85,24 +85,33 @@ def test_slot_finish_matches_histogram(gpu, n):
 
 @pytest.mark.parametrize("n", [0, 1, 1000, 3000017])
 def test_slot_finish_map_matches_histogram(gpu, n):
-    """wrk_slot_finish_map: the slot of pool entry p is slotMap[p >> 8]."""
+    """wrk_slot_finish_map: the slot of pool entry p is slotMap[p >> 8];
+    entries past the map are skipped (a few are planted), and more entries
+    than the map covers are refused."""
     rng = np.random.default_rng(4321 + n)
     granules = max((2 * n + 255) // 256, 1)
     slot_map = rng.integers(0, 256, size=granules).astype(np.uint8)
     slot_map[: granules // 2] = np.sort(slot_map[: granules // 2])
     finished = rng.permutation(granules * 256)[:n].astype(np.uint32)
     finished[: n // 2] = np.sort(finished[: n // 2])
+    if n > 10:
+        finished[[3, n // 2, n - 7]] = [granules * 256, granules * 256 + 999, 0xFFFFFF00]
     m = max(n - 5, 0)  # device-side count shorter than the launch size
     live0 = rng.integers(1 << 20, 1 << 24, size=256).astype(np.uint32)
     q = G.dev(finished if n else np.zeros(1, np.uint32))
     cnt = G.dev(np.array([m], np.uint32))
     live = G.dev(live0)
     smap = G.dev(slot_map)
-    G.check(wr.lib().wrk_slot_finish_map(G.vp(q), n, G.vp(cnt), G.vp(smap), G.vp(live), G.stream()))
+    G.check(wr.lib().wrk_slot_finish_map(G.vp(q), n, G.vp(cnt), G.vp(smap), granules, G.vp(live), G.stream()))
     torch.cuda.synchronize()
     got = live.cpu().numpy().view(np.uint32)
-    want = live0 - np.bincount(slot_map[finished[:m] >> 8], minlength=256).astype(np.uint32)
+    inside = finished[:m][(finished[:m] >> 8) < granules]
+    want = live0 - np.bincount(slot_map[inside >> 8], minlength=256).astype(np.uint32)
     assert np.array_equal(got, want)
+    if n:
+        rc = wr.lib().wrk_slot_finish_map(G.vp(q), 256 * granules + 1, G.vp(cnt), G.vp(smap), granules, G.vp(live),
+                                          G.stream())
+        assert rc != 0 and b"slot map" in wr.lib().wrk_last_error()
 
 
 
