@@ -68,7 +68,8 @@ struct RenderStats {
     uint64_t closestNodeFetches = 0, closestTriFetches = 0, anyNodeFetches = 0, anyTriFetches = 0;
     // with WavefrontConfig::timeKernels: items the classify, partition and
     // shade launches processed (queue entries; for shade the entries of the
-    // five material queues)
+    // five material queues). The wavefront engine resolves trace results in
+    // the traversal kernels, so msClassify and classifyItems stay 0 there.
     uint64_t classifyItems = 0, partitionItems = 0, shadeItems = 0;
 };
 

@@ -279,6 +279,28 @@ uint32_t wrk_partition_noalloc_max_items(void);
 int wrk_generate(const wrk_camera* cam, const wrk_render_params* params, const wrk_paths* paths, uint64_t first,
                  uint32_t n, uint32_t poolBase, uint32_t slot, uint32_t* queue, uint8_t* category,
                  uint32_t categoryValue, void* stream);
+/* The traversals with the per-entry decisions of wrk_classify taken where
+ * each ray ends, so no classify pass runs (the engine's path). Run
+ * wrk_trace_any_resolve before wrk_trace_closest_resolve of the same
+ * iteration and queue part, so a trace + shadow entry adds its shadow
+ * contribution before a miss adds the environment (trace.cu:27-51 before
+ * trace.cu:143-148, as in libWR's stage order).
+ * Any hit over queue entries [0, numTraceShadow + numShadow) of the shadow
+ * rays in paths->savedOrg / savedDir: an unoccluded ray adds its pending
+ * contribution to the path's radiance; for entries i >= numTraceShadow
+ * (shadow only) categories[i] = 0 (the path resumes as Lambertian, with the
+ * reference-mode iteration cap) or 5 (flushed: result = clamped radiance).
+ * Closest hit over queue entries [0, n) of the rays in paths->org / dir: a
+ * hit writes paths->hit and categories[i] = the triangle's shade category
+ * (tagged records, else from the scene's material table); a miss adds
+ * throughput x environment radiance, flushes and writes categories[i] = 5.
+ * paramsTable is the device table of per-slot render parameters. */
+int wrk_trace_any_resolve(const wrk_bvh* bvh, const wrk_render_params* paramsTable, const wrk_paths* paths,
+                          const uint32_t* queue, uint32_t numTraceShadow, uint32_t numShadow, uint8_t* categories,
+                          unsigned long long* fetches, void* stream);
+int wrk_trace_closest_resolve(const wrk_bvh* bvh, const wrk_scene_view* scene, const wrk_render_params* paramsTable,
+                              const wrk_paths* paths, const uint32_t* queue, uint32_t n, uint8_t* categories,
+                              unsigned long long* fetches, void* stream);
 /* Resolves the trace results of the active queue of one scene,
  * [trace numTrace | trace + shadow numTraceShadow | shadow numShadow]
  * (closest-hit results for the first two parts, any-hit results of the

@@ -188,6 +188,8 @@ _SIGS = {
     "wrk_generate": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32,
                                _P]),
     "wrk_classify": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_uint32, _P]),
+    "wrk_trace_any_resolve": (C.c_int, [C.POINTER(wrk_bvh), _P, _P, _P, C.c_uint32, C.c_uint32, _P, _P, _P]),
+    "wrk_trace_closest_resolve": (C.c_int, [C.POINTER(wrk_bvh), _P, _P, _P, _P, C.c_uint32, _P, _P, _P]),
     "wrk_bvh_tag_categories": (C.c_int, [C.POINTER(wrk_bvh), _P, _P]),
     "wrk_trace_closest_categorized": (C.c_int, [C.POINTER(wrk_bvh), _P, _P, _P, C.c_uint32, _P, _P, _P, _P, _P]),
     "wrk_shade": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]),

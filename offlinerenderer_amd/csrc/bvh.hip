@@ -656,7 +656,67 @@ __global__ __launch_bounds__(256) void k_reorder_tris(const uint32_t* __restrict
 #ifndef WRK_LEAF_BATCH_DEN
 #define WRK_LEAF_BATCH_DEN 2
 #endif
-template <bool ANY, bool COUNT, int W>
+// What a finished ray does to its path when the traversal also resolves the
+// result (the per-entry decisions of wrk_classify, taken where the ray ends
+// instead of in a pass of their own). Pool arrays are indexed by the pool
+// entry the queue names; categories by queue position.
+struct TraceResolve {
+    float4* rad;
+    const float4* thr;
+    const float4* contrib;
+    const uint4* st;
+    float4* result;
+    const wrk_render_params* prm;  // indexed by the path's slot
+    uint8_t* cat;
+    const uint32_t* triMat;        // for hits on untagged triangles
+    const wrk_material* materials;
+    uint32_t numMaterials;
+    uint32_t numTraceShadow;       // any hit: entries past this are shadow-only
+    float env[3];
+};
+
+// Shadow ray done (trace.cu:27-51): an unoccluded ray adds the pending
+// contribution; a shadow-only entry then resumes as a Lambertian path
+// (reference mode: the continuation waited for the result) or is flushed.
+__device__ __forceinline__ void resolve_shadow(const TraceResolve& R, uint32_t p, uint32_t qi, bool occluded) {
+    if (!occluded) {
+        const float4 L = R.rad[p];
+        const float4 C = R.contrib[p];
+        R.rad[p] = make_float4(L.x + C.x, L.y + C.y, L.z + C.z, L.w);
+    }
+    if (qi >= R.numTraceShadow) {
+        const uint32_t z = R.st[p].z;
+        if (z & FLAG_NEE_DONE) {
+            R.cat[qi] = (uint8_t)WRK_MAT_LAMBERTIAN;
+        } else {
+            const float4 L = R.rad[p];
+            R.result[p] = f4(clamp_fireflies(mk(L.x, L.y, L.z), R.prm[z >> FLAG_SLOT_SHIFT].fireflyClamp), 0.0f);
+            R.cat[qi] = (uint8_t)kCatDone;
+        }
+    }
+}
+
+// Closest-hit ray done: a hit's shade category, or for a miss the
+// environment term (trace.cu:143-145) and the flush (trace.cu:146-148).
+__device__ __forceinline__ void resolve_closest(const TraceResolve& R, uint2* hits, uint32_t p, uint32_t qi,
+                                                uint32_t bestPrim, float tbest) {
+    if (bestPrim != 0xFFFFFFFFu) {
+        const uint32_t prim = bestPrim & kPrimMask;
+        hits[p] = make_uint2(__float_as_uint(tbest), prim);
+        const uint32_t tag = bestPrim >> 28;
+        R.cat[qi] = (uint8_t)(tag != 0u ? tag - 1u : shade_category(R.triMat[prim], R.numMaterials, R.materials));
+    } else {
+        const float4 T = R.thr[p], L = R.rad[p];
+        const uint32_t slot = R.st[p].z >> FLAG_SLOT_SHIFT;
+        const f3 Lr = mk(L.x + T.x * R.env[0], L.y + T.y * R.env[1], L.z + T.z * R.env[2]);
+        const f3 Lc = clamp_fireflies(Lr, R.prm[slot].fireflyClamp);
+        R.rad[p] = f4(Lr, 0.0f);
+        R.result[p] = f4(Lc, 0.0f);
+        R.cat[qi] = (uint8_t)kCatDone;
+    }
+}
+
+template <bool ANY, bool COUNT, int W, bool RESOLVE>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(WRK_TRACE_WAVES, WRK_TRACE_WAVES)))
 void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
@@ -664,10 +724,10 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                                                const uint32_t* __restrict__ queue, uint32_t n,
                                                const uint32_t* __restrict__ countDev, uint2* __restrict__ hits,
                                                uint8_t* __restrict__ occl, unsigned long long* __restrict__ fetches,
-                                               uint32_t raysPerLane) {
+                                               uint32_t raysPerLane, TraceResolve R) {
     __shared__ uint32_t sNode[kStackLds][256];
     __shared__ float sT[kStackLds][256];
-    __shared__ uint32_t sQi[ANY ? 1 : 256];  // closest hit: queue position of each lane's ray
+    __shared__ uint32_t sQi[(ANY && !RESOLVE) ? 1 : 256];  // queue position of each lane's ray
     const uint32_t lane = threadIdx.x;
     uint32_t cnt = n;
     if (countDev) cnt = min(cnt, *countDev);
@@ -700,7 +760,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
             if (!active && qi < chunkEnd) {
                 ray = queue ? queue[qi] : (uint32_t)qi;
-                if (!ANY && occl) sQi[lane] = qi;
+                if (RESOLVE || (!ANY && occl)) sQi[lane] = qi;
                 o = org[ray];
                 float4 d = dir[ray];
                 tmin = o.w;
@@ -884,8 +944,12 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
             }
             if (leaf != kEmptyChild) done = false;
             if (done) {
-                if (ANY) occl[ray] = found ? 1 : 0;
-                else {
+                if constexpr (RESOLVE) {
+                    if constexpr (ANY) resolve_shadow(R, ray, sQi[lane], found);
+                    else resolve_closest(R, hits, ray, sQi[lane], bestPrim, tbest);
+                } else if (ANY) {
+                    occl[ray] = found ? 1 : 0;
+                } else {
                     const bool hit = bestPrim != 0xFFFFFFFFu;
                     hits[ray] = make_uint2(__float_as_uint(tbest), hit ? (bestPrim & kPrimMask) : 0xFFFFFFFFu);
                     if (occl) {
@@ -1156,7 +1220,7 @@ static uint32_t rays_per_lane(uint32_t n) {
 template <bool ANY>
 static int launch_trace(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue, uint32_t n,
                         const uint32_t* countDev, uint32_t* hits, uint8_t* occl, unsigned long long* fetches,
-                        void* stream) {
+                        void* stream, const TraceResolve* resolve = nullptr) {
     WRK_REQUIRE(bvh && bvh->nodes && bvh->tris && org4 && dir4, "wrk_trace: null argument");
     WRK_REQUIRE(bvh->nodeBytes == sizeof(Node8), "wrk_trace: unknown node format");
     WRK_REQUIRE((kBvhWidth - 1u) * bvh->maxDepth + 8u <= kStackTotal, "wrk_trace: BVH deeper than the traversal stack");
@@ -1166,14 +1230,26 @@ static int launch_trace(const wrk_bvh* bvh, const void* org4, const void* dir4, 
     dim3 grid(ceil_div(n, 256u * rpl)), block(256);
     hipStream_t s = as_stream(stream);
     const void* nd = (const void*)bvh->nodes;
-    if (fetches)
-        hipLaunchKernelGGL((k_trace<ANY, true, 8>), grid, block, 0, s, nd, (const float4*)bvh->tris,
-                           (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl, fetches,
-                           rpl);
-    else
-        hipLaunchKernelGGL((k_trace<ANY, false, 8>), grid, block, 0, s, nd, (const float4*)bvh->tris,
-                           (const float4*)org4, (const float4*)dir4, queue, n, countDev, (uint2*)hits, occl, nullptr,
-                           rpl);
+    const float4* tr = (const float4*)bvh->tris;
+    const float4* o4 = (const float4*)org4;
+    const float4* d4 = (const float4*)dir4;
+    uint2* h2 = (uint2*)hits;
+    if (resolve) {
+        if (fetches)
+            hipLaunchKernelGGL((k_trace<ANY, true, 8, true>), grid, block, 0, s, nd, tr, o4, d4, queue, n, countDev,
+                               h2, occl, fetches, rpl, *resolve);
+        else
+            hipLaunchKernelGGL((k_trace<ANY, false, 8, true>), grid, block, 0, s, nd, tr, o4, d4, queue, n, countDev,
+                               h2, occl, nullptr, rpl, *resolve);
+    } else {
+        const TraceResolve none{};
+        if (fetches)
+            hipLaunchKernelGGL((k_trace<ANY, true, 8, false>), grid, block, 0, s, nd, tr, o4, d4, queue, n, countDev,
+                               h2, occl, fetches, rpl, none);
+        else
+            hipLaunchKernelGGL((k_trace<ANY, false, 8, false>), grid, block, 0, s, nd, tr, o4, d4, queue, n, countDev,
+                               h2, occl, nullptr, rpl, none);
+    }
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }
@@ -1224,6 +1300,53 @@ extern "C" int wrk_trace_closest_counted(const wrk_bvh* bvh, const void* org4, c
                                          unsigned long long* fetches, void* stream) {
     WRK_REQUIRE(hits && fetches, "wrk_trace_closest_counted: null argument");
     return launch_trace<false>(bvh, org4, dir4, queue, n, countDev, hits, nullptr, fetches, stream);
+}
+
+static TraceResolve resolve_of(const wrk_scene_view* scene, const wrk_render_params* paramsTable,
+                                const wrk_paths* paths, uint8_t* categories, uint32_t numTraceShadow) {
+    TraceResolve R{};
+    R.rad = (float4*)paths->rad;
+    R.thr = (const float4*)paths->thr;
+    R.contrib = (const float4*)paths->contrib;
+    R.st = (const uint4*)paths->state;
+    R.result = (float4*)paths->result;
+    R.prm = paramsTable;
+    R.cat = categories;
+    R.numTraceShadow = numTraceShadow;
+    if (scene) {
+        R.triMat = (const uint32_t*)scene->triMaterialIds;
+        R.materials = (const wrk_material*)scene->materials;
+        R.numMaterials = scene->numMaterials;
+        for (int a = 0; a < 3; ++a) R.env[a] = scene->envRadiance[a];
+    }
+    return R;
+}
+
+extern "C" int wrk_trace_closest_resolve(const wrk_bvh* bvh, const wrk_scene_view* scene,
+                                         const wrk_render_params* paramsTable, const wrk_paths* paths,
+                                         const uint32_t* queue, uint32_t n, uint8_t* categories,
+                                         unsigned long long* fetches, void* stream) {
+    WRK_REQUIRE(scene && paramsTable && paths && queue && categories, "wrk_trace_closest_resolve: null argument");
+    WRK_REQUIRE(paths->org && paths->dir && paths->hit && paths->rad && paths->thr && paths->state && paths->result,
+                "wrk_trace_closest_resolve: incomplete path pool");
+    WRK_REQUIRE(scene->triMaterialIds && (scene->numMaterials == 0 || scene->materials),
+                "wrk_trace_closest_resolve: incomplete scene view");
+    const TraceResolve R = resolve_of(scene, paramsTable, paths, categories, 0);
+    return launch_trace<false>(bvh, (const void*)paths->org, (const void*)paths->dir, queue, n, nullptr,
+                               (uint32_t*)paths->hit, nullptr, fetches, stream, &R);
+}
+
+extern "C" int wrk_trace_any_resolve(const wrk_bvh* bvh, const wrk_render_params* paramsTable,
+                                     const wrk_paths* paths, const uint32_t* queue, uint32_t numTraceShadow,
+                                     uint32_t numShadow, uint8_t* categories, unsigned long long* fetches,
+                                     void* stream) {
+    WRK_REQUIRE(paramsTable && paths && queue && categories, "wrk_trace_any_resolve: null argument");
+    WRK_REQUIRE(paths->savedOrg && paths->savedDir && paths->rad && paths->contrib && paths->state && paths->result,
+                "wrk_trace_any_resolve: incomplete path pool");
+    WRK_REQUIRE((uint64_t)numTraceShadow + numShadow <= 0xFFFFFFFFull, "wrk_trace_any_resolve: too many rays");
+    const TraceResolve R = resolve_of(nullptr, paramsTable, paths, categories, numTraceShadow);
+    return launch_trace<true>(bvh, (const void*)paths->savedOrg, (const void*)paths->savedDir, queue,
+                              numTraceShadow + numShadow, nullptr, nullptr, nullptr, fetches, stream, &R);
 }
 
 extern "C" int wrk_trace_any_counted(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue,

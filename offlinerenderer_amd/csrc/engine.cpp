@@ -453,38 +453,31 @@ void Engine::launch() {
         const wrk_bvh& bvh = scenes_[s]->getBVH();
         const wrk_scene_view& view = scenes_[s]->getDeviceView();
         const uint32_t nC = nT + nTS, nA = nTS + nS;
-        if (nC) {
-            timed(T_CLOSEST, ss, [&] {
-                // hits plus the shade category of tagged triangles, written
-                // at the queue positions classify reads
-                wrk_check(wrk_trace_closest_categorized(&bvh, org_.ptr, dir_.ptr, qlive + a, nC, nullptr,
-                                                        hit_.as<uint32_t>(), cat + a,
-                                                        cfg_.countTraversal ? fc : nullptr, ss),
-                          "wrk_trace_closest_categorized");
-            });
-            stats_.launchesTraceClosest++;
-        }
+        // Each traversal resolves its rays where they end (the decisions of
+        // wrk_classify): shadow rays first, adding unoccluded contributions
+        // and placing the shadow-only entries; then the closest-hit rays,
+        // whose misses add the environment and flush after those
+        // contributions (libWR's Shadow -> Shade -> Trace order), and whose
+        // hits write their shade category.
         if (nA) {
             timed(T_ANY, ss, [&] {
-                if (cfg_.countTraversal)
-                    wrk_check(wrk_trace_any_counted(&bvh, savedOrg_.ptr, savedDir_.ptr, qlive + a + nT, nA, nullptr,
-                                                    occl_.as<uint8_t>(), fc + 2, ss),
-                              "wrk_trace_any_counted");
-                else
-                    wrk_check(wrk_trace_any(&bvh, savedOrg_.ptr, savedDir_.ptr, qlive + a + nT, nA, nullptr,
-                                            occl_.as<uint8_t>(), ss),
-                              "wrk_trace_any");
+                wrk_check(wrk_trace_any_resolve(&bvh, prmTable, &paths_, qlive + a + nT, nTS, nS, cat + a + nT,
+                                                cfg_.countTraversal ? fc + 2 : nullptr, ss),
+                          "wrk_trace_any_resolve");
             });
             stats_.launchesTraceAny++;
         }
+        if (nC) {
+            timed(T_CLOSEST, ss, [&] {
+                wrk_check(wrk_trace_closest_resolve(&bvh, &view, prmTable, &paths_, qlive + a, nC, cat + a,
+                                                    cfg_.countTraversal ? fc : nullptr, ss),
+                          "wrk_trace_closest_resolve");
+            });
+            stats_.launchesTraceClosest++;
+        }
         stats_.closestRays += nC;
         stats_.shadowRays += nA;
-        stats_.classifyItems += n;
         stats_.partitionItems += n;
-        timed(T_CLASSIFY, ss, [&] {
-            wrk_check(wrk_classify(&view, prmTable, &paths_, qlive + a, nT, nTS, nS, cat + a, nC ? 1u : 0u, ss),
-                      "wrk_classify");
-        });
         timed(T_PARTITION, ss, [&] {
             // five material queues + the paths classify finished
             wrk_check(wrk_partition(cat + a, qlive + a, n, 6, qb + a, off + 8 * s, sceneScratch_[s].ptr,

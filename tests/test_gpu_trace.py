@@ -159,3 +159,92 @@ def test_closest_hit_categories(gpu):
     assert np.array_equal(got[hit], want)
     assert np.all(got[~hit] == 0xFF)
     assert len(np.unique(want)) >= 3
+
+
+@pytest.mark.parametrize("tagged", [True, False])
+def test_resolving_traversals_match_trace_then_classify(gpu, tagged):
+    """wrk_trace_any_resolve + wrk_trace_closest_resolve leave the path pool
+    (radiance, flushed results, hit records of hits) and the categories
+    exactly as wrk_trace_any + wrk_trace_closest_categorized + wrk_classify
+    do, for a random pool state: trace-only, trace + shadow and shadow-only
+    entries, occluded and unoccluded shadow rays, misses, firefly clamps per
+    slot, NEE-done flags; with tagged triangle records and without."""
+    import ctypes as C
+
+    import torch
+
+    import offlinerenderer_amd as wr
+
+    d = scenes.material_scene(96, 54, 1, tex_size=64)
+    ps = G.product_scene(d)
+    bvh = ps.getBVH()
+    view = ps.getDeviceView()
+    cap = 300_000
+    rng = np.random.default_rng(7 + tagged)
+    org4, dir4 = make_rays(d, cap, 21)
+    sorg, sdir = make_rays(d, cap, 22)
+    sdir[:, 3] = np.where(rng.uniform(size=cap) < 0.5, sdir[:, 3], rng.uniform(0.05, 20.0, cap))
+    f = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()  # noqa: E731
+    state = np.zeros((cap, 4), np.uint32)
+    state[:, 0] = rng.integers(0, 1 << 32, cap, dtype=np.uint64)
+    slots = rng.integers(0, 4, cap).astype(np.uint32)
+    nee = (rng.uniform(size=cap) < 0.3).astype(np.uint32) << 18
+    state[:, 2] = (slots << 24) | nee | rng.integers(0, 8, cap).astype(np.uint32)
+    thr, rad, con = (rng.uniform(0, 3, (cap, 4)).astype(np.float32) for _ in range(3))
+    prm = np.zeros((256, 8), np.uint32)
+    prm.view(np.float32)[:4, 5] = [0.0, 0.5, 2.0, 0.0]  # fireflyClamp per slot
+    n = 240_000
+    nT, nTS, nS = 120_000, 70_000, 50_000
+    queue = rng.permutation(cap)[:n].astype(np.uint32)
+
+    def pool():
+        bufs = dict(org=f(org4), dir=f(dir4), thr=f(thr), rad=f(rad), state=G.dev(state.view(np.int32)),
+                    contrib=f(con), savedOrg=f(sorg), savedDir=f(sdir),
+                    hit=torch.full((cap, 2), -1, dtype=torch.int32, device="cuda"),
+                    occluded=torch.zeros(cap, dtype=torch.uint8, device="cuda"),
+                    result=torch.zeros((cap, 4), dtype=torch.float32, device="cuda"))
+        P = wr.wrk_paths()
+        for k, t in bufs.items():
+            setattr(P, k, t.data_ptr())
+        P.capacity = cap
+        return P, bufs
+
+    b = wr.wrk_bvh.from_buffer_copy(bvh)
+    if not tagged:
+        # untagged triangle records: the category comes from the material table
+        raw = torch.zeros(b.numTris * 12, dtype=torch.float32, device="cuda")
+        import tests.test_gpu_bvh as TB
+        recs = TB.d2h(b.tris, b.numTris * 48).view(np.float32).reshape(b.numTris, 3, 4).copy()
+        recs[:, 1, 3] = 0.0
+        raw = torch.from_numpy(recs.ravel()).cuda()
+        b.tris = raw.data_ptr()
+    prm_d = G.dev(prm.view(np.int32))
+    q = G.dev(queue)
+    L = wr.lib()
+
+    P1, B1 = pool()
+    cat1 = torch.full((n,), 0x77, dtype=torch.uint8, device="cuda")
+    G.check(L.wrk_trace_any(C.byref(b), C.c_void_p(P1.savedOrg), C.c_void_p(P1.savedDir), G.vp(q[nT:]), nTS + nS,
+                            None, C.c_void_p(P1.occluded), G.stream()))
+    G.check(L.wrk_trace_closest_categorized(C.byref(b), C.c_void_p(P1.org), C.c_void_p(P1.dir), G.vp(q), nT + nTS,
+                                            None, C.c_void_p(P1.hit), G.vp(cat1), None, G.stream()))
+    G.check(L.wrk_classify(C.byref(view), G.vp(prm_d), C.byref(P1), G.vp(q), nT, nTS, nS, G.vp(cat1), 1,
+                           G.stream()))
+    P2, B2 = pool()
+    cat2 = torch.full((n,), 0x77, dtype=torch.uint8, device="cuda")
+    G.check(L.wrk_trace_any_resolve(C.byref(b), G.vp(prm_d), C.byref(P2), G.vp(q[nT:]), nTS, nS,
+                                    C.c_void_p(cat2.data_ptr() + nT), None, G.stream()))
+    G.check(L.wrk_trace_closest_resolve(C.byref(b), C.byref(view), G.vp(prm_d), C.byref(P2), G.vp(q), nT + nTS,
+                                        G.vp(cat2), None, G.stream()))
+    torch.cuda.synchronize()
+    c1, c2 = cat1.cpu().numpy(), cat2.cpu().numpy()
+    assert np.array_equal(c1, c2)
+    for k in ("rad", "result"):
+        assert np.array_equal(B1[k].cpu().numpy().view(np.uint32), B2[k].cpu().numpy().view(np.uint32)), k
+    hit1, hit2 = B1["hit"].cpu().numpy().view(np.uint32), B2["hit"].cpu().numpy().view(np.uint32)
+    hits = c1 < 5
+    hq = queue[:nT + nTS][hits[:nT + nTS]]
+    assert np.array_equal(hit1[hq], hit2[hq])
+    # the mix is exercised: misses, hits of several kinds, resumed and flushed shadow-only entries
+    assert (c1[:nT + nTS] == 5).sum() > 1000 and len(np.unique(c1[:nT + nTS])) >= 4
+    assert (c1[nT + nTS:] == 0).sum() > 1000 and (c1[nT + nTS:] == 5).sum() > 1000
