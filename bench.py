@@ -186,8 +186,10 @@ def measure_render(args, workload_kind, W, H, spp, steps, warmup):
     tris_per_ray = cs["closestTriFetches"] / max(cs["closestRays"], 1)
     any_nodes = cs["anyNodeFetches"] / max(cs["shadowRays"], 1)
     any_tris = cs["anyTriFetches"] / max(cs["shadowRays"], 1)
-    log(f"[bench] closest: {nodes_per_ray:.2f} nodes, {tris_per_ray:.2f} tris per ray; "
-        f"any: {any_nodes:.2f} nodes, {any_tris:.2f} tris per ray")
+    spills_per_ray = cs["closestStackSpills"] / max(cs["closestRays"], 1)
+    any_spills = cs["anyStackSpills"] / max(cs["shadowRays"], 1)
+    log(f"[bench] closest: {nodes_per_ray:.2f} nodes, {tris_per_ray:.2f} tris, {spills_per_ray:.4f} stack spills "
+        f"per ray; any: {any_nodes:.2f} nodes, {any_tris:.2f} tris, {any_spills:.4f} spills per ray")
 
     r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=args.max_paths,
                                             maxChunkPaths=args.max_chunk or 0, timeKernels=True))
@@ -289,6 +291,8 @@ def measure_render(args, workload_kind, W, H, spp, steps, warmup):
             "bytes_per_ray": round(bytes_per_ray, 2),
             "nodes_per_ray": round(nodes_per_ray, 3),
             "tris_per_ray": round(tris_per_ray, 3),
+            "stack_spills_per_ray": round(spills_per_ray, 5),
+            "any_stack_spills_per_ray": round(any_spills, 5),
             "avg_launch_ms": round(avg_launch_ms, 4),
             "rays_per_launch": round(rays_c / max(launches_c, 1), 1),
             "any_hit_gbs": round(any_gbs, 2),

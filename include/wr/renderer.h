@@ -64,8 +64,10 @@ struct RenderStats {
     double msGenerate = 0, msTraceClosest = 0, msTraceAny = 0, msClassify = 0, msPartition = 0, msShade = 0,
            msAccumulate = 0;
     uint32_t launchesTraceClosest = 0, launchesTraceAny = 0;
-    // with WavefrontConfig::countTraversal
+    // with WavefrontConfig::countTraversal (stack spills: traversal stack
+    // entries moved from LDS to the per-lane scratch area)
     uint64_t closestNodeFetches = 0, closestTriFetches = 0, anyNodeFetches = 0, anyTriFetches = 0;
+    uint64_t closestStackSpills = 0, anyStackSpills = 0;
     // with WavefrontConfig::timeKernels: items the classify, partition and
     // shade launches processed (queue entries; for shade the entries of the
     // five material queues). The wavefront engine resolves trace results in

@@ -37,6 +37,7 @@ typedef struct wr_c_stats {
     uint32_t launchesTraceClosest, launchesTraceAny;
     uint64_t closestNodeFetches, closestTriFetches, anyNodeFetches, anyTriFetches;
     uint64_t classifyItems, partitionItems, shadeItems;
+    uint64_t closestStackSpills, anyStackSpills;
 } wr_c_stats;
 
 const char* wr_c_last_error(void);

@@ -206,7 +206,9 @@ int wrk_trace_closest_categorized(const wrk_bvh* bvh, const void* org4, const vo
                                   unsigned long long* fetches, void* stream);
 
 /* The same traversals built with fetch counters: fetches[0] += inner nodes
- * visited, fetches[1] += triangles tested (device uint64 pair). They give the
+ * visited, fetches[1] += triangles tested, fetches[2] += stack entries moved
+ * from the LDS part of the traversal stack to the per-lane scratch spill
+ * (device uint64[3]). They give the
  * mean node/triangle fetches per ray that the algorithmic-bytes model of the
  * traversal kernels needs; results are identical to the uncounted kernels. */
 int wrk_trace_closest_counted(const wrk_bvh* bvh, const void* org4, const void* dir4, const uint32_t* queue,

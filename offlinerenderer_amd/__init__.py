@@ -64,7 +64,8 @@ class wr_c_stats(C.Structure):
                 ("msAccumulate", C.c_double), ("launchesTraceClosest", C.c_uint32),
                 ("launchesTraceAny", C.c_uint32), ("closestNodeFetches", C.c_uint64),
                 ("closestTriFetches", C.c_uint64), ("anyNodeFetches", C.c_uint64), ("anyTriFetches", C.c_uint64),
-                ("classifyItems", C.c_uint64), ("partitionItems", C.c_uint64), ("shadeItems", C.c_uint64)]
+                ("classifyItems", C.c_uint64), ("partitionItems", C.c_uint64), ("shadeItems", C.c_uint64),
+                ("closestStackSpills", C.c_uint64), ("anyStackSpills", C.c_uint64)]
 
 
 class wrk_paths(C.Structure):

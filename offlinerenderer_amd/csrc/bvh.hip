@@ -749,7 +749,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
     RayPre rp;
     rp.kx = 0; rp.ky = 1; rp.kz = 2; rp.Sx = rp.Sy = rp.Sz = 0.0f; rp.o[0] = rp.o[1] = rp.o[2] = 0.0f;
     bool found = false;
-    uint32_t nodeFetches = 0, triFetches = 0;
+    uint32_t nodeFetches = 0, triFetches = 0, stackSpills = 0;
 
     for (;;) {
         // refill lanes without a ray from the wave's chunk
@@ -905,6 +905,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                 for (int c = W - 1; c >= 1; --c) {
                     if (key[c] != INFINITY) {
                         if (sp >= kStackLds) {
+                            if (COUNT) ++stackSpills;  // the LDS ring's oldest entry moves to scratch
                             uint32_t slot = sp & (kStackLds - 1);
                             spillN[sp - kStackLds] = sNode[slot][lane];
                             spillT[sp - kStackLds] = sT[slot][lane];
@@ -967,6 +968,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
     if (COUNT) {
         atomicAdd(&fetches[0], (unsigned long long)nodeFetches);
         atomicAdd(&fetches[1], (unsigned long long)triFetches);
+        atomicAdd(&fetches[2], (unsigned long long)stackSpills);
     }
 }
 

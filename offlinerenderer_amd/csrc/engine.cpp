@@ -462,7 +462,7 @@ void Engine::launch() {
         if (nA) {
             timed(T_ANY, ss, [&] {
                 wrk_check(wrk_trace_any_resolve(&bvh, prmTable, &paths_, qlive + a + nT, nTS, nS, cat + a + nT,
-                                                cfg_.countTraversal ? fc + 2 : nullptr, ss),
+                                                cfg_.countTraversal ? fc + 4 : nullptr, ss),
                           "wrk_trace_any_resolve");
             });
             stats_.launchesTraceAny++;
@@ -622,10 +622,10 @@ bool Engine::finish() {
         }
     }
     if (cfg_.countTraversal) {
-        unsigned long long c[4];
+        unsigned long long c[8];
         hip_check(hipMemcpy(c, fetch_.ptr, sizeof(c), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-        stats_.closestNodeFetches = c[0]; stats_.closestTriFetches = c[1];
-        stats_.anyNodeFetches = c[2]; stats_.anyTriFetches = c[3];
+        stats_.closestNodeFetches = c[0]; stats_.closestTriFetches = c[1]; stats_.closestStackSpills = c[2];
+        stats_.anyNodeFetches = c[4]; stats_.anyTriFetches = c[5]; stats_.anyStackSpills = c[6];
     }
     return !pending_.empty() || numActiveSlots() != 0;
 }

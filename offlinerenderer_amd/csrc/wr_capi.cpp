@@ -71,6 +71,7 @@ void fill_stats(const wr::RenderStats& st, wr_c_stats* out) {
     out->closestNodeFetches = st.closestNodeFetches; out->closestTriFetches = st.closestTriFetches;
     out->anyNodeFetches = st.anyNodeFetches; out->anyTriFetches = st.anyTriFetches;
     out->classifyItems = st.classifyItems; out->partitionItems = st.partitionItems; out->shadeItems = st.shadeItems;
+    out->closestStackSpills = st.closestStackSpills; out->anyStackSpills = st.anyStackSpills;
 }
 wr::Renderer* R(void* p) { return need(static_cast<wr::Renderer*>(p), "renderer handle"); }
 }  // namespace

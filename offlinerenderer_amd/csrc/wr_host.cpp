@@ -702,6 +702,7 @@ const RenderStats& RenderServer::getStats() const {
         t.closestNodeFetches += x.closestNodeFetches; t.closestTriFetches += x.closestTriFetches;
         t.anyNodeFetches += x.anyNodeFetches; t.anyTriFetches += x.anyTriFetches;
         t.classifyItems += x.classifyItems; t.partitionItems += x.partitionItems; t.shadeItems += x.shadeItems;
+        t.closestStackSpills += x.closestStackSpills; t.anyStackSpills += x.anyStackSpills;
     }
     return s.stats;
 }

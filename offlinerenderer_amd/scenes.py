@@ -228,8 +228,9 @@ def material_scene(width: int = 1920, height: int = 1080, spp: int = 256, seed: 
         elif kind == 1:
             kinds.append(("glass", (1.0, 1.0, 1.0), 1.5))
         elif kind == 2:
+            # GGX reflection, roughness U(0.1, 0.6), metallic at libWR's default 0 (scene.h:45)
             kinds.append(("ggx", tuple(float(x) for x in rng.uniform(0.5, 1.0, 3)), float(rng.uniform(0.1, 0.6)),
-                          1.0))
+                          0.0))
         elif kind == 3:
             kinds.append(("ggxt", (1.0, 1.0, 1.0), 0.2, 1.5))
         else:
