@@ -76,6 +76,12 @@ def main():
                     help="serving: one chunk per pending request per round (default: in-order admission)")
     ap.add_argument("--express-max-paths", type=int, default=1 << 22,
                     help="serving: requests of at most this many paths take the express lane (0 = off)")
+    ap.add_argument("--fused-max-paths", type=int, default=0,
+                    help="serving: requests of at most this many paths render in one fused launch per chunk")
+    ap.add_argument("--serial-fused-max-paths", type=int, default=0,
+                    help="serving: the same threshold for the serial per-request renders")
+    ap.add_argument("--serial-max-paths", type=int, default=1 << 27,
+                    help="serving: paths in flight of the serial per-request renders")
     ap.add_argument("--express-pool", type=int, default=1 << 23, help="serving: express lane paths in flight")
     ap.add_argument("--express-iters", type=int, default=32,
                     help="serving: express lane iterations per server step")
@@ -333,9 +339,12 @@ def serving(args):
     def digest(img):
         return hashlib.sha1(np.ascontiguousarray(img).tobytes()).hexdigest()
 
-    # serial: one request after another on the same pool size
+    # serial: one request after another, each on a pool that holds it whole
+    # (128M paths, the largest request is 1920x1080 x 64 spp = 133M)
+    serial_pool = args.serial_max_paths
     r = wr.Renderer()
-    r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=pool))
+    r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=serial_pool,
+                                            fusedMaxPaths=args.serial_fused_max_paths))
     r.render(*jobs[0])  # warmup
     service, ref = np.zeros(len(jobs)), []
     for i, job in enumerate(jobs):
@@ -360,7 +369,8 @@ def serving(args):
                                               expressMaxPaths=args.express_max_paths,
                                               expressPathsInFlight=args.express_pool,
                                               expressIterations=args.express_iters,
-                                              expressOverlap=args.express_overlap))
+                                              expressOverlap=args.express_overlap,
+                                              fusedMaxPaths=args.fused_max_paths))
         ids, done_t, images = [None] * len(jobs), np.zeros(len(jobs)), [None] * len(jobs)
         open_ids = {}
         t0 = srv.clock()
@@ -418,7 +428,8 @@ def serving(args):
             "workload": f"serving_{len(jobs)}req_poisson{args.rate:g}",
             "requests": len(jobs), "total_paths": int(total_paths), "resolutions": list(scenes.SERVING_RESOLUTIONS),
             "spp": list(scenes.SERVING_SPP), "max_depth": 8, "arrival_rate_per_s": args.rate,
-            "paths_in_flight": pool, "chunk_paths": chunk,
+            "paths_in_flight": pool, "chunk_paths": chunk, "serial_paths_in_flight": serial_pool,
+            "fused_max_paths": args.fused_max_paths, "serial_fused_max_paths": args.serial_fused_max_paths,
             "admission": args.policy + (" round-robin" if args.round_robin else " in order"),
             "express_lane": ({"max_paths": args.express_max_paths, "paths_in_flight": args.express_pool,
                               "iterations_per_step": args.express_iters,
@@ -437,6 +448,7 @@ def serving(args):
             if m.any()},
         "offered_load": round(float(paths.sum() / arrivals[-1]) / (serial_mpaths * 1e6), 3) if arrivals[-1] > 0 else None,
         "server_steps": {"saturated": steps_sat, "open_loop": steps_ol},
+        "fused_paths_saturated": int(st_sat["fusedPaths"]),
         "mixed_equals_alone": {"saturated": f"{eq_sat}/{len(jobs)}", "open_loop": f"{eq_ol}/{len(jobs)}"},
         "mrays_per_s_saturated": round((st_sat["closestRays"] + st_sat["shadowRays"]) / makespan / 1e6, 3),
         "roofline": None,

@@ -51,6 +51,10 @@ struct WavefrontConfig {
     uint32_t maxChunkPaths = 0;
     bool timeKernels = false;     // per-kernel device time with HIP events
     bool countTraversal = false;  // node/triangle fetch counters (slower)
+    // requests of at most this many paths (width * height * spp) are
+    // rendered by one fused launch per chunk, each path to its end on one
+    // thread, with the same per-path results; 0 = bounce iterations only
+    uint64_t fusedMaxPaths = 0;
 };
 
 struct RenderStats {
@@ -73,6 +77,10 @@ struct RenderStats {
     // five material queues). The wavefront engine resolves trace results in
     // the traversal kernels, so msClassify and classifyItems stay 0 there.
     uint64_t classifyItems = 0, partitionItems = 0, shadeItems = 0;
+    // paths rendered by the fused one-launch kernel (WavefrontConfig::fusedMaxPaths)
+    // and, with timeKernels, its device milliseconds
+    uint64_t fusedPaths = 0;
+    double msFused = 0;
 };
 
 struct RendererImpl;
@@ -143,6 +151,9 @@ struct ServerConfig {
     uint32_t expressPathsInFlight = 1u << 23;
     uint32_t expressIterations = 32;
     bool expressOverlap = false;
+    // requests of at most this many paths render in one fused launch per
+    // chunk (as WavefrontConfig::fusedMaxPaths) in the lane they take
+    uint64_t fusedMaxPaths = 0;
 };
 
 struct RequestStatus {

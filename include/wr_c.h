@@ -38,6 +38,8 @@ typedef struct wr_c_stats {
     uint64_t closestNodeFetches, closestTriFetches, anyNodeFetches, anyTriFetches;
     uint64_t classifyItems, partitionItems, shadeItems;
     uint64_t closestStackSpills, anyStackSpills;
+    uint64_t fusedPaths;
+    double msFused;
 } wr_c_stats;
 
 const char* wr_c_last_error(void);
@@ -69,6 +71,9 @@ void wr_c_renderer_destroy(void* renderer);
 /* flags: bit0 time kernels with HIP events, bit1 count traversal fetches */
 int wr_c_renderer_set_wavefront(void* renderer, uint32_t iterationCap, uint32_t maxPathsInFlight, uint32_t maxChunkPaths,
                                 uint32_t flags);
+/* WavefrontConfig::fusedMaxPaths of the renderer (kept by later
+ * wr_c_renderer_set_wavefront calls) */
+int wr_c_renderer_set_fused(void* renderer, uint64_t fusedMaxPaths);
 /* Renderer::render; out = width*height*3 floats (averaged linear RGB) */
 int wr_c_render(void* renderer, void* scene, const wr_c_camera* cam, const wr_c_render_params* params, float* out);
 /* Renderer::renderPaths; out = count*4 floats */
@@ -91,7 +96,7 @@ typedef struct wr_c_server_config {
     uint32_t expressMaxPaths;       /* ServerConfig::expressMaxPaths (0 = no express lane) */
     uint32_t expressPathsInFlight;  /* express pool size (0 = ServerConfig's default) */
     uint32_t expressIterations;     /* express iterations per step (0 = ServerConfig's default) */
-    uint32_t reserved;
+    uint32_t fusedMaxPaths;         /* ServerConfig::fusedMaxPaths (0 = bounce iterations only) */
 } wr_c_server_config;
 
 typedef struct wr_c_request_status {

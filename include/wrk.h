@@ -329,6 +329,22 @@ int wrk_classify(const wrk_scene_view* scene, const wrk_render_params* paramsTab
 int wrk_shade(const wrk_scene_view* scene, const wrk_render_params* paramsTable, const wrk_paths* paths,
               const uint32_t* queue, const uint32_t* offsets, uint32_t maxItems, uint32_t categoryBase,
               uint32_t finishedCategory, uint8_t* nextCategory, void* stream);
+/* Renders paths firstPath .. firstPath + n - 1 of a request (global path
+ * index = sample * width * height + pixel, as wrk_generate) into pool
+ * entries poolBase .. poolBase + n - 1 to completion in one launch, one path
+ * per thread: the camera ray (trace.cu:53-125), then per visit the stages of
+ * one wavefront iteration in their order (wrk_trace_any_resolve,
+ * wrk_trace_closest_resolve, wrk_shade: libWR's Shadow -> Trace -> Shade
+ * stage machine, trace.cu:27-151 and shade.cu:24-85). paths->result then
+ * holds each path's flushed radiance exactly as the wavefront iteration
+ * leaves it; accumulate it with wrk_accumulate. paramsTable[slot] (device)
+ * holds the request's parameters. No counterpart in libWR, whose renderer
+ * runs every request as bounce launches (renderer.cpp:185-240): for short
+ * requests, where launches and host round trips per bounce cost more than
+ * the work. */
+int wrk_render_fused(const wrk_bvh* bvh, const wrk_scene_view* scene, const wrk_camera* camera,
+                     const wrk_render_params* paramsTable, uint32_t slot, const wrk_paths* paths,
+                     uint64_t firstPath, uint32_t n, uint32_t poolBase, void* stream);
 /* Finished paths per slot (no counterpart in libWR, whose host loop renders
  * one batch at a time): live[slot] -= number of the first
  * n = min(*countDev, maxItems) entries of `finished` (pool entries of paths

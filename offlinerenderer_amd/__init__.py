@@ -65,7 +65,8 @@ class wr_c_stats(C.Structure):
                 ("launchesTraceAny", C.c_uint32), ("closestNodeFetches", C.c_uint64),
                 ("closestTriFetches", C.c_uint64), ("anyNodeFetches", C.c_uint64), ("anyTriFetches", C.c_uint64),
                 ("classifyItems", C.c_uint64), ("partitionItems", C.c_uint64), ("shadeItems", C.c_uint64),
-                ("closestStackSpills", C.c_uint64), ("anyStackSpills", C.c_uint64)]
+                ("closestStackSpills", C.c_uint64), ("anyStackSpills", C.c_uint64), ("fusedPaths", C.c_uint64),
+                ("msFused", C.c_double)]
 
 
 class wrk_paths(C.Structure):
@@ -78,7 +79,7 @@ class wrk_paths(C.Structure):
 class wr_c_server_config(C.Structure):
     _fields_ = [("maxPathsInFlight", C.c_uint32), ("maxChunkPaths", C.c_uint32), ("memoryBudgetBytes", C.c_uint64),
                 ("iterationCap", C.c_uint32), ("flags", C.c_uint32), ("expressMaxPaths", C.c_uint32),
-                ("expressPathsInFlight", C.c_uint32), ("expressIterations", C.c_uint32), ("reserved", C.c_uint32)]
+                ("expressPathsInFlight", C.c_uint32), ("expressIterations", C.c_uint32), ("fusedMaxPaths", C.c_uint32)]
 
 
 class wr_c_request_status(C.Structure):
@@ -113,6 +114,7 @@ _SIGS = {
     "wr_c_renderer_create": (_P, []),
     "wr_c_renderer_destroy": (None, [_P]),
     "wr_c_renderer_set_wavefront": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "wr_c_renderer_set_fused": (C.c_int, [_P, C.c_uint64]),
     "wr_c_render": (C.c_int, [_P, _P, C.POINTER(wr_c_camera), C.POINTER(wr_c_render_params), _P]),
     "wr_c_render_paths": (C.c_int, [_P, _P, C.POINTER(wr_c_camera), C.POINTER(wr_c_render_params), C.c_uint64,
                                     C.c_uint32, _P]),
@@ -194,6 +196,7 @@ _SIGS = {
     "wrk_bvh_tag_categories": (C.c_int, [C.POINTER(wrk_bvh), _P, _P]),
     "wrk_trace_closest_categorized": (C.c_int, [C.POINTER(wrk_bvh), _P, _P, _P, C.c_uint32, _P, _P, _P, _P, _P]),
     "wrk_shade": (C.c_int, [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]),
+    "wrk_render_fused": (C.c_int, [_P, _P, _P, _P, C.c_uint32, _P, C.c_uint64, C.c_uint32, C.c_uint32, _P]),
     "wrk_slot_finish": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
     "wrk_slot_finish_map": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, _P, _P]),
     "wrk_accumulate": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P, _P, _P]),
@@ -292,12 +295,15 @@ class RenderParams:
 
 @dataclass
 class WavefrontConfig:
-    """iterationCap 8 = libWR's host loop (renderer.cpp:213); 0 = run to completion."""
+    """iterationCap 8 = libWR's host loop (renderer.cpp:213); 0 = run to completion.
+    fusedMaxPaths: renders of at most that many paths run as one fused
+    launch per chunk (same per-path results; 0 = bounce iterations only)."""
     iterationCap: int = 8
     maxPathsInFlight: int = 1 << 26
     maxChunkPaths: int = 0
     timeKernels: bool = False
     countTraversal: bool = False
+    fusedMaxPaths: int = 0
 
 
 @dataclass
@@ -446,6 +452,7 @@ class Renderer:
         _check(lib().wr_c_renderer_set_wavefront(self._h, int(cfg.iterationCap), int(cfg.maxPathsInFlight),
                                                       int(cfg.maxChunkPaths), flags),
                "setWavefrontConfig")
+        _check(lib().wr_c_renderer_set_fused(self._h, int(cfg.fusedMaxPaths)), "setWavefrontConfig")
 
     def render(self, scene: Scene, camera: Camera, params: RenderParams) -> np.ndarray:
         """Returns the averaged linear image, shape (height, width, 3), float32."""
@@ -486,7 +493,9 @@ class Renderer:
 class ServerConfig:
     """Continuous-batching server (wr::RenderServer): path pool, admission
     chunk, memory budget for accumulation/image buffers (0 = unlimited), and
-    the express lane for requests of at most expressMaxPaths paths (0 = off)."""
+    the express lane for requests of at most expressMaxPaths paths (0 = off);
+    requests of at most fusedMaxPaths paths render as one fused launch per
+    chunk in the lane they take (0 = off)."""
     maxPathsInFlight: int = 1 << 26
     maxChunkPaths: int = 1 << 24
     memoryBudgetBytes: int = 0
@@ -498,6 +507,7 @@ class ServerConfig:
     expressPathsInFlight: int = 1 << 23
     expressIterations: int = 32
     expressOverlap: bool = False
+    fusedMaxPaths: int = 0
 
 
 class RenderServer:
@@ -511,7 +521,7 @@ class RenderServer:
                                 int(c.iterationCap), (1 if c.timeKernels else 0) | (2 if c.shortestFirst else 0)
                                 | (4 if c.roundRobin else 0) | (8 if c.expressOverlap else 0),
                                 int(c.expressMaxPaths), int(c.expressPathsInFlight),
-                                int(c.expressIterations), 0)
+                                int(c.expressIterations), int(c.fusedMaxPaths))
         self._h = lib().wr_c_server_create(C.byref(cc))
         if not self._h:
             raise RuntimeError(f"RenderServer: {_err()}")

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <vector>
 
+#include "traverse.h"
 #include "wrk_common.h"
 #include "wrk_device.h"
 
@@ -737,7 +738,6 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
     const uint32_t chunkBegin = waveId * (64u * raysPerLane);
     const uint32_t chunkEnd = min(chunkBegin + 64u * raysPerLane, cnt);
     uint32_t cursor = chunkBegin;
-    const float kRobust = 1.0000005f;
     static_assert(W == 8, "traversal of compressed 8-wide nodes");
 
     uint32_t spillN[kStackSpill];
@@ -767,9 +767,9 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                 tbest = d.w;
                 bestPrim = 0xFFFFFFFFu;
                 rp = ray_pre(o, d);
-                ix = 1.0f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x));
-                iy = 1.0f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y));
-                iz = 1.0f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
+                ix = slab_inv(d.x);
+                iy = slab_inv(d.y);
+                iz = slab_inv(d.z);
                 sp = 0;
                 cur = 0;  // root
                 leaf = kEmptyChild;
@@ -826,80 +826,16 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
             if (COUNT) ++nodeFetches;
             float key[W];
             uint32_t code[W];
-            // slab test of one child box: near distance, or +inf on a miss
-            auto slab = [&](float lx, float ly, float lz, float hx, float hy, float hz, uint32_t cc) -> float {
-                float t0x = (lx - o.x) * ix, t1x = (hx - o.x) * ix;
-                float t0y = (ly - o.y) * iy, t1y = (hy - o.y) * iy;
-                float t0z = (lz - o.z) * iz, t1z = (hz - o.z) * iz;
-                float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-                float tf = fminf(fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)), tbest) * kRobust;
-                return (cc != kEmptyChild && tn <= tf) ? tn : INFINITY;
-            };
-            {
-                // six 16-byte loads: origin | child 0-3 | child 4-7 | 48 quantised planes
-                const uint4* raw = reinterpret_cast<const uint4*>(static_cast<const Node8*>(nodesIn) + cur);
-                const uint4 r0 = raw[0], r1 = raw[1], r2 = raw[2], r3 = raw[3], r4 = raw[4], r5 = raw[5];
-                const float ox = __uint_as_float(r0.x), oy = __uint_as_float(r0.y), oz = __uint_as_float(r0.z);
-                const float sx = __uint_as_float((r0.w & 0xFFu) << 23);
-                const float sy = __uint_as_float(((r0.w >> 8) & 0xFFu) << 23);
-                const float sz = __uint_as_float(((r0.w >> 16) & 0xFFu) << 23);
-                code[0] = r1.x; code[1] = r1.y; code[2] = r1.z; code[3] = r1.w;
-                code[4] = r2.x; code[5] = r2.y; code[6] = r2.z; code[7] = r2.w;
-                // plane words: qlo x (r3.x r3.y) y (r3.z r3.w) z (r4.x r4.y),
-                //              qhi x (r4.z r4.w) y (r5.x r5.y) z (r5.z r5.w)
-                const uint32_t qw[12] = {r3.x, r3.y, r3.z, r3.w, r4.x, r4.y, r4.z, r4.w, r5.x, r5.y, r5.z, r5.w};
-#pragma unroll
-                for (int c = 0; c < W; ++c) {
-                    const int wi = c >> 2, sh = 8 * (c & 3);
-                    auto q = [&](int word) { return (float)((qw[word + wi] >> sh) & 0xFFu); };
-                    key[c] = slab(fmaf(q(0), sx, ox), fmaf(q(2), sy, oy), fmaf(q(4), sz, oz),
-                                  fmaf(q(6), sx, ox), fmaf(q(8), sy, oy), fmaf(q(10), sz, oz), code[c]);
-
-                }
-            }
+            node8_children(reinterpret_cast<const uint4*>(static_cast<const Node8*>(nodesIn) + cur), o, ix, iy, iz,
+                           tmin, tbest, key, code);
             if (ANY && found) {
                 cur = kEmptyChild;
                 sp = 0;
             } else {
-                // children in visiting order (sorting network on W keys,
-                // already clamped to the ray)
-// branch-free compare-exchange (selects, no divergent blocks)
-#define WRK_CSWAP(a, b)                                                             \
-    {                                                                               \
-        const float ka = key[a], kb = key[b];                                       \
-        const uint32_t ca = code[a], cb = code[b];                                  \
-        const bool sw = kb < ka;                                                    \
-        key[a] = sw ? kb : ka; key[b] = sw ? ka : kb;                               \
-        code[a] = sw ? cb : ca; code[b] = sw ? ca : cb;                             \
-    }
-                if constexpr (ANY) {
-                    // shadow rays visit far children first: a ray leaving a
-                    // surface meets its occluders toward its end more often
-                    // than among the boxes around its origin (measured: 2-3x
-                    // fewer node visits than near-first). Any order is exact
-                    // for an any-hit query, and tbest never shrinks, so the
-                    // negated keys never cull on pop.
-#pragma unroll
-                    for (int c = 0; c < W; ++c)
-                        if (key[c] != INFINITY) key[c] = -key[c];
-                }
-                if (ANY) {
-                    // shadow rays: a tournament puts the farthest child first;
-                    // the order of the others barely matters to an any-hit
-                    // query (measured: 21.3 -> 20.5 ms per step; for closest-
-                    // hit rays the same shortcut visits 4% more nodes)
-                    WRK_CSWAP(0, 1) WRK_CSWAP(2, 3) WRK_CSWAP(4, 5) WRK_CSWAP(6, 7)
-                    WRK_CSWAP(0, 2) WRK_CSWAP(4, 6) WRK_CSWAP(0, 4)
-                } else {
-                    // Batcher odd-even merge network, 19 comparators
-                    WRK_CSWAP(0, 1) WRK_CSWAP(2, 3) WRK_CSWAP(4, 5) WRK_CSWAP(6, 7)
-                    WRK_CSWAP(0, 2) WRK_CSWAP(1, 3) WRK_CSWAP(4, 6) WRK_CSWAP(5, 7)
-                    WRK_CSWAP(1, 2) WRK_CSWAP(5, 6)
-                    WRK_CSWAP(0, 4) WRK_CSWAP(1, 5) WRK_CSWAP(2, 6) WRK_CSWAP(3, 7)
-                    WRK_CSWAP(2, 4) WRK_CSWAP(3, 5)
-                    WRK_CSWAP(1, 2) WRK_CSWAP(3, 4) WRK_CSWAP(5, 6)
-                }
-#undef WRK_CSWAP
+                // children in visiting order (the any-hit order costs 4%
+                // more node visits if used for closest-hit rays; measured
+                // 21.3 -> 20.5 ms per step for shadow rays against a full sort)
+                order_children<ANY>(key, code);
                 // push far children (farthest first), descend into the nearest
 #pragma unroll
                 for (int c = W - 1; c >= 1; --c) {

@@ -58,7 +58,7 @@ constexpr uint32_t kMaxScenes = 4;
 // final queue: [finished | per scene: trace, trace + shadow, shadow]
 constexpr uint32_t kFinalCategories = 1 + 3 * kMaxScenes;
 
-enum TimedCat { T_CLOSEST, T_ANY, T_CLASSIFY, T_PARTITION, T_SHADE, T_GENERATE, T_ACCUM };
+enum TimedCat { T_CLOSEST, T_ANY, T_CLASSIFY, T_PARTITION, T_SHADE, T_GENERATE, T_FUSED, T_ACCUM };
 
 Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
@@ -401,21 +401,32 @@ bool Engine::admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase) {
     sl.firstPath = r.nextPath;
     // pinned staging: each slot's entries are written once per step and the
     // step ends with a stream synchronisation
+    const bool fused = cfg_.fusedMaxPaths && r.endPath - r.firstPath <= cfg_.fusedMaxPaths;
     hostPrm_[s] = r.prm;
-    hostLiveInit_[s] = n;
+    hostLiveInit_[s] = fused ? 0u : n;  // a fused chunk has ended when its launch completes
     hip_check(hipMemcpyAsync(prmTable_.as<wrk_render_params>() + s, hostPrm_ + s, sizeof(wrk_render_params),
                              hipMemcpyHostToDevice, stream_),
               "hipMemcpyAsync");
     hip_check(hipMemcpyAsync(offs_.as<uint32_t>() + 64 + s, hostLiveInit_ + s, sizeof(uint32_t),
                              hipMemcpyHostToDevice, stream_),
               "hipMemcpyAsync");
-    const uint32_t at = queueBase + *added;
-    timed(T_GENERATE, stream_, [&] {
-        wrk_check(wrk_generate(&r.cam, &r.prm, &paths_, r.nextPath, n, base, s, qB_.as<uint32_t>() + at,
-                               cat_.as<uint8_t>() + at, 1 + 3 * r.sceneIdx, stream_),
-                  "wrk_generate");
-    });
-    *added += n;
+    if (fused) {
+        // the whole chunk in one launch; the slot completes at this step's end
+        timed(T_FUSED, stream_, [&] {
+            wrk_check(wrk_render_fused(&r.scene->getBVH(), &r.scene->getDeviceView(), &r.cam,
+                                       prmTable_.as<wrk_render_params>(), s, &paths_, r.nextPath, n, base, stream_),
+                      "wrk_render_fused");
+        });
+        stats_.fusedPaths += n;
+    } else {
+        const uint32_t at = queueBase + *added;
+        timed(T_GENERATE, stream_, [&] {
+            wrk_check(wrk_generate(&r.cam, &r.prm, &paths_, r.nextPath, n, base, s, qB_.as<uint32_t>() + at,
+                                   cat_.as<uint8_t>() + at, 1 + 3 * r.sceneIdx, stream_),
+                      "wrk_generate");
+        });
+        *added += n;
+    }
     r.nextPath += n;
     r.chunksInFlight++;
     r.chunks++;
@@ -618,6 +629,7 @@ bool Engine::finish() {
             case T_PARTITION: stats_.msPartition += ms; break;
             case T_SHADE: stats_.msShade += ms; break;
             case T_GENERATE: stats_.msGenerate += ms; break;
+            case T_FUSED: stats_.msFused += ms; break;
             default: stats_.msAccumulate += ms; break;
         }
     }

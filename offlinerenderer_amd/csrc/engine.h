@@ -60,6 +60,10 @@ struct EngineConfig {
     bool shortestFirst = false;          // admission order: fewest remaining paths first
     bool roundRobin = false;             // one chunk per pending request per round (else the first takes all that fit)
     uint64_t* sharedFootprint = nullptr; // budget counter shared with another engine (nullptr = this engine's own)
+    // requests of at most this many paths render each admitted chunk to
+    // completion in one fused launch (wrk_render_fused) instead of joining
+    // the bounce iterations (0 = never)
+    uint64_t fusedMaxPaths = 0;
 };
 
 class Engine {

@@ -7,6 +7,7 @@
 // -ffp-contract=off: the arithmetic order follows the reference source.
 #include <hip/hip_runtime.h>
 
+#include "traverse.h"
 #include "wrk_common.h"
 #include "wrk_device.h"
 
@@ -72,15 +73,11 @@ static SceneDev scene_of(const wrk_scene_view* v) {
 }
 
 // ------------------------------------------------------------- generate
-// trace.cu:53-125: camera ray for (pixel, sample), path state reset.
-__global__ __launch_bounds__(256) void k_generate(wrk_camera cam, wrk_render_params prm, Pool P, uint64_t first,
-                                                  uint32_t n, uint32_t poolBase, uint32_t slot, uint32_t* queue,
-                                                  uint8_t* cat, uint32_t catValue) {
-    uint32_t k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t i = poolBase + k;
+// trace.cu:53-125: camera ray of global path g = (pixel, sample) into pool
+// entry i, path state reset.
+__device__ __forceinline__ void generate_path(const wrk_camera& cam, const wrk_render_params& prm, const Pool& P,
+                                              uint64_t g, uint32_t i, uint32_t slot) {
     const uint64_t npix = (uint64_t)prm.width * prm.height;
-    const uint64_t g = first + k;
     const uint32_t pixel = (uint32_t)(g % npix);
     const uint32_t sample = (uint32_t)(g / npix);
     const uint32_t px = pixel % prm.width;
@@ -123,6 +120,15 @@ __global__ __launch_bounds__(256) void k_generate(wrk_camera cam, wrk_render_par
         P.guides[2 * (size_t)i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         P.guides[2 * (size_t)i + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
+}
+
+__global__ __launch_bounds__(256) void k_generate(wrk_camera cam, wrk_render_params prm, Pool P, uint64_t first,
+                                                  uint32_t n, uint32_t poolBase, uint32_t slot, uint32_t* queue,
+                                                  uint8_t* cat, uint32_t catValue) {
+    uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = poolBase + k;
+    generate_path(cam, prm, P, first + k, i, slot);
     queue[k] = i;
     if (cat) cat[k] = (uint8_t)catValue;
 }
@@ -491,35 +497,12 @@ __device__ __forceinline__ f3 effective_normal(const SceneDev& S, const wrk_mate
     return h.normal;
 }
 
-// One launch over the material-sorted queue: blocks are assigned to the
-// five category segments in order, so every block runs one material
-// shader; the trailing blocks mark queue positions past the last segment
-// as finished.
-__global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_params* __restrict__ prmTable, Pool P,
-                                               const uint32_t* queue, const uint32_t* __restrict__ offsets,
-                                               uint32_t maxItems, uint32_t catBase, uint32_t catFinished,
-                                               uint8_t* nextStage) {
-    uint32_t off[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) off[k] = offsets[k];
-    uint32_t b = blockIdx.x;
-    int seg = -1;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        uint32_t nb = (off[k + 1] - off[k] + 255) / 256;
-        if (seg < 0) {
-            if (b < nb) seg = k; else b -= nb;
-        }
-    }
-    if (seg < 0) {
-        uint32_t j = off[5] + b * 256 + threadIdx.x;
-        if (j < maxItems) nextStage[j] = (uint8_t)catFinished;
-        return;
-    }
-    uint32_t j = off[seg] + b * 256 + threadIdx.x;
-    if (j >= off[seg + 1]) return;
-    uint32_t p = queue[j];
-
+// One shade visit of pool entry p in material queue seg (shade.cu:24-85 and
+// the materials/ headers): reads the path state and hit record, runs the
+// material's shader and writes the next state. Returns NEXT_TRACE,
+// NEXT_TRACE_SHADOW, NEXT_SHADOW or NEXT_DONE.
+__device__ __forceinline__ uint32_t shade_path(const SceneDev& S, const wrk_render_params* __restrict__ prmTable,
+                                               const Pool& P, uint32_t p, int seg) {
     PathState r;
     uint4 st = P.st[p];
     const uint32_t slot = st.z >> FLAG_SLOT_SHIFT;
@@ -610,7 +593,108 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_para
                          (slot << FLAG_SLOT_SHIFT);
         P.st[p] = make_uint4(r.seed, r.dim, flags, r.pixel);
     }
+    return next;
+}
+
+// One launch over the material-sorted queue: blocks are assigned to the
+// five category segments in order, so every block runs one material
+// shader; the trailing blocks mark queue positions past the last segment
+// as finished.
+__global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_params* __restrict__ prmTable, Pool P,
+                                               const uint32_t* queue, const uint32_t* __restrict__ offsets,
+                                               uint32_t maxItems, uint32_t catBase, uint32_t catFinished,
+                                               uint8_t* nextStage) {
+    uint32_t off[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) off[k] = offsets[k];
+    uint32_t b = blockIdx.x;
+    int seg = -1;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        uint32_t nb = (off[k + 1] - off[k] + 255) / 256;
+        if (seg < 0) {
+            if (b < nb) seg = k; else b -= nb;
+        }
+    }
+    if (seg < 0) {
+        uint32_t j = off[5] + b * 256 + threadIdx.x;
+        if (j < maxItems) nextStage[j] = (uint8_t)catFinished;
+        return;
+    }
+    uint32_t j = off[seg] + b * 256 + threadIdx.x;
+    if (j >= off[seg + 1]) return;
+    const uint32_t next = shade_path(S, prmTable, P, queue[j], seg);
     nextStage[j] = next == NEXT_DONE ? (uint8_t)catFinished : (uint8_t)(catBase + next);
+}
+
+// ----------------------------------------------------------- fused paths
+// A chunk of paths rendered to completion in one launch, one path per thread:
+// the camera ray, then per visit the stages of one wavefront iteration in
+// its order (the shadow ray and its resolution, wrk_trace_any_resolve; the
+// continuation ray and its resolution, wrk_trace_closest_resolve; the shade
+// step, wrk_shade). Every decision reads only the path's own state, the
+// closest hit does not depend on the traversal order (ties go to the smaller
+// primitive id) and the arithmetic is the same code, so each path ends with
+// the result the wavefront iteration gives it, bit for bit. A short request
+// then costs one launch instead of one iteration of launches per bounce.
+__global__ __launch_bounds__(256) void k_paths_fused(wrk_camera cam, SceneDev S, const Node8* __restrict__ nodes,
+                                                     const float4* __restrict__ tris,
+                                                     const wrk_render_params* __restrict__ prmTable, Pool P,
+                                                     uint64_t first, uint32_t n, uint32_t poolBase, uint32_t slot) {
+    __shared__ uint32_t sNode[kStackLds * 256];
+    __shared__ float sT[kStackLds * 256];
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = poolBase + k;
+    uint32_t* stN = sNode + threadIdx.x;
+    float* stT = sT + threadIdx.x;
+    const wrk_render_params prm = prmTable[slot];
+    generate_path(cam, prm, P, first + k, i, slot);
+    uint32_t next = NEXT_TRACE;
+    for (;;) {
+        uint32_t cat = CAT_DONE;
+        if (next == NEXT_TRACE_SHADOW || next == NEXT_SHADOW) {
+            // trace.cu:27-51: an unoccluded shadow ray adds its contribution
+            float th;
+            uint32_t ph;
+            if (!trace_thread<true>(nodes, tris, P.savedOrg[i], P.savedDir[i], stN, stT, &th, &ph)) {
+                const float4 L = P.rad[i];
+                const float4 C = P.contrib[i];
+                P.rad[i] = make_float4(L.x + C.x, L.y + C.y, L.z + C.z, L.w);
+            }
+            if (next == NEXT_SHADOW) {
+                // the deferred continuation resumes in the Lambertian shader,
+                // or the path ends and is flushed
+                if (P.st[i].z & FLAG_NEE_DONE) {
+                    cat = WRK_MAT_LAMBERTIAN;
+                } else {
+                    const float4 L = P.rad[i];
+                    P.result[i] = f4(clamp_fireflies(mk(L.x, L.y, L.z), prm.fireflyClamp), 0.0f);
+                    break;
+                }
+            }
+        }
+        if (next == NEXT_TRACE || next == NEXT_TRACE_SHADOW) {
+            float tb;
+            uint32_t bp;
+            if (trace_thread<false>(nodes, tris, P.org[i], P.dir[i], stN, stT, &tb, &bp)) {
+                const uint32_t prim = bp & kPrimMask;
+                P.hit[i] = make_uint2(__float_as_uint(tb), prim);
+                const uint32_t tag = bp >> 28;
+                cat = tag != 0u ? tag - 1u : shade_category(S.triMat[prim], S.numMaterials, S.materials);
+            } else {
+                // trace.cu:143-148: environment term, then the flush
+                const float4 T = P.thr[i], L = P.rad[i];
+                const f3 Lr = mk(L.x + T.x * S.env[0], L.y + T.y * S.env[1], L.z + T.z * S.env[2]);
+                P.rad[i] = f4(Lr, 0.0f);
+                P.result[i] = f4(clamp_fireflies(Lr, prm.fireflyClamp), 0.0f);
+                break;
+            }
+        }
+        if (cat == CAT_DONE) break;  // out-of-range material id: ends unflushed (shade.cu:35-38)
+        next = shade_path(S, prmTable, P, i, (int)cat);
+        if (next == NEXT_DONE) break;
+    }
 }
 
 // ----------------------------------------------------------- slot finish
@@ -756,6 +840,26 @@ extern "C" int wrk_shade(const wrk_scene_view* scene, const wrk_render_params* p
     hipLaunchKernelGGL(k_shade, dim3(ceil_div(maxItems, 256) + 6), dim3(256), 0, as_stream(stream), scene_of(scene),
                        paramsTable, pool_of(paths), queue, offsets, maxItems, categoryBase, finishedCategory,
                        nextCategory);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_render_fused(const wrk_bvh* bvh, const wrk_scene_view* scene, const wrk_camera* camera,
+                                const wrk_render_params* paramsTable, uint32_t slot, const wrk_paths* paths,
+                                uint64_t firstPath, uint32_t n, uint32_t poolBase, void* stream) {
+    WRK_REQUIRE(bvh && scene && camera && paramsTable && paths, "wrk_render_fused: null argument");
+    WRK_REQUIRE(bvh->nodes && bvh->tris && bvh->nodeBytes == sizeof(Node8), "wrk_render_fused: unknown BVH format");
+    WRK_REQUIRE((kBvhWidth - 1u) * bvh->maxDepth + 8u <= kStackTotal,
+                "wrk_render_fused: BVH deeper than the traversal stack");
+    WRK_REQUIRE(slot < WRK_MAX_SLOTS, "wrk_render_fused: slot out of range");
+    WRK_REQUIRE((uint64_t)poolBase + n <= paths->capacity, "wrk_render_fused: range exceeds the path pool");
+    WRK_REQUIRE(paths->org && paths->dir && paths->thr && paths->rad && paths->state && paths->contrib &&
+                    paths->savedOrg && paths->savedDir && paths->hit && paths->result,
+                "wrk_render_fused: incomplete path pool");
+    if (n == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_paths_fused, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), *camera,
+                       scene_of(scene), (const Node8*)bvh->nodes, (const float4*)bvh->tris, paramsTable,
+                       pool_of(paths), firstPath, n, poolBase, slot);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }

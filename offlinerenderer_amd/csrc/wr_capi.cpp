@@ -72,6 +72,8 @@ void fill_stats(const wr::RenderStats& st, wr_c_stats* out) {
     out->anyNodeFetches = st.anyNodeFetches; out->anyTriFetches = st.anyTriFetches;
     out->classifyItems = st.classifyItems; out->partitionItems = st.partitionItems; out->shadeItems = st.shadeItems;
     out->closestStackSpills = st.closestStackSpills; out->anyStackSpills = st.anyStackSpills;
+    out->fusedPaths = st.fusedPaths;
+    out->msFused = st.msFused;
 }
 wr::Renderer* R(void* p) { return need(static_cast<wr::Renderer*>(p), "renderer handle"); }
 }  // namespace
@@ -167,6 +169,14 @@ int wr_c_renderer_set_wavefront(void* r, uint32_t cap, uint32_t maxPaths, uint32
         c.maxChunkPaths = maxChunk;
         c.timeKernels = (flags & 1u) != 0;
         c.countTraversal = (flags & 2u) != 0;
+        c.fusedMaxPaths = R(r)->getWavefrontConfig().fusedMaxPaths;
+        R(r)->setWavefrontConfig(c);
+    });
+}
+int wr_c_renderer_set_fused(void* r, uint64_t fusedMaxPaths) {
+    return guard([&] {
+        wr::WavefrontConfig c = R(r)->getWavefrontConfig();
+        c.fusedMaxPaths = fusedMaxPaths;
         R(r)->setWavefrontConfig(c);
     });
 }
@@ -209,6 +219,7 @@ void* wr_c_server_create(const wr_c_server_config* cfg) {
         c.expressMaxPaths = cfg->expressMaxPaths;
         if (cfg->expressPathsInFlight) c.expressPathsInFlight = cfg->expressPathsInFlight;
         if (cfg->expressIterations) c.expressIterations = cfg->expressIterations;
+        c.fusedMaxPaths = cfg->fusedMaxPaths;
         out = new wr::RenderServer(c);
     });
     return out;

@@ -423,6 +423,7 @@ struct RendererImpl {
                                    1024u);
         c.timeKernels = wf.timeKernels;
         c.countTraversal = wf.countTraversal;
+        c.fusedMaxPaths = wf.fusedMaxPaths;
         if (!eng) eng = std::make_unique<detail::Engine>(c);
         c.capacity = std::max(c.capacity, eng->config().capacity);
         eng->setConfig(c);
@@ -579,6 +580,7 @@ struct ServerImpl {
         e.shortestFirst = c.shortestFirst;
         e.roundRobin = c.roundRobin;
         e.sharedFootprint = &footprint;
+        e.fusedMaxPaths = c.fusedMaxPaths;
         eng = std::make_unique<detail::Engine>(e);
         eng->ensureCapacity(e.capacity);
         if (c.expressMaxPaths) {
@@ -702,6 +704,7 @@ const RenderStats& RenderServer::getStats() const {
         t.closestNodeFetches += x.closestNodeFetches; t.closestTriFetches += x.closestTriFetches;
         t.anyNodeFetches += x.anyNodeFetches; t.anyTriFetches += x.anyTriFetches;
         t.classifyItems += x.classifyItems; t.partitionItems += x.partitionItems; t.shadeItems += x.shadeItems;
+        t.fusedPaths += x.fusedPaths; t.msFused += x.msFused;
         t.closestStackSpills += x.closestStackSpills; t.anyStackSpills += x.anyStackSpills;
     }
     return s.stats;
