@@ -76,7 +76,7 @@ def main():
                     help="serving: one chunk per pending request per round (default: in-order admission)")
     ap.add_argument("--express-max-paths", type=int, default=1 << 22,
                     help="serving: requests of at most this many paths take the express lane (0 = off)")
-    ap.add_argument("--fused-max-paths", type=int, default=0,
+    ap.add_argument("--fused-max-paths", type=int, default=1 << 20,
                     help="serving: requests of at most this many paths render in one fused launch per chunk")
     ap.add_argument("--serial-fused-max-paths", type=int, default=0,
                     help="serving: the same threshold for the serial per-request renders")
