@@ -46,9 +46,13 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 TRI_BYTES = 48     # leaf-ordered triangle record (3 x float4)
-# closest-hit per ray: queue index 4 + ray (origin,tMin | dir,tMax) 32 + hit record 8
-CLOSEST_FIXED = 4 + 32 + 8
-# any-hit per ray: queue index 4 + ray 32 + occlusion byte 1
+# closest-hit per ray: queue index 4 + ray (origin,tMin | dir,tMax) 32 + hit
+# record 8 + shade category 1. The resolve epilogue's path-state accesses on a
+# miss (throughput, radiance and state read, radiance and result written: 80 B)
+# are not counted, so the figure is a lower bound.
+CLOSEST_FIXED = 4 + 32 + 8 + 1
+# any-hit per ray: queue index 4 + ray 32 + category 1 (an unoccluded ray's
+# radiance and contribution, 48 B, not counted)
 ANY_FIXED = 4 + 32 + 1
 
 
