@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--dn-loss", choices=["relmse_split", "relmse", "relmse_pred", "log"], default="relmse_split",
                     help="denoise: training loss")
     ap.add_argument("--dn-lr", type=float, default=1e-3, help="denoise: peak learning rate")
+    ap.add_argument("--dn-no-noise", action="store_true",
+                    help="denoise: train without the per-pixel noise input (channel 10 left at 0)")
     ap.add_argument("--dn-eval-size", type=str, default="1920x1080", help="denoise: evaluation resolution")
     ap.add_argument("--dn-ref-spp", type=int, default=4096, help="denoise: spp of the evaluation reference")
     ap.add_argument("--dn-save", type=str, default=None, help="denoise: write the trained weights here")
@@ -474,6 +476,8 @@ def denoise(args):
     data = dt.build_dataset(num_scenes=args.dn_scenes, cams_per_scene=args.dn_cams, spp_target=args.dn_target_spp,
                             log=log)
     t_data = time.perf_counter() - t0
+    if args.dn_no_noise:
+        data["noise"] = None  # ablation: input channel 10 stays 0
     net = denoiser.UNet(seed=1)
     t0 = time.perf_counter()
     losses = dt.train(net, data, steps=args.dn_steps, lr=args.dn_lr, loss=args.dn_loss, log=log,
@@ -507,6 +511,7 @@ def denoise(args):
             "train_target_spp": args.dn_target_spp,
             "train_steps": args.dn_steps, "batch": "16 x 128^2 crops", "params": net.num_params,
             "loss": args.dn_loss,
+            "noise_input": not args.dn_no_noise,
             "eval_reference_spp": args.dn_ref_spp,
         },
         "speedup_basis": basis,
@@ -562,9 +567,11 @@ def denoise_infer(args):
     net = denoiser.UNet()
     net.load(str(wr.LIB_PATH.parent.parent / "models" / "denoiser_unet.f32"))
     q = wr.RenderParams(width=W, height=H, spp=4, maxBounces=d.maxBounces, russianRouletteDepth=d.rrDepth)
-    dev = [torch.from_numpy(a).cuda().unsqueeze(0) for a in r.renderWithGuides(s, cam, q)]
+    dev = [torch.from_numpy(a).cuda().unsqueeze(0) for a in r.renderWithGuides(s, cam, q, noise=True)]
+    noise = dev.pop()
     out = torch.empty_like(dev[0])
     net.set_sample_counts([4])
+    net.set_noise(noise)
     net.forward(*dev, out)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

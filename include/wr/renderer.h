@@ -112,9 +112,12 @@ public:
                      uint32_t count, float* out);
     // The averaged image plus the denoiser guide layers of libWR
     // (trace.cu:267-286): first-hit albedo and ray-facing geometric normal,
-    // averaged over spp; each buffer holds width*height Vec3.
+    // averaged over spp; each buffer holds width*height Vec3. noise (optional,
+    // width*height floats): the standard error of each pixel's mean
+    // (r + g + b) / 3 over its samples, the learned denoiser's noise input
+    // (no counterpart in libWR).
     void renderWithGuides(Scene* scene, const Camera& camera, const RenderParams& params, Vec3* color,
-                          Vec3* albedo, Vec3* normal);
+                          Vec3* albedo, Vec3* normal, float* noise = nullptr);
     const RenderStats& getStats() const;
 
 private:

@@ -82,9 +82,10 @@ int wr_c_render_paths(void* renderer, void* scene, const wr_c_camera* cam, const
 int wr_c_renderer_stats(void* renderer, wr_c_stats* out);
 /* Renderer::loadDenoiser */
 int wr_c_renderer_load_denoiser(void* renderer, const char* path);
-/* Renderer::renderWithGuides; each output width*height*3 floats */
+/* Renderer::renderWithGuides; colour, albedo and normal width*height*3
+ * floats each; noise (NULL skips) width*height floats */
 int wr_c_render_guides(void* renderer, void* scene, const wr_c_camera* cam, const wr_c_render_params* params,
-                       float* color, float* albedo, float* normal);
+                       float* color, float* albedo, float* normal, float* noise);
 
 /* --- continuous-batching render server (wr::RenderServer) --- */
 typedef struct wr_c_server_config {

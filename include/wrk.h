@@ -364,11 +364,18 @@ int wrk_slot_finish_map(const uint32_t* finished, uint32_t maxItems, const uint3
  * With paths->guides and non-NULL accumAlbedo/accumNormal the first-hit
  * guides are summed the same way (trace.cu:267-286: first-hit albedo and
  * ray-facing geometric normal, summed over samples; paths that miss or end
- * before their first shade add zero). */
+ * before their first shade add zero). accumNoise (2 floats per pixel, NULL
+ * skips) sums y and y^2 over the pixel's samples, y = (r + g + b) / 3 of each
+ * path's result, for the learned denoiser's noise input; no counterpart in
+ * libWR, whose OptiX denoiser takes colour, albedo and normal only. */
 int wrk_accumulate(const wrk_paths* paths, uint32_t poolBase, uint64_t first, uint32_t n, uint32_t npix,
-                   float* accum, float* accumAlbedo, float* accumNormal, void* stream);
+                   float* accum, float* accumAlbedo, float* accumNormal, float* accumNoise, void* stream);
 /* out = accum * (1/spp) (renderer.cpp:308-313). */
 int wrk_average(const float* accum, uint32_t npix, uint32_t spp, float* out, void* stream);
+/* out[pixel] = standard error of the pixel's mean y from accumNoise moments
+ * over spp samples: sqrt(max(s2 - s1^2 / spp, 0) / (spp - 1) / spp), or
+ * |s1| when spp = 1. No counterpart in libWR (see accumNoise). */
+int wrk_noise_stderr(const float* moments, uint32_t npix, uint32_t spp, float* out, void* stream);
 /* Exposure 2.2, ACES (Narkowicz), gamma 1/2.2, truncation to uint8
  * (libWR/test/helpers/image.cpp:14-49). */
 int wrk_tonemap_aces(const float* image, uint32_t npix, uint8_t* rgb8, void* stream);

@@ -49,14 +49,17 @@ int wrk_up2_bwd(const uint16_t* dy, uint32_t dyPitch, uint32_t c, uint16_t* dx, 
                 uint32_t h, uint32_t w, void* stream);
 /* Network input (32 channels): log(1 + max(colour, 0)), albedo, normal,
  * a per-image feature (channel 9: feature[image] for image < numFeature,
- * else 0; the network's is log2(spp) / 16), zeros, from FP32
- * (img, srcH, srcW, 3) layers; origins (device, 3 per crop: image, x0, y0)
- * select n crops of h x w, or NULL for whole images, in which case h >= srcH
- * and w >= srcW: positions past the last source row or column repeat it
- * (edge padding). feature is a device array or NULL. */
-int wrk_unet_pack(const float* color, const float* albedo, const float* normal, uint32_t srcH, uint32_t srcW,
-                  const uint32_t* origins, const float* feature, uint32_t numFeature, uint16_t* out, uint32_t n,
-                  uint32_t h, uint32_t w, void* stream);
+ * else 0; the network's is log2(spp) / 16), a per-pixel noise level
+ * (channel 10: log(1 + max(noise, 0) / (1 + max(y, 0))), y the mean of the
+ * pixel's three colour channels; 0 when noise is NULL), zeros, from FP32
+ * (img, srcH, srcW, 3) layers and the (img, srcH, srcW) noise layer (the
+ * standard error of mean y, wrk_noise_stderr); origins (device, 3 per
+ * crop: image, x0, y0) select n crops of h x w, or NULL for whole images, in
+ * which case h >= srcH and w >= srcW: positions past the last source row or
+ * column repeat it (edge padding). feature is a device array or NULL. */
+int wrk_unet_pack(const float* color, const float* albedo, const float* normal, const float* noise, uint32_t srcH,
+                  uint32_t srcW, const uint32_t* origins, const float* feature, uint32_t numFeature, uint16_t* out,
+                  uint32_t n, uint32_t h, uint32_t w, void* stream);
 /* out (npix, 3) FP32 = max(exp(q) - 1, 0) of channels 0..2, q = pred, or
  * pred + base (the packed input: a correction of its log colour) */
 int wrk_unet_unpack(const uint16_t* pred, uint32_t pitch, const uint16_t* base, uint32_t basePitch, float* out,
@@ -96,6 +99,12 @@ int wrk_unet_get_grads(void* net, float* hostOut);
  * next forward / training calls see: input channel 9 = log2(spp) / 16 of
  * each pixel's image (0 for images beyond count, or before any call). */
 int wrk_unet_set_sample_counts(void* net, const uint32_t* spp, uint32_t count);
+/* Noise layer of the source images (device, (img, srcH, srcW) FP32 as from
+ * wrk_noise_stderr, or NULL) that the next forward / training calls pack
+ * into input channel 10. Weights trained without it hold zeros for that
+ * channel (its input was 0, so Adam never moved them) and are unaffected.
+ * The pointer is kept, not copied: it must stay valid while in use. */
+int wrk_unet_set_noise(void* net, const float* noise);
 /* denoised colour (n, h, w, 3) FP32 into out; h, w multiples of 8. Without
  * origins, h x w may exceed srcH x srcW (edge padding, see wrk_unet_pack). */
 int wrk_unet_forward(void* net, const float* color, const float* albedo, const float* normal, uint32_t srcH,

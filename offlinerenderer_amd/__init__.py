@@ -120,7 +120,8 @@ _SIGS = {
                                     C.c_uint32, _P]),
     "wr_c_renderer_stats": (C.c_int, [_P, C.POINTER(wr_c_stats)]),
     "wr_c_renderer_load_denoiser": (C.c_int, [_P, C.c_char_p]),
-    "wr_c_render_guides": (C.c_int, [_P, _P, C.POINTER(wr_c_camera), C.POINTER(wr_c_render_params), _P, _P, _P]),
+    "wr_c_render_guides": (C.c_int, [_P, _P, C.POINTER(wr_c_camera), C.POINTER(wr_c_render_params), _P, _P, _P,
+                                     _P]),
     "wr_c_server_create": (_P, [C.POINTER(wr_c_server_config)]),
     "wr_c_server_destroy": (None, [_P]),
     "wr_c_server_submit": (C.c_uint64, [_P, _P, C.POINTER(wr_c_camera), C.POINTER(wr_c_render_params)]),
@@ -145,8 +146,8 @@ _SIGS = {
                                 C.c_uint32, _P]),
     "wrk_up2_fwd": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "wrk_up2_bwd": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
-    "wrk_unet_pack": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32,
-                                C.c_uint32, _P]),
+    "wrk_unet_pack": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, _P, C.c_uint32,
+                                C.c_uint32, C.c_uint32, _P]),
     "wrk_unet_unpack": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint64, _P]),
     "wrk_unet_loss_scratch_bytes": (C.c_uint64, [C.c_uint32] * 3),
     "wrk_unet_loss": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint32, _P,
@@ -163,6 +164,7 @@ _SIGS = {
     "wrk_unet_get_state": (C.c_int, [_P, _P, _P, _P]),
     "wrk_unet_set_state": (C.c_int, [_P, _P, _P, C.c_uint32]),
     "wrk_unet_set_sample_counts": (C.c_int, [_P, _P, C.c_uint32]),
+    "wrk_unet_set_noise": (C.c_int, [_P, _P]),
     "wrk_unet_forward": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                                    _P, _P]),
     "wrk_unet_train_step": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32,
@@ -199,7 +201,8 @@ _SIGS = {
     "wrk_render_fused": (C.c_int, [_P, _P, _P, _P, C.c_uint32, _P, C.c_uint64, C.c_uint32, C.c_uint32, _P]),
     "wrk_slot_finish": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
     "wrk_slot_finish_map": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, _P, _P]),
-    "wrk_accumulate": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P, _P, _P]),
+    "wrk_accumulate": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P]),
+    "wrk_noise_stderr": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_average": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_tonemap_aces": (C.c_int, [_P, C.c_uint32, _P, _P]),
 }
@@ -466,14 +469,17 @@ class Renderer:
         when params.denoiser.enabled."""
         _check(lib().wr_c_renderer_load_denoiser(self._h, str(path).encode()), "loadDenoiser")
 
-    def renderWithGuides(self, scene: Scene, camera: Camera, params: RenderParams):
-        """(color, albedo, normal), each (height, width, 3) float32 averaged over spp."""
+    def renderWithGuides(self, scene: Scene, camera: Camera, params: RenderParams, noise: bool = False):
+        """(color, albedo, normal), each (height, width, 3) float32 averaged
+        over spp; with noise=True also the (height, width) standard error of
+        each pixel's mean (r + g + b) / 3."""
         shape = (params.height, params.width, 3)
         col, alb, nrm = (np.zeros(shape, dtype=np.float32) for _ in range(3))
+        nse = np.zeros(shape[:2], dtype=np.float32) if noise else None
         cam, prm = camera.to_c(), params.to_c()
         _check(lib().wr_c_render_guides(self._h, scene._h, C.byref(cam), C.byref(prm), _ptr(col), _ptr(alb),
-                                        _ptr(nrm)), "renderWithGuides")
-        return col, alb, nrm
+                                        _ptr(nrm), _ptr(nse) if noise else None), "renderWithGuides")
+        return (col, alb, nrm, nse) if noise else (col, alb, nrm)
 
     def renderPaths(self, scene: Scene, camera: Camera, params: RenderParams, firstPath: int, count: int) -> np.ndarray:
         """Per-path values added to the accumulation buffer, shape (count, 3)."""

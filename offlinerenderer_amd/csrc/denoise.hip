@@ -360,7 +360,8 @@ __global__ __launch_bounds__(256) void k_up2_bwd(const bf16* __restrict__ dy, ui
 // Positions past the source's last row or column repeat it (edge padding of
 // images whose sides are not multiples of 8).
 __global__ __launch_bounds__(256) void k_unet_pack(const float* __restrict__ color, const float* __restrict__ albedo,
-                                                   const float* __restrict__ normal, uint32_t srcH, uint32_t srcW,
+                                                   const float* __restrict__ normal, const float* __restrict__ noise,
+                                                   uint32_t srcH, uint32_t srcW,
                                                    const uint32_t* __restrict__ origins,
                                                    const float* __restrict__ feature, uint32_t numFeature,
                                                    bf16* __restrict__ out, uint32_t N, uint32_t H, uint32_t W) {
@@ -388,6 +389,12 @@ __global__ __launch_bounds__(256) void k_unet_pack(const float* __restrict__ col
     // per-image feature (the sample count, log2(spp) / 16): tells the
     // network how noisy its input is
     if (feature && img < numFeature) v[9] = feature[img];
+    // per-pixel noise: the standard error of the mean y = (r + g + b) / 3
+    // relative to 1 + y, about the noise of the log colour channels
+    if (noise) {
+        const float y = (color[s * 3 + 0] + color[s * 3 + 1] + color[s * 3 + 2]) / 3.0f;
+        v[10] = log1pf(fmaxf(noise[s], 0.0f) / (1.0f + fmaxf(y, 0.0f)));
+    }
     union { uint4 u[4]; bf16 h[32]; } o;
 #pragma unroll
     for (int c = 0; c < 32; ++c) o.h[c] = f2bf(v[c]);
@@ -641,16 +648,16 @@ extern "C" int wrk_up2_bwd(const uint16_t* dy, uint32_t dyPitch, uint32_t c, uin
     return WRK_OK;
 }
 
-extern "C" int wrk_unet_pack(const float* color, const float* albedo, const float* normal, uint32_t srcH,
-                             uint32_t srcW, const uint32_t* origins, const float* feature, uint32_t numFeature,
-                             uint16_t* out, uint32_t n, uint32_t h, uint32_t w, void* stream) {
+extern "C" int wrk_unet_pack(const float* color, const float* albedo, const float* normal, const float* noise,
+                             uint32_t srcH, uint32_t srcW, const uint32_t* origins, const float* feature,
+                             uint32_t numFeature, uint16_t* out, uint32_t n, uint32_t h, uint32_t w, void* stream) {
     WRK_REQUIRE(color && albedo && normal && out, "wrk_unet_pack: null argument");
     WRK_REQUIRE(origins || (h >= srcH && w >= srcW), "wrk_unet_pack: crops need origins");
     WRK_REQUIRE(srcH > 0 && srcW > 0, "wrk_unet_pack: empty source");
     const uint64_t np = (uint64_t)n * h * w;
     if (np == 0) return WRK_OK;
     hipLaunchKernelGGL(k_unet_pack, dim3(ceil_div(np, 256)), dim3(256), 0, as_stream(stream), color, albedo, normal,
-                       srcH, srcW, origins, feature, numFeature, (bf16*)out, n, h, w);
+                       noise, srcH, srcW, origins, feature, numFeature, (bf16*)out, n, h, w);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }

@@ -216,7 +216,7 @@ uint64_t Engine::submit(Scene* scene, const wrk_camera& cam, const wrk_render_pa
     r->pathsOut = pathsOut;
     r->tSubmit = now();
     r->guides = guides && mode == Mode::Accumulate;
-    r->footprint = mode == Mode::Accumulate ? (uint64_t)r->npix * (r->guides ? 72u : 24u) : 0u;
+    r->footprint = mode == Mode::Accumulate ? (uint64_t)r->npix * (r->guides ? 84u : 24u) : 0u;
     uint64_t id = r->id;
     requests_[id] = std::move(r);
     pending_.push_back(id);
@@ -243,11 +243,14 @@ void Engine::release(uint64_t id) {
     requests_.erase(it);
 }
 
-void Engine::fetch(uint64_t id, float* out, float* albedo, float* normal) {
+void Engine::fetch(uint64_t id, float* out, float* albedo, float* normal, float* noise) {
     RequestState& r = *requests_.at(id);
     if (!r.done) throw std::runtime_error("request still in flight");
     if (r.mode != Mode::Accumulate) throw std::runtime_error("request has no image");
-    if ((albedo || normal) && !r.guides) throw std::runtime_error("request was submitted without guides");
+    if ((albedo || normal || noise) && !r.guides) throw std::runtime_error("request was submitted without guides");
+    if (noise)
+        hip_check(hipMemcpy(noise, r.imgNoise.ptr, (size_t)r.npix * sizeof(float), hipMemcpyDeviceToHost),
+                  "hipMemcpy D2H");
     const size_t bytes = (size_t)r.npix * 3 * sizeof(float);
     if (out) hip_check(hipMemcpy(out, r.image.ptr, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
     if (albedo) hip_check(hipMemcpy(albedo, r.imgAlb.ptr, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
@@ -269,6 +272,11 @@ const float* Engine::deviceNormal(uint64_t id) const {
     const RequestState& r = request(id);
     if (!r.done || !r.guides) throw std::runtime_error("request has no finished guides");
     return r.imgNrm.as<float>();
+}
+const float* Engine::deviceNoise(uint64_t id) const {
+    const RequestState& r = request(id);
+    if (!r.done || !r.guides) throw std::runtime_error("request has no finished guides");
+    return r.imgNoise.as<float>();
 }
 
 template <typename F> void Engine::timed(int category, hipStream_t stream, F&& launch) {
@@ -320,7 +328,8 @@ void Engine::complete(uint32_t s) {
             timed(T_ACCUM, stream_, [&] {
                 wrk_check(wrk_accumulate(&paths_, h.poolBase, h.firstPath, h.count, r.npix, r.accum.as<float>(),
                                          r.guides ? r.accAlb.as<float>() : nullptr,
-                                         r.guides ? r.accNrm.as<float>() : nullptr, stream_),
+                                         r.guides ? r.accNrm.as<float>() : nullptr,
+                                         r.guides ? r.accNoise.as<float>() : nullptr, stream_),
                           "wrk_accumulate");
             });
             r.accumNext += h.count;
@@ -334,6 +343,8 @@ void Engine::complete(uint32_t s) {
         if (r.guides) {
             wrk_check(wrk_average(r.accAlb.as<float>(), r.npix, r.spp, r.imgAlb.as<float>(), stream_), "wrk_average");
             wrk_check(wrk_average(r.accNrm.as<float>(), r.npix, r.spp, r.imgNrm.as<float>(), stream_), "wrk_average");
+            wrk_check(wrk_noise_stderr(r.accNoise.as<float>(), r.npix, r.spp, r.imgNoise.as<float>(), stream_),
+                      "wrk_noise_stderr");
         }
         r.done = true;
         r.tDone = -1;  // stamped after the step's synchronisation
@@ -376,8 +387,12 @@ bool Engine::admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase) {
         if (r.guides) {
             const size_t bytes = (size_t)r.npix * 3 * sizeof(float);
             r.accAlb.alloc(bytes); r.accNrm.alloc(bytes); r.imgAlb.alloc(bytes); r.imgNrm.alloc(bytes);
+            r.accNoise.alloc((size_t)r.npix * 2 * sizeof(float));
+            r.imgNoise.alloc((size_t)r.npix * sizeof(float));
             hip_check(hipMemsetAsync(r.accAlb.ptr, 0, bytes, stream_), "hipMemsetAsync");
             hip_check(hipMemsetAsync(r.accNrm.ptr, 0, bytes, stream_), "hipMemsetAsync");
+            hip_check(hipMemsetAsync(r.accNoise.ptr, 0, (size_t)r.npix * 2 * sizeof(float), stream_),
+                      "hipMemsetAsync");
             if (!guides_.ptr) {
                 // first request with guides: per-path guide records from now on
                 guides_.alloc((size_t)capacity_ * 32);

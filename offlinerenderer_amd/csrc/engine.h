@@ -82,6 +82,7 @@ public:
         DevBuf accum, image;
         bool guides = false;        // first-hit albedo / normal sums (trace.cu:267-286)
         DevBuf accAlb, accNrm, imgAlb, imgNrm;
+        DevBuf accNoise, imgNoise;  // with guides: per-pixel (sum y, sum y^2) and the standard error of mean y
         // chunks finish in any order; their results enter the accumulation
         // buffer in sample order: accumNext is the first path not yet added,
         // held maps the first path of each finished chunk waiting for its
@@ -133,12 +134,15 @@ public:
     // request to host memory and releases the request.
     // albedo / normal (W*H*3 floats each, averaged over spp) when the
     // request was submitted with guides; NULL skips them.
-    void fetch(uint64_t id, float* out, float* albedo = nullptr, float* normal = nullptr);
+    // noise (W*H floats) with guides: the standard error of each pixel's
+    // mean y = (r + g + b) / 3 (wrk_noise_stderr).
+    void fetch(uint64_t id, float* out, float* albedo = nullptr, float* normal = nullptr, float* noise = nullptr);
     // The averaged images of a finished request, on the device (valid until
     // fetch or release).
     const float* deviceImage(uint64_t id) const;
     const float* deviceAlbedo(uint64_t id) const;
     const float* deviceNormal(uint64_t id) const;
+    const float* deviceNoise(uint64_t id) const;
     hipStream_t stream() const { return stream_; }
     void release(uint64_t id);
     size_t numPending() const { return pending_.size(); }

@@ -135,6 +135,7 @@ public:
         }
         featCount_ = count;
     }
+    void setNoise(const float* noise) { noise_ = noise; }
     // Adam state for training checkpoints: both moment vectors (layout of
     // the parameters) and the step count
     void getState(float* m, float* v, uint32_t* step) const {
@@ -157,8 +158,8 @@ public:
     void forward(const float* color, const float* albedo, const float* normal, uint32_t srcH, uint32_t srcW,
                  const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w, float* out, hipStream_t s) {
         shape(n, h, w, false);
-        wrk_ok(wrk_unet_pack(color, albedo, normal, srcH, srcW, origins, featCount_ ? feat_.f() : nullptr, featCount_,
-                             in_.buf.h(), n, h, w, s),
+        wrk_ok(wrk_unet_pack(color, albedo, normal, noise_, srcH, srcW, origins, featCount_ ? feat_.f() : nullptr,
+                             featCount_, in_.buf.h(), n, h, w, s),
                "wrk_unet_pack");
         runForward(s);
         if (out)
@@ -170,8 +171,8 @@ public:
                     uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w, float lr,
                     uint32_t lossMode, hipStream_t s) {
         shape(n, h, w, true);
-        wrk_ok(wrk_unet_pack(color, albedo, normal, srcH, srcW, origins, featCount_ ? feat_.f() : nullptr, featCount_,
-                             in_.buf.h(), n, h, w, s),
+        wrk_ok(wrk_unet_pack(color, albedo, normal, noise_, srcH, srcW, origins, featCount_ ? feat_.f() : nullptr,
+                             featCount_, in_.buf.h(), n, h, w, s),
                "wrk_unet_pack");
         runForward(s);
         wrk_ok(wrk_unet_loss(pred_.buf.h(), 32, target, srcH, srcW, origins, target2, in_.buf.h(), 32, gPred_.buf.h(), n, h,
@@ -366,6 +367,7 @@ private:
     Act gPred_, g0a_, g0b_, gCat0_, g1a_, g1b_, gCat1_, gP1_, g2a_, g2b_, gCat2_, gP2_, g3a_, g3b_, gP3_;
     Buf lossPartial_, wgradScratch_, feat_;
     uint32_t featCount_ = 0;
+    const float* noise_ = nullptr;  // device noise layer of the source images, or none
 };
 
 thread_local std::string g_unetError;
@@ -423,6 +425,9 @@ int wrk_unet_get_grads(void* net, float* out) {
 }
 int wrk_unet_set_sample_counts(void* net, const uint32_t* spp, uint32_t count) {
     return guard([&] { N(net)->setSampleCounts(spp, count); });
+}
+int wrk_unet_set_noise(void* net, const float* noise) {
+    return guard([&] { N(net)->setNoise(noise); });
 }
 int wrk_unet_forward(void* net, const float* color, const float* albedo, const float* normal, uint32_t srcH,
                      uint32_t srcW, const uint32_t* origins, uint32_t n, uint32_t h, uint32_t w, float* out,

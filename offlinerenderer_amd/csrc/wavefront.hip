@@ -756,16 +756,29 @@ __global__ __launch_bounds__(256) void k_slot_finish(const uint32_t* __restrict_
 __global__ __launch_bounds__(256) void k_accumulate(const float4* __restrict__ result,
                                                     const float4* __restrict__ guides, uint32_t poolBase,
                                                     uint64_t first, uint32_t n, uint32_t npix, float* accum,
-                                                    float* accAlb, float* accNrm) {
+                                                    float* accAlb, float* accNrm, float2* accNoise) {
     uint32_t k = blockIdx.x * 256 + threadIdx.x;
     uint32_t cover = n < npix ? n : npix;
     if (k >= cover) return;
     uint64_t g0 = first + k;  // first path of this pixel in the chunk
     uint32_t pixel = (uint32_t)(g0 % npix);
     float a0 = accum[pixel * 3 + 0], a1 = accum[pixel * 3 + 1], a2 = accum[pixel * 3 + 2];
-    for (uint64_t i = k; i < n; i += npix) {
-        float4 v = result[poolBase + i];
-        a0 = a0 + v.x; a1 = a1 + v.y; a2 = a2 + v.z;
+    if (accNoise) {
+        // first two moments of each sample's mean channel y = (r + g + b) / 3
+        float2 m = accNoise[pixel];
+        for (uint64_t i = k; i < n; i += npix) {
+            float4 v = result[poolBase + i];
+            a0 = a0 + v.x; a1 = a1 + v.y; a2 = a2 + v.z;
+            const float y = (v.x + v.y + v.z) / 3.0f;
+            m.x = m.x + y;
+            m.y = m.y + y * y;
+        }
+        accNoise[pixel] = m;
+    } else {
+        for (uint64_t i = k; i < n; i += npix) {
+            float4 v = result[poolBase + i];
+            a0 = a0 + v.x; a1 = a1 + v.y; a2 = a2 + v.z;
+        }
     }
     accum[pixel * 3 + 0] = a0; accum[pixel * 3 + 1] = a1; accum[pixel * 3 + 2] = a2;
     if (guides && accAlb && accNrm) {
@@ -785,6 +798,23 @@ __global__ __launch_bounds__(256) void k_average(const float* __restrict__ accum
                                                  float* out) {
     uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i < n3) out[i] = accum[i] * invSpp;
+}
+
+// Standard error of a pixel's mean y from its sample moments (sum y, sum
+// y^2) over spp samples: sqrt(max(s2 - s1^2 / n, 0) / (n - 1) / n); with one
+// sample, |y| itself.
+__global__ __launch_bounds__(256) void k_noise_stderr(const float2* __restrict__ moments, uint32_t npix,
+                                                      uint32_t spp, float* out) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= npix) return;
+    const float2 m = moments[i];
+    if (spp < 2) {
+        out[i] = fabsf(m.x);
+        return;
+    }
+    const float n = (float)spp;
+    const float var = fmaxf(m.y - m.x * m.x / n, 0.0f) / (n - 1.0f);
+    out[i] = sqrtf(var / n);
 }
 
 // libWR/test/helpers/image.cpp:14-49 (gamma in FP64, see oracle_capi.cpp)
@@ -896,14 +926,15 @@ extern "C" int wrk_slot_finish_map(const uint32_t* finished, uint32_t maxItems, 
 }
 
 extern "C" int wrk_accumulate(const wrk_paths* paths, uint32_t poolBase, uint64_t first, uint32_t n,
-                              uint32_t npix, float* accum, float* accumAlbedo, float* accumNormal, void* stream) {
+                              uint32_t npix, float* accum, float* accumAlbedo, float* accumNormal, float* accumNoise,
+                              void* stream) {
     WRK_REQUIRE(paths && accum && npix > 0, "wrk_accumulate: bad argument");
     WRK_REQUIRE((uint64_t)poolBase + n <= paths->capacity, "wrk_accumulate: range exceeds the path pool");
     if (n == 0) return WRK_OK;
     uint32_t cover = n < npix ? n : npix;
     hipLaunchKernelGGL(k_accumulate, dim3(ceil_div(cover, 256)), dim3(256), 0, as_stream(stream),
                        (const float4*)paths->result, (const float4*)paths->guides, poolBase, first, n, npix, accum,
-                       accumAlbedo, accumNormal);
+                       accumAlbedo, accumNormal, (float2*)accumNoise);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }
@@ -914,6 +945,15 @@ extern "C" int wrk_average(const float* accum, uint32_t npix, uint32_t spp, floa
     if (n3 == 0) return WRK_OK;
     hipLaunchKernelGGL(k_average, dim3(ceil_div(n3, 256)), dim3(256), 0, as_stream(stream), accum, n3,
                        1.0f / (float)spp, out);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_noise_stderr(const float* moments, uint32_t npix, uint32_t spp, float* out, void* stream) {
+    WRK_REQUIRE(moments && out && spp > 0, "wrk_noise_stderr: bad argument");
+    if (npix == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_noise_stderr, dim3(ceil_div(npix, 256)), dim3(256), 0, as_stream(stream),
+                       (const float2*)moments, npix, spp, out);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }

@@ -190,10 +190,10 @@ int wr_c_render_paths(void* r, void* s, const wr_c_camera* cam, const wr_c_rende
     return guard([&] { R(r)->renderPaths(S(s), to_camera(cam), to_params(p), first, count, need(out, "paths")); });
 }
 int wr_c_render_guides(void* r, void* s, const wr_c_camera* cam, const wr_c_render_params* p, float* color,
-                       float* albedo, float* normal) {
+                       float* albedo, float* normal, float* noise) {
     return guard([&] {
         R(r)->renderWithGuides(S(s), to_camera(cam), to_params(p), reinterpret_cast<wr::Vec3*>(need(color, "colour")),
-                               reinterpret_cast<wr::Vec3*>(albedo), reinterpret_cast<wr::Vec3*>(normal));
+                               reinterpret_cast<wr::Vec3*>(albedo), reinterpret_cast<wr::Vec3*>(normal), noise);
     });
 }
 int wr_c_renderer_load_denoiser(void* r, const char* path) {

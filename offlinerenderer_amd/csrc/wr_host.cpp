@@ -434,7 +434,7 @@ struct RendererImpl {
     // renderer.cpp:132-316 — the whole image as one request, in chunks of
     // at most maxPathsInFlight paths, accumulated per pixel in sample order.
     void render(Scene* scene, const Camera& camera, const RenderParams& rp, float* outHost, float* albedo = nullptr,
-                float* normal = nullptr) {
+                float* normal = nullptr, float* noise = nullptr) {
         if (!scene) throw std::runtime_error("render: null scene");
         if (rp.width == 0 || rp.height == 0 || rp.spp == 0) throw std::runtime_error("render: empty image");
         const uint64_t total = (uint64_t)rp.width * rp.height * rp.spp;
@@ -471,6 +471,7 @@ struct RendererImpl {
             denoised.alloc((size_t)pw * ph * 3 * sizeof(float));
             const uint32_t spp = rp.spp;  // the network is told how noisy its input is
             if (wrk_unet_set_sample_counts(unet, &spp, 1) != WRK_OK ||
+                wrk_unet_set_noise(unet, e.deviceNoise(id)) != WRK_OK ||
                 wrk_unet_forward(unet, e.deviceImage(id), e.deviceAlbedo(id), e.deviceNormal(id), rp.height,
                                  rp.width, nullptr, 1, ph, pw, denoised.as<float>(), e.stream()) != WRK_OK)
                 throw std::runtime_error(std::string("denoiser: ") + wrk_unet_last_error());
@@ -478,9 +479,9 @@ struct RendererImpl {
             hip_check(hipMemcpy2D(dst, rowBytes, denoised.ptr, (size_t)pw * 3 * sizeof(float), rowBytes, rp.height,
                                   hipMemcpyDeviceToHost),
                       "hipMemcpy2D D2H");
-            e.fetch(id, nullptr, albedo, normal);
+            e.fetch(id, nullptr, albedo, normal, noise);
         } else {
-            e.fetch(id, dst, albedo, normal);
+            e.fetch(id, dst, albedo, normal, noise);
         }
         stats = e.stats();
         stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -540,10 +541,10 @@ void Renderer::renderPaths(Scene* scene, const Camera& camera, const RenderParam
 }
 const RenderStats& Renderer::getStats() const { return m_impl->stats; }
 void Renderer::renderWithGuides(Scene* scene, const Camera& camera, const RenderParams& params, Vec3* color,
-                                Vec3* albedo, Vec3* normal) {
+                                Vec3* albedo, Vec3* normal, float* noise) {
     if (!albedo || !normal) throw std::runtime_error("renderWithGuides: null guide buffer");
     m_impl->render(scene, camera, params, reinterpret_cast<float*>(color), reinterpret_cast<float*>(albedo),
-                   reinterpret_cast<float*>(normal));
+                   reinterpret_cast<float*>(normal), noise);
 }
 
 // ----------------------------------------------------------- RenderServer

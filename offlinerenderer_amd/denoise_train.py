@@ -55,16 +55,18 @@ def rel_mse(x: np.ndarray, ref: np.ndarray, eps: float = 1e-2) -> float:
 def build_dataset(num_scenes=8, cams_per_scene=48, res=256, noisy_spps=(4, 16, 64, 256), spp_target=1024, seed=100,
                   log=print):
     """CUDA float32 tensors (N, res, res, 3): color, albedo, normal, target,
-    target2, and "spp" (numpy uint32, N): the input's sample count. Every
-    view is rendered at each noise level of noisy_spps (samples
-    0..spp-1); target and target2 cover two disjoint runs of spp_target / 2
-    samples after those."""
+    target2; "noise" (N, res, res): the standard error of each pixel's mean
+    (r + g + b) / 3 in the input; and "spp" (numpy uint32, N): the input's
+    sample count. Every view is rendered at each noise level of noisy_spps
+    (samples 0..spp-1); target and target2 cover two disjoint runs of
+    spp_target / 2 samples after those."""
     r = wr.Renderer()
     r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=1 << 26))
     rng = np.random.default_rng(seed)
     n = num_scenes * cams_per_scene * len(noisy_spps)
     out = {k: torch.empty((n, res, res, 3), dtype=torch.float32, device="cuda")
            for k in ("color", "albedo", "normal", "target", "target2")}
+    out["noise"] = torch.empty((n, res, res), dtype=torch.float32, device="cuda")
     out["spp"] = np.zeros(n, np.uint32)
     t0 = time.perf_counter()
     i = 0
@@ -80,8 +82,8 @@ def build_dataset(num_scenes=8, cams_per_scene=48, res=256, noisy_spps=(4, 16, 6
             tgt2 = torch.from_numpy(r.render(s, cam, p))
             for spp in noisy_spps:
                 p.spp, p.firstSample = spp, 0
-                col, alb, nrm = r.renderWithGuides(s, cam, p)
-                for k, a in (("color", col), ("albedo", alb), ("normal", nrm)):
+                col, alb, nrm, nse = r.renderWithGuides(s, cam, p, noise=True)
+                for k, a in (("color", col), ("albedo", alb), ("normal", nrm), ("noise", nse)):
                     out[k][i].copy_(torch.from_numpy(a))
                 out["target"][i].copy_(tgt)
                 out["target2"][i].copy_(tgt2)
@@ -106,6 +108,7 @@ def train(net: denoiser.UNet, data: dict, steps=3000, batch=16, crop=128, lr=1e-
     schedule still spans `steps`)."""
     n, res = data["color"].shape[0], data["color"].shape[1]
     net.set_sample_counts(data["spp"])  # input channel 9 per source image
+    net.set_noise(data.get("noise"))  # input channel 10 per source pixel (None: left at 0)
     g = torch.Generator(device="cpu").manual_seed(seed)
     first = 0
     if resume:
@@ -194,9 +197,11 @@ def evaluate(net: denoiser.UNet, width=1920, height=1080, seed=9001, spp_noisy=4
     log(f"[denoise] eval: learned relMSE {rel_l:.4f} in {t_learned * 1e3:.1f} ms "
         f"(noisy {spp_noisy} spp: relMSE {rel_mse(noisy, ref):.4f}, {t_noisy * 1e3:.1f} ms)")
     # inference alone, device-resident inputs
-    dev = [torch.from_numpy(a).cuda().unsqueeze(0) for a in r.renderWithGuides(s, cam, params(spp_noisy))]
+    dev = [torch.from_numpy(a).cuda().unsqueeze(0) for a in r.renderWithGuides(s, cam, params(spp_noisy), noise=True)]
+    noise = dev.pop()
     out = torch.empty_like(dev[0])
     net.set_sample_counts([spp_noisy])
+    net.set_noise(noise)
     net.forward(*dev, out)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

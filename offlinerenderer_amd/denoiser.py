@@ -2,9 +2,11 @@
 csrc/denoise.hip) over the C ABI of include/wrk_unet.h.
 
 Stands in for libWR's OptiX AI denoiser (libWR/src/internal/denoiser.cpp),
-which takes the same three layers: noisy colour, first-hit albedo and
-normal (Renderer.renderWithGuides). Image data stays on the device (torch
-CUDA tensors as buffers); the parameters are one FP32 vector.
+which takes three of its layers: noisy colour, first-hit albedo and
+normal (Renderer.renderWithGuides). The network also takes each image's
+sample count and each pixel's noise level (the standard error of its mean,
+renderWithGuides(noise=True)). Image data stays on the device (torch CUDA
+tensors as buffers); the parameters are one FP32 vector.
 """
 from __future__ import annotations
 
@@ -115,6 +117,15 @@ class UNet:
         a = np.ascontiguousarray(np.atleast_1d(spp), dtype=np.uint32)
         self._check(lib().wrk_unet_set_sample_counts(self._h, a.ctypes.data_as(C.c_void_p), a.size),
                     "set_sample_counts")
+
+    def set_noise(self, noise):
+        """Per-pixel noise layer of the source images (CUDA float32 (imgs, H,
+        W): the standard error of each pixel's mean (r + g + b) / 3, as
+        Renderer.renderWithGuides(noise=True) returns it), packed into input
+        channel 10 by later forward and training calls; None clears it. The
+        tensor is kept referenced while set."""
+        self._noise = noise
+        self._check(lib().wrk_unet_set_noise(self._h, _ptr(noise) if noise is not None else None), "set_noise")
 
     def forward(self, color, albedo, normal, out, origins=None, crop=None, stream=None):
         """color/albedo/normal: CUDA float32 (imgs, H, W, 3); out: (n, h, w, 3).

@@ -28,15 +28,20 @@ def torch_weight(wflat: np.ndarray, cin: int, cout: int) -> torch.Tensor:
     return torch.from_numpy(wflat.reshape(3, 3, cout, cin)).permute(2, 3, 0, 1).contiguous()
 
 
-def pack_input(color, albedo, normal, spp=None):
+def pack_input(color, albedo, normal, spp=None, noise=None):
     """(n, h, w, 3) each -> (n, 32, h, w) network input (bf16-rounded);
-    spp (per image) sets channel 9 to log2(spp) / 16."""
+    spp (per image) sets channel 9 to log2(spp) / 16; noise (n, h, w), the
+    standard error of each pixel's mean y = (r + g + b) / 3, sets channel 10
+    to log(1 + noise / (1 + y))."""
     x = np.zeros(color.shape[:3] + (32,), np.float32)
     x[..., 0:3] = np.log1p(np.maximum(color, 0.0))
     x[..., 3:6] = albedo
     x[..., 6:9] = normal
     if spp is not None:
         x[..., 9] = (np.log2(np.asarray(spp, np.float32)) / np.float32(16.0))[:, None, None]
+    if noise is not None:
+        y = color.astype(np.float64).sum(-1) / 3.0
+        x[..., 10] = np.log1p(np.maximum(noise, 0.0) / (1.0 + np.maximum(y, 0.0)))
     return torch.from_numpy(bf16_round(x)).permute(0, 3, 1, 2)
 
 
