@@ -316,19 +316,23 @@ int wrk_trace_closest_resolve(const wrk_bvh* bvh, const wrk_scene_view* scene, c
 int wrk_classify(const wrk_scene_view* scene, const wrk_render_params* paramsTable, const wrk_paths* paths,
                  const uint32_t* queue, uint32_t numTrace, uint32_t numTraceShadow, uint32_t numShadow,
                  uint8_t* category, uint32_t flags, void* stream);
-/* Runs the material shaders over the material-sorted queue of one scene
- * (offsets as from wrk_partition with 6 categories, the sixth holding the
- * paths classify finished): shade.cu:24-85 and the materials/ headers.
- * nextCategory[i] = categoryBase + 0 (trace the continuation), + 1 (trace
- * the continuation and a shadow ray), + 2 (trace a shadow ray only) or
- * finishedCategory (path ended) for queue positions i < maxItems.
+/* Runs the material shaders over one scene's queue part in queue order
+ * (shade.cu:24-85 and the materials/ headers; libWR's shade kernel also
+ * takes every active ray in one launch and switches per ray). queueIn[i] is
+ * a pool entry and category[i] its shade category from the resolving
+ * traversals: a material kind 0..4 (gpu_types.h:11-17) or 5 = path ended.
+ * category[i] becomes the entry's next category: categoryBase + 0 (trace
+ * the continuation), + 1 (trace the continuation and a shadow ray), + 2
+ * (trace a shadow ray only) or finishedCategory (path ended); queueOut[i] =
+ * queueIn[i], for i < n. Within each run of 256 entries the shaders run in
+ * material order, so a wave mostly runs one shader.
  * With iterationCap = 0 a Lambertian hit emits its shadow ray and its
  * continuation together; with a cap the continuation waits for the shadow
  * result as in libWR's stage machine, and paths reaching the cap are
  * dropped unflushed (renderer.cpp:213). */
 int wrk_shade(const wrk_scene_view* scene, const wrk_render_params* paramsTable, const wrk_paths* paths,
-              const uint32_t* queue, const uint32_t* offsets, uint32_t maxItems, uint32_t categoryBase,
-              uint32_t finishedCategory, uint8_t* nextCategory, void* stream);
+              const uint32_t* queueIn, uint32_t* queueOut, uint32_t n, uint32_t categoryBase,
+              uint32_t finishedCategory, uint8_t* category, void* stream);
 /* Renders paths firstPath .. firstPath + n - 1 of a request (global path
  * index = sample * width * height + pixel, as wrk_generate) into pool
  * entries poolBase .. poolBase + n - 1 to completion in one launch, one path

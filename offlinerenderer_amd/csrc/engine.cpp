@@ -67,21 +67,18 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
         hip_check(hipEventCreateWithFlags(&sceneDone_[s], hipEventDisableTiming), "hipEventCreate");
     }
     hip_check(hipEventCreateWithFlags(&stepStart_, hipEventDisableTiming), "hipEventCreate");
-    // one pinned block read back each step: the 16 words at offs_ + 48 (the
+    // one pinned block read back each step: the 16 words at offs_ (the
     // queue offsets) and the per-slot live counts behind them
     hip_check(hipHostMalloc((void**)&hostOffs_, (16 + WRK_MAX_SLOTS) * sizeof(uint32_t), hipHostMallocDefault),
               "hipHostMalloc");
     hostLive_ = hostOffs_ + 16;
-    hip_check(hipHostMalloc((void**)&hostSceneOffs_, 8 * kMaxScenes * sizeof(uint32_t), hipHostMallocDefault),
-              "hipHostMalloc");
     hip_check(hipHostMalloc((void**)&hostPrm_, WRK_MAX_SLOTS * sizeof(wrk_render_params), hipHostMallocDefault),
               "hipHostMalloc");
     hip_check(hipHostMalloc((void**)&hostLiveInit_, WRK_MAX_SLOTS * sizeof(uint32_t), hipHostMallocDefault),
               "hipHostMalloc");
     std::memset(hostPrm_, 0, WRK_MAX_SLOTS * sizeof(wrk_render_params));
-    // offs_: per-scene partition offsets at 8 * scene, the queue offsets at
-    // 48, the per-slot live counts at 64
-    offs_.alloc((64 + WRK_MAX_SLOTS) * sizeof(uint32_t));
+    // offs_: the queue offsets at 0, the per-slot live counts at 16
+    offs_.alloc((16 + WRK_MAX_SLOTS) * sizeof(uint32_t));
     prmTable_.alloc(WRK_MAX_SLOTS * sizeof(wrk_render_params));
     fetch_.alloc(64);
     hip_check(hipMemsetAsync(fetch_.ptr, 0, 64, stream_), "hipMemsetAsync");
@@ -91,7 +88,6 @@ Engine::Engine(const EngineConfig& cfg) : cfg_(cfg) {
 Engine::~Engine() {
     for (hipEvent_t e : eventPool_) (void)hipEventDestroy(e);
     if (hostOffs_) (void)hipHostFree(hostOffs_);
-    if (hostSceneOffs_) (void)hipHostFree(hostSceneOffs_);
     if (hostPrm_) (void)hipHostFree(hostPrm_);
     if (hostLiveInit_) (void)hipHostFree(hostLiveInit_);
     for (uint32_t s = 0; s < kMaxScenes; ++s) {
@@ -122,7 +118,6 @@ void Engine::reserve(uint32_t cap) {
     qA_.alloc(c * 4); qB_.alloc(c * 4); cat_.alloc(c);
     slotMap_.alloc(c / 256);
     scratch_.alloc(wrk_partition_scratch_bytes(cap, kFinalCategories));
-    for (uint32_t s = 0; s < kMaxScenes; ++s) sceneScratch_[s].alloc(wrk_partition_scratch_bytes(cap, 6));
     capacity_ = cap;
     paths_.org = org_.addr(); paths_.dir = dir_.addr(); paths_.thr = thr_.addr(); paths_.rad = rad_.addr();
     paths_.state = st_.addr(); paths_.contrib = contrib_.addr(); paths_.savedOrg = savedOrg_.addr();
@@ -422,7 +417,7 @@ bool Engine::admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase) {
     hip_check(hipMemcpyAsync(prmTable_.as<wrk_render_params>() + s, hostPrm_ + s, sizeof(wrk_render_params),
                              hipMemcpyHostToDevice, stream_),
               "hipMemcpyAsync");
-    hip_check(hipMemcpyAsync(offs_.as<uint32_t>() + 64 + s, hostLiveInit_ + s, sizeof(uint32_t),
+    hip_check(hipMemcpyAsync(offs_.as<uint32_t>() + 16 + s, hostLiveInit_ + s, sizeof(uint32_t),
                              hipMemcpyHostToDevice, stream_),
               "hipMemcpyAsync");
     if (fused) {
@@ -457,7 +452,7 @@ void Engine::launch() {
     uint8_t* cat = cat_.as<uint8_t>();
     uint32_t* off = offs_.as<uint32_t>();
     const wrk_render_params* prmTable = prmTable_.as<wrk_render_params>();
-    uint32_t* live = off + 64;
+    uint32_t* live = off + 16;
     unsigned long long* fc = fetch_.as<unsigned long long>();
     eventNext_ = 0;
     marks_.clear();
@@ -503,15 +498,10 @@ void Engine::launch() {
         }
         stats_.closestRays += nC;
         stats_.shadowRays += nA;
-        stats_.partitionItems += n;
-        timed(T_PARTITION, ss, [&] {
-            // five material queues + the paths classify finished
-            wrk_check(wrk_partition(cat + a, qlive + a, n, 6, qb + a, off + 8 * s, sceneScratch_[s].ptr,
-                                    sceneScratch_[s].bytes, ss),
-                      "wrk_partition");
-        });
+        // the material shaders in queue order, which keeps each admitted
+        // chunk's paths in pool order
         timed(T_SHADE, ss, [&] {
-            wrk_check(wrk_shade(&view, prmTable, &paths_, qb + a, off + 8 * s, n, 1 + 3 * s, 0, cat + a, ss),
+            wrk_check(wrk_shade(&view, prmTable, &paths_, qlive + a, qb + a, n, 1 + 3 * s, 0, cat + a, ss),
                       "wrk_shade");
         });
         hip_check(hipEventRecord(sceneDone_[s], ss), "hipEventRecord");
@@ -570,27 +560,22 @@ void Engine::launch() {
     const uint32_t items = total + added;
     if (items) {
         timed(T_PARTITION, stream_, [&] {
-            wrk_check(wrk_partition(cat, qb, items, K, qa, off + 48, scratch_.ptr, scratch_.bytes, stream_),
+            wrk_check(wrk_partition(cat, qb, items, K, qa, off, scratch_.ptr, scratch_.bytes, stream_),
                       "wrk_partition");
         });
         // live paths per slot: subtract the paths that ended
         timed(T_PARTITION, stream_, [&] {
-            wrk_check(wrk_slot_finish_map(qa, items, off + 48 + 1, slotMap_.as<uint8_t>(), capacity_ / 256, live,
+            wrk_check(wrk_slot_finish_map(qa, items, off + 1, slotMap_.as<uint8_t>(), capacity_ / 256, live,
                                           stream_),
                       "wrk_slot_finish_map");
         });
     }
     // queue offsets and live counts in one read-back
     if (items || numActiveSlots())
-        hip_check(hipMemcpyAsync(hostOffs_, off + 48, (16 + WRK_MAX_SLOTS) * sizeof(uint32_t),
+        hip_check(hipMemcpyAsync(hostOffs_, off, (16 + WRK_MAX_SLOTS) * sizeof(uint32_t),
                                  hipMemcpyDeviceToHost, stream_),
                   "hipMemcpyAsync");
     stats_.partitionItems += items;
-    // per-scene material-queue sizes (item counts for the kernel statistics)
-    if (cfg_.timeKernels && total)
-        hip_check(hipMemcpyAsync(hostSceneOffs_, off, 8 * kMaxScenes * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                 stream_),
-                  "hipMemcpyAsync");
     launchedTotal_ = total;
     launchedItems_ = items;
     launchedScenes_ = S2;
@@ -609,9 +594,7 @@ bool Engine::finish() {
     launched_ = false;
     const uint32_t items = launchedItems_, S2 = launchedScenes_;
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-    if (cfg_.timeKernels && launchedTotal_)
-        for (uint32_t s = 0; s < S2; ++s)
-            if (segT_[s] + segTS_[s] + segS_[s]) stats_.shadeItems += hostSceneOffs_[8 * s + 5];
+    stats_.shadeItems += launchedTotal_;
     deadCount_ = items ? hostOffs_[1] : 0;
     for (uint32_t s = 0; s < S2; ++s) {
         const uint32_t* o = hostOffs_ + 1 + 3 * s;

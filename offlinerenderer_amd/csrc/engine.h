@@ -180,7 +180,6 @@ private:
     // of an iteration touch disjoint queue parts and run concurrently
     hipStream_t sceneStream_[4] = {};
     hipEvent_t stepStart_ = nullptr, sceneDone_[4] = {};
-    DevBuf sceneScratch_[4];
     uint32_t capacity_ = 0;
     DevBuf org_, dir_, thr_, rad_, st_, contrib_, savedOrg_, savedDir_, hit_, occl_, result_, guides_;
     DevBuf qA_, qB_, cat_, scratch_, offs_, prmTable_, fetch_, slotMap_;
@@ -188,7 +187,6 @@ private:
     uint32_t deadCount_ = 0;              // entries ended last iteration: qA_[0, deadCount_)
     bool launched_ = false;               // launch() done, finish() not yet
     uint32_t launchedItems_ = 0, launchedScenes_ = 0, launchedTotal_ = 0;
-    uint32_t* hostSceneOffs_ = nullptr;   // pinned: per-scene partition offsets (kernel statistics only)
     uint32_t* hostOffs_ = nullptr;        // pinned: queue offsets, then the live counts
     uint32_t* hostLive_ = nullptr;        // hostOffs_ + 16: live paths per slot, read back each step
     uint32_t* hostLiveInit_ = nullptr;    // pinned staging: initial live count per slot

@@ -596,35 +596,52 @@ __device__ __forceinline__ uint32_t shade_path(const SceneDev& S, const wrk_rend
     return next;
 }
 
-// One launch over the material-sorted queue: blocks are assigned to the
-// five category segments in order, so every block runs one material
-// shader; the trailing blocks mark queue positions past the last segment
-// as finished.
-__global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_params* __restrict__ prmTable, Pool P,
-                                               const uint32_t* queue, const uint32_t* __restrict__ offsets,
-                                               uint32_t maxItems, uint32_t catBase, uint32_t catFinished,
-                                               uint8_t* nextStage) {
-    uint32_t off[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) off[k] = offsets[k];
-    uint32_t b = blockIdx.x;
-    int seg = -1;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        uint32_t nb = (off[k + 1] - off[k] + 255) / 256;
-        if (seg < 0) {
-            if (b < nb) seg = k; else b -= nb;
-        }
+// One launch over a scene's queue part in queue order (no material
+// partition). Each block takes 256 consecutive entries, which the queue keeps
+// in pool order within each admitted chunk, so the block's pool accesses
+// stay within a short pool range; it orders them by category in LDS, so a
+// wave mostly runs one material's shader. Each entry's next category goes
+// back to its own queue position, and queueOut receives the queue as it
+// was (the queue rebuild reads it there). Measured on the material scene
+// against material-partitioned segments (a stable partition by material,
+// then one shader per block): shade 74.3 -> 72.6 ms and partition 14.5 ->
+// 8.9 ms per 1080p 256-spp render; without the LDS ordering shade took 79.0
+// ms, with 1024-entry windows (4 per thread) 77.1.
+__global__ __launch_bounds__(256) void k_shade(SceneDev S, const wrk_render_params* __restrict__ prmTable,
+                                                     Pool P, const uint32_t* __restrict__ queueIn,
+                                                     uint32_t* __restrict__ queueOut, uint32_t n, uint32_t catBase,
+                                                     uint32_t catFinished, uint8_t* category) {
+    __shared__ uint32_t sCount[8];
+    __shared__ uint32_t sEntry[256];  // pool entry of each sorted position
+    __shared__ uint16_t sPos[256];    // block-local queue position of each sorted position
+    __shared__ uint8_t sCat[256];
+    const uint32_t t = threadIdx.x;
+    const uint32_t j = blockIdx.x * 256u + t;
+    uint32_t p = 0, c = 7;  // positions past the queue sort last and do nothing
+    if (j < n) {
+        p = queueIn[j];
+        queueOut[j] = p;
+        c = min((uint32_t)category[j], (uint32_t)kCatDone);
     }
-    if (seg < 0) {
-        uint32_t j = off[5] + b * 256 + threadIdx.x;
-        if (j < maxItems) nextStage[j] = (uint8_t)catFinished;
+    if (t < 8) sCount[t] = 0;
+    __syncthreads();
+    const uint32_t rank = atomicAdd(&sCount[c], 1u);
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t k = 0; k < c; ++k) base += sCount[k];
+    sEntry[base + rank] = p;
+    sPos[base + rank] = (uint16_t)t;
+    sCat[base + rank] = (uint8_t)c;
+    __syncthreads();
+    const uint32_t mc = sCat[t];
+    if (mc > kCatDone) return;
+    const uint32_t q = blockIdx.x * 256u + sPos[t];
+    if (mc == kCatDone) {
+        category[q] = (uint8_t)catFinished;
         return;
     }
-    uint32_t j = off[seg] + b * 256 + threadIdx.x;
-    if (j >= off[seg + 1]) return;
-    const uint32_t next = shade_path(S, prmTable, P, queue[j], seg);
-    nextStage[j] = next == NEXT_DONE ? (uint8_t)catFinished : (uint8_t)(catBase + next);
+    const uint32_t next = shade_path(S, prmTable, P, sEntry[t], (int)mc);
+    category[q] = next == NEXT_DONE ? (uint8_t)catFinished : (uint8_t)(catBase + next);
 }
 
 // ----------------------------------------------------------- fused paths
@@ -862,14 +879,14 @@ extern "C" int wrk_classify(const wrk_scene_view* scene, const wrk_render_params
 }
 
 extern "C" int wrk_shade(const wrk_scene_view* scene, const wrk_render_params* paramsTable, const wrk_paths* paths,
-                         const uint32_t* queue, const uint32_t* offsets, uint32_t maxItems, uint32_t categoryBase,
-                         uint32_t finishedCategory, uint8_t* nextCategory, void* stream) {
-    WRK_REQUIRE(scene && paramsTable && paths && queue && offsets && nextCategory, "wrk_shade: null argument");
+                         const uint32_t* queueIn, uint32_t* queueOut, uint32_t n, uint32_t categoryBase,
+                         uint32_t finishedCategory, uint8_t* category, void* stream) {
+    WRK_REQUIRE(scene && paramsTable && paths && queueIn && queueOut && category, "wrk_shade: null argument");
     WRK_REQUIRE(categoryBase + 2u <= 0xFFu && finishedCategory <= 0xFFu, "wrk_shade: category out of range");
-    if (maxItems == 0) return WRK_OK;
-    hipLaunchKernelGGL(k_shade, dim3(ceil_div(maxItems, 256) + 6), dim3(256), 0, as_stream(stream), scene_of(scene),
-                       paramsTable, pool_of(paths), queue, offsets, maxItems, categoryBase, finishedCategory,
-                       nextCategory);
+    WRK_REQUIRE(n <= paths->capacity, "wrk_shade: queue longer than the path pool");
+    if (n == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_shade, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), scene_of(scene),
+                       paramsTable, pool_of(paths), queueIn, queueOut, n, categoryBase, finishedCategory, category);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
 }
