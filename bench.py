@@ -76,6 +76,9 @@ def main():
                     "8M for serving)")
     ap.add_argument("--requests", type=int, default=256, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=40.0, help="serving: Poisson arrival rate, requests/s")
+    ap.add_argument("--rate-sweep", type=str, default="",
+                    help="serving: further arrival rates (comma-separated req/s), each an open-loop run of its own "
+                    "reported with serial FIFO latency under the same arrivals")
     ap.add_argument("--round-robin", action="store_true",
                     help="serving: one chunk per pending request per round (default: in-order admission)")
     ap.add_argument("--express-max-paths", type=int, default=1 << 22,
@@ -101,6 +104,11 @@ def main():
     ap.add_argument("--dn-loss", choices=["relmse_split", "relmse", "relmse_pred", "log"], default="relmse_split",
                     help="denoise: training loss")
     ap.add_argument("--dn-lr", type=float, default=1e-3, help="denoise: peak learning rate")
+    ap.add_argument("--dn-footprint", type=str, default=None,
+                    help="denoise: train on 256^2 windows at the pixel scale of a WxH frame (e.g. 1920x1080); "
+                    "default: whole 256^2 frames")
+    ap.add_argument("--dn-load", type=str, default=None,
+                    help="denoise: start from these weights (with --dn-steps 0: evaluate them only)")
     ap.add_argument("--dn-no-noise", action="store_true",
                     help="denoise: train without the per-pixel noise input (channel 10 left at 0)")
     ap.add_argument("--dn-eval-size", type=str, default="1920x1080", help="denoise: evaluation resolution")
@@ -338,9 +346,12 @@ def serving(args):
                              russianRouletteDepth=q["rrDepth"])) for q in reqs]
     paths = np.array([p.width * p.height * p.spp for _, _, p in jobs], dtype=np.float64)
     total_paths = float(paths.sum())
-    rng = np.random.default_rng(11)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=len(jobs)))
-    arrivals -= arrivals[0]
+    def arrivals_at(rate):
+        # the same unit-rate exponential draws at every rate: only the time scale changes
+        a = np.cumsum(np.random.default_rng(11).exponential(1.0, size=len(jobs))) / rate
+        return a - a[0]
+
+    arrivals = arrivals_at(args.rate)
 
     def digest(img):
         return hashlib.sha1(np.ascontiguousarray(img).tobytes()).hexdigest()
@@ -361,12 +372,15 @@ def serving(args):
     del r
     serial_mpaths = total_paths / service.sum() / 1e6
     log(f"[serve] serial: {service.sum():.2f} s busy, {serial_mpaths:.1f} Mpaths/s")
-    # serial FIFO latency under the same arrivals (one request at a time)
-    finish, t = np.zeros(len(jobs)), 0.0
-    for i in range(len(jobs)):
-        t = max(t, arrivals[i]) + service[i]
-        finish[i] = t
-    serial_lat = finish - arrivals
+    def fifo_latency(arrive):
+        # serial FIFO latency under these arrivals (one request at a time)
+        finish, t = np.zeros(len(jobs)), 0.0
+        for i in range(len(jobs)):
+            t = max(t, arrive[i]) + service[i]
+            finish[i] = t
+        return finish - arrive
+
+    serial_lat = fifo_latency(arrivals)
 
     def run_server(arrive):
         srv = wr.RenderServer(wr.ServerConfig(maxPathsInFlight=pool, maxChunkPaths=chunk,
@@ -417,6 +431,20 @@ def serving(args):
                 "p99_ms": round(float(np.percentile(a, 99)) * 1e3, 1),
                 "mean_ms": round(float(a.mean()) * 1e3, 1)}
 
+    sweep = []
+    for rate in [float(v) for v in args.rate_sweep.split(",") if v.strip()]:
+        arr = arrivals_at(rate)
+        done_r, eq_r, _, _ = run_server(arr)
+        lat_r, fifo_r = done_r - arr, fifo_latency(arr)
+        sweep.append({"rate_per_s": rate,
+                      "offered_load": round(float(paths.sum() / arr[-1]) / (serial_mpaths * 1e6), 3),
+                      "server": pct(lat_r), "serial_fifo": pct(fifo_r),
+                      "mixed_equals_alone": f"{eq_r}/{len(jobs)}"})
+        log(f"[serve] server at {rate:g} req/s: p50/p99 {sweep[-1]['server']['p50_ms']:.0f}/"
+            f"{sweep[-1]['server']['p99_ms']:.0f} ms, serial FIFO {sweep[-1]['serial_fifo']['p50_ms']:.0f}/"
+            f"{sweep[-1]['serial_fifo']['p99_ms']:.0f} ms")
+        assert eq_r == len(jobs), "a served image differs from its stand-alone render"
+
     line = {
         "metric": "Mpaths/s @1080p; time-to-relMSE<=1e-2 learned vs plain; p50/p99 req latency",
         "value": round(server_mpaths, 3),
@@ -453,6 +481,7 @@ def serving(args):
             for cls, m in (("le_4M_paths", paths <= (1 << 22)), ("gt_4M_paths", paths > (1 << 22)))
             if m.any()},
         "offered_load": round(float(paths.sum() / arrivals[-1]) / (serial_mpaths * 1e6), 3) if arrivals[-1] > 0 else None,
+        "rate_sweep": sweep or None,
         "server_steps": {"saturated": steps_sat, "open_loop": steps_ol},
         "fused_paths_saturated": int(st_sat["fusedPaths"]),
         "mixed_equals_alone": {"saturated": f"{eq_sat}/{len(jobs)}", "open_loop": f"{eq_ol}/{len(jobs)}"},
@@ -472,19 +501,25 @@ def denoise(args):
     from offlinerenderer_amd import denoise_train as dt
     from offlinerenderer_amd import denoiser
 
-    t0 = time.perf_counter()
-    data = dt.build_dataset(num_scenes=args.dn_scenes, cams_per_scene=args.dn_cams, spp_target=args.dn_target_spp,
-                            log=log)
-    t_data = time.perf_counter() - t0
-    if args.dn_no_noise:
-        data["noise"] = None  # ablation: input channel 10 stays 0
+    footprint = tuple(int(v) for v in args.dn_footprint.split("x")) if args.dn_footprint else None
     net = denoiser.UNet(seed=1)
-    t0 = time.perf_counter()
-    losses = dt.train(net, data, steps=args.dn_steps, lr=args.dn_lr, loss=args.dn_loss, log=log,
-                      checkpoint=args.dn_checkpoint, checkpoint_every=args.dn_checkpoint_every,
-                      resume=args.dn_resume)
-    t_train = time.perf_counter() - t0
-    del data
+    if args.dn_load:
+        net.load(args.dn_load)
+    t_data = t_train = 0.0
+    losses = [float("nan")]
+    if args.dn_steps > 0:
+        t0 = time.perf_counter()
+        data = dt.build_dataset(num_scenes=args.dn_scenes, cams_per_scene=args.dn_cams,
+                                spp_target=args.dn_target_spp, footprint=footprint, log=log)
+        t_data = time.perf_counter() - t0
+        if args.dn_no_noise:
+            data["noise"] = None  # ablation: input channel 10 stays 0
+        t0 = time.perf_counter()
+        losses = dt.train(net, data, steps=args.dn_steps, lr=args.dn_lr, loss=args.dn_loss, log=log,
+                          checkpoint=args.dn_checkpoint, checkpoint_every=args.dn_checkpoint_every,
+                          resume=args.dn_resume)
+        t_train = time.perf_counter() - t0
+        del data
     if args.dn_save:
         net.save(args.dn_save)
     W, H = (int(v) for v in args.dn_eval_size.split("x"))
@@ -510,6 +545,9 @@ def denoise(args):
             "train_views": args.dn_scenes * args.dn_cams, "train_input_spp": [4, 16, 64, 256], "train_res": 256,
             "train_target_spp": args.dn_target_spp,
             "train_steps": args.dn_steps, "batch": "16 x 128^2 crops", "params": net.num_params,
+            "train_view_kind": ("256^2 windows at the pixel scale of " + args.dn_footprint) if footprint
+            else "whole 256^2 frames",
+            "weights_loaded": args.dn_load,
             "loss": args.dn_loss,
             "noise_input": not args.dn_no_noise,
             "eval_reference_spp": args.dn_ref_spp,

@@ -10,9 +10,11 @@ spp_noisy), so input noise and target noise are independent.
 Evaluation (time to relMSE <= 1e-2, relMSE = mean over pixels and channels
 of (x - ref)^2 / (ref^2 + 1e-2)): the reference is a high-spp render over
 samples disjoint from everything it is compared with; the plain tracer
-renders 4, 8, 16, ... spp until it reaches the bar; the learned path is
-Renderer.render at 4 spp with the network loaded (guides, render and
-inference all on the device).
+renders 4, 8, 16, ... spp until it reaches the bar, then bisects the sample
+count to within 1/32 of the smallest that reaches it; the learned path is
+Renderer.render with the network loaded (guides, render and inference all
+on the device) at 4 spp and, swept the same way, at the smallest sample
+count that reaches the bar.
 """
 from __future__ import annotations
 
@@ -38,6 +40,32 @@ def large_camera(rng, w: int, h: int) -> dict:
                 fovY=math.radians(float(rng.uniform(40.0, 60.0))), aspect=w / h)
 
 
+def window_camera(rng, res: int, full_w: int, full_h: int) -> dict:
+    """A res x res window of a random full_w x full_h view (large_camera's
+    distribution): same eye, the view direction turned to a uniformly placed
+    window of the full frame, and the field of view narrowed so that a pixel
+    covers the same solid angle as in the full frame. Training views drawn
+    this way show the network surfaces, edges and noise at the pixel scale of
+    full_w x full_h renders."""
+    c = large_camera(rng, full_w, full_h)
+    pos, tgt = np.array(c["position"]), np.array(c["target"])
+    f = tgt - pos
+    f /= np.linalg.norm(f)
+    r = np.cross(f, np.array(c["up"]))
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    t = math.tan(c["fovY"] / 2)
+    # window centre in NDC of the full frame, kept inside it
+    sx = 1.0 - res / full_w
+    sy = 1.0 - res / full_h
+    nx, ny = rng.uniform(-sx, sx), rng.uniform(-sy, sy)
+    d = f + r * (nx * t * c["aspect"]) + u * (ny * t)
+    d /= np.linalg.norm(d)
+    fov = 2.0 * math.atan(t * res / full_h)
+    return dict(position=c["position"], target=tuple(float(x) for x in pos + d * 10.0), up=(0.0, 1.0, 0.0),
+                fovY=fov, aspect=1.0)
+
+
 def build_scene(seed: int, w: int, h: int):
     d = scenes.large_scene(w, h, 1, seed=seed)
     s = wr.Scene()
@@ -53,13 +81,15 @@ def rel_mse(x: np.ndarray, ref: np.ndarray, eps: float = 1e-2) -> float:
 
 
 def build_dataset(num_scenes=8, cams_per_scene=48, res=256, noisy_spps=(4, 16, 64, 256), spp_target=1024, seed=100,
-                  log=print):
+                  footprint=None, log=print):
     """CUDA float32 tensors (N, res, res, 3): color, albedo, normal, target,
     target2; "noise" (N, res, res): the standard error of each pixel's mean
     (r + g + b) / 3 in the input; and "spp" (numpy uint32, N): the input's
     sample count. Every view is rendered at each noise level of noisy_spps
     (samples 0..spp-1); target and target2 cover two disjoint runs of
-    spp_target / 2 samples after those."""
+    spp_target / 2 samples after those. footprint = (W, H): each view is a
+    res x res window of a W x H frame at that frame's pixel scale
+    (window_camera); None: whole res x res frames (large_camera)."""
     r = wr.Renderer()
     r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=1 << 26))
     rng = np.random.default_rng(seed)
@@ -73,7 +103,7 @@ def build_dataset(num_scenes=8, cams_per_scene=48, res=256, noisy_spps=(4, 16, 6
     for si in range(num_scenes):
         d, s = build_scene(seed + si, res, res)
         for _ in range(cams_per_scene):
-            cam = wr.Camera(**large_camera(rng, res, res))
+            cam = wr.Camera(**(window_camera(rng, res, *footprint) if footprint else large_camera(rng, res, res)))
             half = spp_target // 2
             p = wr.RenderParams(width=res, height=res, spp=half, maxBounces=d.maxBounces,
                                 russianRouletteDepth=d.rrDepth, firstSample=max(noisy_spps))
@@ -161,15 +191,39 @@ def evaluate(net: denoiser.UNet, width=1920, height=1080, seed=9001, spp_noisy=4
     ref, t_ref = _timed_render(r, s, cam, params(ref_spp, first=1 << 20))
     log(f"[denoise] eval: reference {ref_spp} spp in {t_ref:.1f} s")
     _timed_render(r, s, cam, params(spp_noisy))  # warm-up
-    plain = []
-    spp = spp_noisy
-    while spp <= max_plain_spp:
-        img, t = _timed_render(r, s, cam, params(spp))
-        plain.append({"spp": spp, "seconds": t, "rel_mse": rel_mse(img, ref)})
-        log(f"[denoise] eval: plain {spp} spp relMSE {plain[-1]['rel_mse']:.4f} in {t * 1e3:.0f} ms")
-        if plain[-1]["rel_mse"] <= bar:
-            break
-        spp *= 2
+
+    def sweep(name, denoise):
+        """Renders at spp_noisy, 2x, 4x, ... until one reaches the bar, then
+        bisects the sample count between the last miss and that render
+        until they are within 1/32 of each other. Returns the renders in
+        the order made; the smallest sample count that reached the bar is
+        re-timed as the median of three renders."""
+        pts, spp = [], spp_noisy
+        while spp <= max_plain_spp:
+            img, t = _timed_render(r, s, cam, params(spp, denoise=denoise))
+            pts.append({"spp": spp, "seconds": t, "rel_mse": rel_mse(img, ref)})
+            log(f"[denoise] eval: {name} {spp} spp relMSE {pts[-1]['rel_mse']:.4f} in {t * 1e3:.0f} ms")
+            if pts[-1]["rel_mse"] <= bar:
+                break
+            spp *= 2
+        hit = [q for q in pts if q["rel_mse"] <= bar]
+        if not hit or hit[0]["spp"] == spp_noisy:
+            return pts, (hit[0] if hit else None)
+        lo, hi = hit[0]["spp"] // 2, hit[0]["spp"]
+        while hi - lo > max(1, lo // 32):
+            mid = (lo + hi) // 2
+            img, t = _timed_render(r, s, cam, params(mid, denoise=denoise))
+            pts.append({"spp": mid, "seconds": t, "rel_mse": rel_mse(img, ref)})
+            log(f"[denoise] eval: {name} {mid} spp relMSE {pts[-1]['rel_mse']:.4f} in {t * 1e3:.0f} ms")
+            if pts[-1]["rel_mse"] <= bar:
+                hi = mid
+            else:
+                lo = mid
+        best = next(q for q in pts if q["spp"] == hi)
+        times = [best["seconds"]] + [_timed_render(r, s, cam, params(hi, denoise=denoise))[1] for _ in range(2)]
+        return pts, dict(best, seconds=float(np.median(times)))
+
+    plain, plain_hit = sweep("plain", False)
     # learned: the renderer's own denoised render (weights through a file)
     fd, path = tempfile.mkstemp(suffix=".f32")
     os.close(fd)
@@ -186,13 +240,10 @@ def evaluate(net: denoiser.UNet, width=1920, height=1080, seed=9001, spp_noisy=4
     t_learned = float(np.median(times))
     rel_l = rel_mse(img_l, ref)
     # the learned path at more input samples, until it reaches the bar
-    learned = [{"spp": spp_noisy, "seconds": t_learned, "rel_mse": rel_l}]
-    spp = spp_noisy * 2
-    while learned[-1]["rel_mse"] > bar and spp <= max_plain_spp:
-        img, t = _timed_render(r, s, cam, params(spp, denoise=True))
-        learned.append({"spp": spp, "seconds": t, "rel_mse": rel_mse(img, ref)})
-        log(f"[denoise] eval: learned {spp} spp relMSE {learned[-1]['rel_mse']:.4f} in {t * 1e3:.0f} ms")
-        spp *= 2
+    learned, learned_hit = sweep("learned", True)
+    learned[0] = {"spp": spp_noisy, "seconds": t_learned, "rel_mse": learned[0]["rel_mse"]}
+    if learned_hit and learned_hit["spp"] == spp_noisy:
+        learned_hit = learned[0]
     noisy, t_noisy = _timed_render(r, s, cam, params(spp_noisy))
     log(f"[denoise] eval: learned relMSE {rel_l:.4f} in {t_learned * 1e3:.1f} ms "
         f"(noisy {spp_noisy} spp: relMSE {rel_mse(noisy, ref):.4f}, {t_noisy * 1e3:.1f} ms)")
@@ -210,24 +261,24 @@ def evaluate(net: denoiser.UNet, width=1920, height=1080, seed=9001, spp_noisy=4
         net.forward(*dev, out)
     torch.cuda.synchronize()
     t_inf = (time.perf_counter() - t0) / reps
-    reached = [p for p in plain if p["rel_mse"] <= bar]
     res = {
         "scene_seed": seed, "width": width, "height": height, "reference_spp": ref_spp, "bar": bar,
         "plain": plain,
-        "plain_spp_to_bar": reached[0]["spp"] if reached else None,
-        "plain_seconds_to_bar": reached[0]["seconds"] if reached else None,
+        "plain_spp_to_bar": plain_hit["spp"] if plain_hit else None,
+        "plain_seconds_to_bar": plain_hit["seconds"] if plain_hit else None,
         "learned_rel_mse": rel_l, "learned_seconds": t_learned, "learned_reaches_bar": rel_l <= bar,
         "learned_sweep": learned,
-        "learned_spp_to_bar": next((q["spp"] for q in learned if q["rel_mse"] <= bar), None),
-        "learned_seconds_to_bar": next((q["seconds"] for q in learned if q["rel_mse"] <= bar), None),
+        "learned_spp_to_bar": learned_hit["spp"] if learned_hit else None,
+        "learned_seconds_to_bar": learned_hit["seconds"] if learned_hit else None,
         "noisy_rel_mse": rel_mse(noisy, ref), "noisy_seconds": t_noisy,
         "inference_ms": t_inf * 1e3,
         "inference_tflops": denoiser.flops_per_pixel() * width * height / t_inf / 1e12,
     }
-    # equal-quality plain time when the bar is out of reach for the learned path
-    rels = np.array([p["rel_mse"] for p in plain])
-    secs = np.array([p["seconds"] for p in plain])
-    spps = np.array([p["spp"] for p in plain], dtype=np.float64)
+    # equal-quality plain time of the learned 4-spp frame
+    pl = sorted(plain, key=lambda q: q["spp"])
+    rels = np.array([p["rel_mse"] for p in pl])
+    secs = np.array([p["seconds"] for p in pl])
+    spps = np.array([p["spp"] for p in pl], dtype=np.float64)
     if len(plain) >= 2 and rel_l < rels[0]:
         # relMSE ~ 1/spp: interpolate log-log between the bracketing renders
         k = int(np.searchsorted(-rels, -rel_l))
