@@ -109,6 +109,10 @@ def main():
                     "default: whole 256^2 frames")
     ap.add_argument("--dn-load", type=str, default=None,
                     help="denoise: start from these weights (with --dn-steps 0: evaluate them only)")
+    ap.add_argument("--dn-input-spp", type=str, default="4,16,64,256",
+                    help="denoise: sample counts of the training inputs (comma-separated)")
+    ap.add_argument("--dn-head", choices=["residual", "kernel"], default="residual",
+                    help="denoise: network output, a log-colour correction or a 7x7 filter per pixel")
     ap.add_argument("--dn-no-noise", action="store_true",
                     help="denoise: train without the per-pixel noise input (channel 10 left at 0)")
     ap.add_argument("--dn-eval-size", type=str, default="1920x1080", help="denoise: evaluation resolution")
@@ -502,7 +506,7 @@ def denoise(args):
     from offlinerenderer_amd import denoiser
 
     footprint = tuple(int(v) for v in args.dn_footprint.split("x")) if args.dn_footprint else None
-    net = denoiser.UNet(seed=1)
+    net = denoiser.UNet(seed=1, head=args.dn_head)
     if args.dn_load:
         net.load(args.dn_load)
     t_data = t_train = 0.0
@@ -510,6 +514,7 @@ def denoise(args):
     if args.dn_steps > 0:
         t0 = time.perf_counter()
         data = dt.build_dataset(num_scenes=args.dn_scenes, cams_per_scene=args.dn_cams,
+                                noisy_spps=tuple(int(v) for v in args.dn_input_spp.split(",")),
                                 spp_target=args.dn_target_spp, footprint=footprint, log=log)
         t_data = time.perf_counter() - t0
         if args.dn_no_noise:
@@ -542,13 +547,14 @@ def denoise(args):
         "data": "synthetic (tracer-generated: seeded large-scene generator, random cameras)",
         "config": {
             "workload": f"denoise_{W}x{H}_4spp_unet",
-            "train_views": args.dn_scenes * args.dn_cams, "train_input_spp": [4, 16, 64, 256], "train_res": 256,
+            "train_views": args.dn_scenes * args.dn_cams, "train_input_spp": [int(v) for v in args.dn_input_spp.split(",")], "train_res": 256,
             "train_target_spp": args.dn_target_spp,
             "train_steps": args.dn_steps, "batch": "16 x 128^2 crops", "params": net.num_params,
             "train_view_kind": ("256^2 windows at the pixel scale of " + args.dn_footprint) if footprint
             else "whole 256^2 frames",
             "weights_loaded": args.dn_load,
             "loss": args.dn_loss,
+            "head": args.dn_head,
             "noise_input": not args.dn_no_noise,
             "eval_reference_spp": args.dn_ref_spp,
         },

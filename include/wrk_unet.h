@@ -73,6 +73,23 @@ uint64_t wrk_unet_loss_scratch_bytes(uint32_t n, uint32_t h, uint32_t w);
 int wrk_unet_loss(const uint16_t* pred, uint32_t pitch, const float* target, uint32_t srcH, uint32_t srcW,
                   const uint32_t* origins, const float* target2, const uint16_t* base, uint32_t basePitch,
                   uint16_t* dpred, uint32_t n, uint32_t h, uint32_t w, uint32_t mode, float* partial, void* stream);
+/* Kernel-predicting output (Bako et al. 2017), in the log domain:
+ * channels 0 .. wrk_kpn_taps() - 1 of pred (bf16, channel pitch >=
+ * wrk_kpn_taps() + 3) are the logits of a 7x7 filter per pixel (row-major,
+ * dy then dx) and the next 3 channels a per-channel offset d. out (n, h, w,
+ * 3) FP32 = max(exp(u) - 1, 0), u = sum_k softmax(logits)_k log(1 +
+ * max(c_k, 0)) + d over the 7x7 source neighbourhood of each pixel of the
+ * noisy colour (img, srcH, srcW, 3); origins as in wrk_unet_pack;
+ * neighbours past the image edges repeat its edge pixels. */
+uint32_t wrk_kpn_taps(void);
+int wrk_kpn_apply(const uint16_t* pred, uint32_t pitch, const float* color, uint32_t srcH, uint32_t srcW,
+                  const uint32_t* origins, float* out, uint32_t n, uint32_t h, uint32_t w, void* stream);
+/* the losses of wrk_unet_loss (modes 0..3) with q = u of wrk_kpn_apply,
+ * per-256-pixel partial sums, and their gradient with respect to the
+ * logits and offsets in dpred (0 beyond them) */
+int wrk_kpn_loss(const uint16_t* pred, uint32_t pitch, const float* color, const float* target, uint32_t srcH,
+                 uint32_t srcW, const uint32_t* origins, const float* target2, uint16_t* dpred, uint32_t n,
+                 uint32_t h, uint32_t w, uint32_t mode, float* partial, void* stream);
 /* Adam on FP32 parameters (step counts from 1) */
 int wrk_adam(float* p, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
              uint32_t step, void* stream);
@@ -82,6 +99,14 @@ int wrk_adam(float* p, const float* g, float* m, float* v, uint64_t n, float lr,
  * pointers for all image data; params as one FP32 vector. */
 const char* wrk_unet_last_error(void);
 void* wrk_unet_create(void);
+/* head 0 (wrk_unet_create): output channels 0..2 correct the input's
+ * log(1 + colour) (922,656 parameters); head 1: a 64-channel output whose
+ * first wrk_kpn_taps() + 3 channels are the per-pixel filter and offsets of
+ * wrk_kpn_apply (931,904 parameters). */
+void* wrk_unet_create_head(uint32_t head);
+uint32_t wrk_unet_head(void* net);
+/* parameter count of a network with this head (0 for an unknown head) */
+uint64_t wrk_unet_params_for_head(uint32_t head);
 void wrk_unet_destroy(void* net);
 uint64_t wrk_unet_num_params(void* net);
 int wrk_unet_init(void* net, uint64_t seed);

@@ -152,9 +152,17 @@ _SIGS = {
     "wrk_unet_loss_scratch_bytes": (C.c_uint64, [C.c_uint32] * 3),
     "wrk_unet_loss": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint32, _P,
                                 C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P]),
+    "wrk_kpn_taps": (C.c_uint32, []),
+    "wrk_kpn_apply": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, C.c_uint32,
+                                C.c_uint32, _P]),
+    "wrk_kpn_loss": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint32, C.c_uint32,
+                               C.c_uint32, C.c_uint32, _P, _P]),
     "wrk_adam": (C.c_int, [_P, _P, _P, _P, C.c_uint64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, _P]),
     "wrk_unet_last_error": (C.c_char_p, []),
     "wrk_unet_create": (_P, []),
+    "wrk_unet_create_head": (_P, [C.c_uint32]),
+    "wrk_unet_head": (C.c_uint32, [_P]),
+    "wrk_unet_params_for_head": (C.c_uint64, [C.c_uint32]),
     "wrk_unet_destroy": (None, [_P]),
     "wrk_unet_num_params": (C.c_uint64, [_P]),
     "wrk_unet_init": (C.c_int, [_P, C.c_uint64]),

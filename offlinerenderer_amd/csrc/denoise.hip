@@ -480,6 +480,156 @@ __global__ __launch_bounds__(256) void k_unet_loss(const bf16* __restrict__ pred
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// ------------------------------------------------------ kernel prediction
+// Kernel-predicting output (Bako et al. 2017, "Kernel-Predicting
+// Convolutional Networks for Denoising Monte Carlo Renderings"), applied in
+// the log domain as their specular branch is: channels 0 .. kKpnTaps - 1 of
+// the network output are the logits of a (2 kKpnRadius + 1)^2 filter per
+// pixel (row-major, dy then dx), normalised by a softmax and applied to
+// l = log(1 + max(c, 0)) of the noisy colour around the pixel's source
+// position; channels kKpnTaps .. kKpnTaps + 2 add a per-channel offset d
+// (the log-domain average of noisy samples sits below the log of their
+// mean). Output: exp(sum_k w_k l_k + d) - 1. Neighbours past the source
+// image's edges repeat its edge pixels. Filtering log colours keeps an
+// emitter's radiance from spreading into its dark neighbours at any weight.
+constexpr int kKpnRadius = 3;
+constexpr int kKpnSide = 2 * kKpnRadius + 1;
+constexpr int kKpnTaps = kKpnSide * kKpnSide;
+constexpr int kKpnChannels = kKpnTaps + 3;
+
+struct SrcPix {
+    uint32_t img, sx, sy;
+};
+// source image and position of output pixel p of n crops of h x w (origins,
+// 3 per crop) or of whole images (origins NULL; positions past the last
+// source row or column repeat it)
+__device__ __forceinline__ SrcPix src_pix(uint64_t p, uint32_t H, uint32_t W, const uint32_t* __restrict__ origins,
+                                          uint32_t srcH, uint32_t srcW) {
+    const uint32_t xx = (uint32_t)(p % W), yy = (uint32_t)((p / W) % H), n = (uint32_t)(p / ((uint64_t)W * H));
+    SrcPix r{n, xx, yy};
+    if (origins) {
+        r.img = origins[3 * n];
+        r.sx += origins[3 * n + 1];
+        r.sy += origins[3 * n + 2];
+    }
+    r.sx = min(r.sx, srcW - 1u);
+    r.sy = min(r.sy, srcH - 1u);
+    return r;
+}
+
+__device__ __forceinline__ const float* kpn_row(const float* color, SrcPix q, int dy, uint32_t srcH, uint32_t srcW) {
+    const uint32_t y = (uint32_t)min(max((int)q.sy + dy, 0), (int)srcH - 1);
+    return color + ((uint64_t)q.img * srcH + y) * srcW * 3;
+}
+__device__ __forceinline__ uint32_t kpn_col(SrcPix q, int dx, uint32_t srcW) {
+    return (uint32_t)min(max((int)q.sx + dx, 0), (int)srcW - 1);
+}
+
+// Filtered log colour of one pixel: u = sum_k w_k l_k + d, with w the
+// softmax of the logits; returns the softmax's max logit and 1 / sum exp.
+__device__ __forceinline__ void kpn_filter(const bf16* __restrict__ z, const float* __restrict__ color, SrcPix q,
+                                           uint32_t srcH, uint32_t srcW, float u[3], float* zmax, float* invSum) {
+    float m = -INFINITY;
+    for (int k = 0; k < kKpnTaps; ++k) m = fmaxf(m, (float)z[k]);
+    float sum = 0.0f, a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    for (int dy = -kKpnRadius; dy <= kKpnRadius; ++dy) {
+        const float* row = kpn_row(color, q, dy, srcH, srcW);
+        for (int dx = -kKpnRadius; dx <= kKpnRadius; ++dx) {
+            const uint32_t x = kpn_col(q, dx, srcW);
+            const float w = expf((float)z[(dy + kKpnRadius) * kKpnSide + dx + kKpnRadius] - m);
+            sum += w;
+            a0 += w * log1pf(fmaxf(row[x * 3 + 0], 0.0f));
+            a1 += w * log1pf(fmaxf(row[x * 3 + 1], 0.0f));
+            a2 += w * log1pf(fmaxf(row[x * 3 + 2], 0.0f));
+        }
+    }
+    const float inv = 1.0f / sum;
+    u[0] = a0 * inv + (float)z[kKpnTaps + 0];
+    u[1] = a1 * inv + (float)z[kKpnTaps + 1];
+    u[2] = a2 * inv + (float)z[kKpnTaps + 2];
+    *zmax = m;
+    *invSum = inv;
+}
+
+__global__ __launch_bounds__(256) void k_kpn_apply(const bf16* __restrict__ pred, uint32_t pitch,
+                                                   const float* __restrict__ color, uint32_t srcH, uint32_t srcW,
+                                                   const uint32_t* __restrict__ origins, float* __restrict__ out,
+                                                   uint32_t N, uint32_t H, uint32_t W) {
+    const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (uint64_t)N * H * W) return;
+    const SrcPix q = src_pix(p, H, W, origins, srcH, srcW);
+    float u[3], m, inv;
+    kpn_filter(pred + p * pitch, color, q, srcH, srcW, u, &m, &inv);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[p * 3 + c] = fmaxf(expm1f(u[c]), 0.0f);
+}
+
+// The losses of k_unet_loss (modes 0..3) with q = u, the filtered log colour
+// (y = exp(u) - 1), and their gradient with respect to the network output:
+// G_c = dL/du_c; logits dL/dz_k = w_k sum_c G_c (l_kc - sum_j w_j l_jc);
+// offsets dL/dd_c = G_c.
+__global__ __launch_bounds__(256) void k_kpn_loss(const bf16* __restrict__ pred, uint32_t pitch,
+                                                  const float* __restrict__ color, const float* __restrict__ target,
+                                                  uint32_t srcH, uint32_t srcW, const uint32_t* __restrict__ origins,
+                                                  const float* __restrict__ target2, bf16* __restrict__ dpred,
+                                                  uint32_t N, uint32_t H, uint32_t W, uint32_t mode,
+                                                  float* __restrict__ partial) {
+    __shared__ float red[256];
+    const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t npix = (uint64_t)N * H * W;
+    float l = 0.0f;
+    if (p < npix) {
+        const SrcPix q = src_pix(p, H, W, origins, srcH, srcW);
+        const bf16* z = pred + p * pitch;
+        float u[3], m, inv;
+        kpn_filter(z, color, q, srcH, srcW, u, &m, &inv);
+        const uint64_t s = ((uint64_t)q.img * srcH + q.sy) * srcW + q.sx;
+        const float scale = 2.0f / (float)(npix * 3);
+        float G[3], mean[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            mean[c] = u[c] - (float)z[kKpnTaps + c];
+            const float t = fmaxf(target[s * 3 + c], 0.0f);
+            if (mode >= 1) {
+                const float e = expf(fminf(u[c], 12.0f));
+                const float y = e - 1.0f;
+                const float t2 = mode == 3 ? fmaxf(target2[s * 3 + c], 0.0f) : t;
+                const float den = 1.0f / (mode == 2 ? y * y + 0.01f : t2 * t2 + 0.01f);
+                const float d = y - t;
+                l += d * d * den;
+                G[c] = u[c] < 12.0f ? d * den * e * scale : 0.0f;
+            } else {
+                const float d = u[c] - log1pf(t);
+                l += d * d;
+                G[c] = d * scale;
+            }
+        }
+        bf16* dz = dpred + p * pitch;
+        for (int dy = -kKpnRadius; dy <= kKpnRadius; ++dy) {
+            const float* row = kpn_row(color, q, dy, srcH, srcW);
+            for (int dx = -kKpnRadius; dx <= kKpnRadius; ++dx) {
+                const uint32_t xx = kpn_col(q, dx, srcW);
+                const int k = (dy + kKpnRadius) * kKpnSide + dx + kKpnRadius;
+                const float w = expf((float)z[k] - m) * inv;
+                const float e = G[0] * (log1pf(fmaxf(row[xx * 3 + 0], 0.0f)) - mean[0]) +
+                                G[1] * (log1pf(fmaxf(row[xx * 3 + 1], 0.0f)) - mean[1]) +
+                                G[2] * (log1pf(fmaxf(row[xx * 3 + 2], 0.0f)) - mean[2]);
+                dz[k] = f2bf(w * e);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dz[kKpnTaps + c] = f2bf(G[c]);
+        for (uint32_t c = kKpnChannels; c < pitch; ++c) dz[c] = f2bf(0.0f);
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 // ---------------------------------------------------------------- weights
 // FP32 master [9][cout][cin] -> bf16 forward layout and the tap-flipped
 // transposed layout used by the data gradient.
@@ -688,6 +838,40 @@ extern "C" int wrk_unet_loss(const uint16_t* pred, uint32_t pitch, const float* 
     if (np == 0) return WRK_OK;
     hipLaunchKernelGGL(k_unet_loss, dim3(ceil_div(np, 256)), dim3(256), 0, as_stream(stream), (const bf16*)pred,
                        pitch, target, srcH, srcW, origins, target2, (const bf16*)base, basePitch, (bf16*)dpred, n, h, w,
+                       mode, partial);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" uint32_t wrk_kpn_taps(void) { return (uint32_t)kKpnTaps; }
+
+extern "C" int wrk_kpn_apply(const uint16_t* pred, uint32_t pitch, const float* color, uint32_t srcH, uint32_t srcW,
+                             const uint32_t* origins, float* out, uint32_t n, uint32_t h, uint32_t w, void* stream) {
+    WRK_REQUIRE(pred && color && out, "wrk_kpn_apply: null argument");
+    WRK_REQUIRE(pitch >= (uint32_t)kKpnChannels, "wrk_kpn_apply: pitch below the filter's channel count");
+    WRK_REQUIRE(srcH > 0 && srcW > 0, "wrk_kpn_apply: empty source image");
+    WRK_REQUIRE(origins || (h >= srcH && w >= srcW), "wrk_kpn_apply: whole images must cover the source");
+    const uint64_t npix = (uint64_t)n * h * w;
+    if (npix == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_kpn_apply, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, as_stream(stream),
+                       (const bf16*)pred, pitch, color, srcH, srcW, origins, out, n, h, w);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
+extern "C" int wrk_kpn_loss(const uint16_t* pred, uint32_t pitch, const float* color, const float* target,
+                            uint32_t srcH, uint32_t srcW, const uint32_t* origins, const float* target2,
+                            uint16_t* dpred, uint32_t n, uint32_t h, uint32_t w, uint32_t mode, float* partial,
+                            void* stream) {
+    WRK_REQUIRE(pred && color && target && dpred && partial, "wrk_kpn_loss: null argument");
+    WRK_REQUIRE(pitch >= (uint32_t)kKpnChannels, "wrk_kpn_loss: pitch below the filter's channel count");
+    WRK_REQUIRE(mode <= 3, "wrk_kpn_loss: unknown loss mode");
+    WRK_REQUIRE(mode != 3 || target2, "wrk_kpn_loss: mode 3 needs target2");
+    WRK_REQUIRE(srcH > 0 && srcW > 0, "wrk_kpn_loss: empty source image");
+    const uint64_t npix = (uint64_t)n * h * w;
+    if (npix == 0) return WRK_OK;
+    hipLaunchKernelGGL(k_kpn_loss, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, as_stream(stream),
+                       (const bf16*)pred, pitch, color, target, srcH, srcW, origins, target2, (bf16*)dpred, n, h, w,
                        mode, partial);
     WRK_CHECK_LAUNCH();
     return WRK_OK;

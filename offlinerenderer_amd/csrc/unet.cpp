@@ -65,11 +65,21 @@ struct ConvSpec {
 
 class UNet {
 public:
-    UNet() {
+    // head 0: channels 0..2 of a 32-channel output correct the input's
+    // log(1 + colour); head 1: channels 0..48 of a 64-channel output are a
+    // 7x7 filter per pixel applied to the noisy log colour and channels
+    // 49..51 a per-channel offset (wrk_kpn_apply)
+    // parameter count of a network with this head (without building one)
+    static uint64_t paramsFor(uint32_t head) {
+        if (head > 1) throw std::runtime_error("wrk_unet: unknown output head");
+        return head ? 931904ull : 922656ull;
+    }
+    explicit UNet(uint32_t head) : head_(head) {
+        if (head > 1) throw std::runtime_error("wrk_unet: unknown output head");
         // c0a c0b | c1a c1b | c2a c2b | c3a c3b | d2a d2b | d1a d1b | d0a d0b | out
         const uint32_t spec[15][3] = {{32, 32, 0},   {32, 32, 0},   {32, 64, 1},  {64, 64, 1},   {64, 96, 2},
                                       {96, 96, 2},   {96, 128, 3},  {128, 128, 3}, {224, 96, 2}, {96, 96, 2},
-                                      {160, 64, 1},  {64, 64, 1},   {96, 32, 0},   {32, 32, 0},  {32, 32, 0}};
+                                      {160, 64, 1},  {64, 64, 1},   {96, 32, 0},   {32, 32, 0},  {32, head ? 64u : 32u, 0}};
         uint64_t off = 0;
         for (int i = 0; i < 15; ++i) {
             ConvSpec c{spec[i][0], spec[i][1], spec[i][2], i != 14, 0, 0};
@@ -80,6 +90,7 @@ public:
             conv_.push_back(c);
         }
         nParams_ = off;
+        if (nParams_ != paramsFor(head)) throw std::runtime_error("wrk_unet: layer table and parameter count disagree");
         P_.need(off * 4); G_.need(off * 4); M_.need(off * 4); V_.need(off * 4);
         hip_ok(hipMemset(M_.p, 0, off * 4), "hipMemset");
         hip_ok(hipMemset(V_.p, 0, off * 4), "hipMemset");
@@ -94,6 +105,7 @@ public:
     }
 
     uint64_t numParams() const { return nParams_; }
+    uint32_t head() const { return head_; }
 
     // He-normal weights over the real input channels, zero biases; the
     // output convolution starts small and only its first 3 channels live.
@@ -103,7 +115,7 @@ public:
         for (int i = 0; i < 15; ++i) {
             const ConvSpec& c = conv_[i];
             const uint32_t realIn = i == 0 ? 9 : c.cin;
-            const uint32_t realOut = i == 14 ? 3 : c.cout;
+            const uint32_t realOut = i == 14 ? (head_ ? wrk_kpn_taps() + 3u : 3u) : c.cout;
             std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / (9.0f * realIn)) * (i == 14 ? 0.1f : 1.0f));
             for (uint32_t tap = 0; tap < 9; ++tap)
                 for (uint32_t co = 0; co < realOut; ++co)
@@ -162,7 +174,9 @@ public:
                              featCount_, in_.buf.h(), n, h, w, s),
                "wrk_unet_pack");
         runForward(s);
-        if (out)
+        if (out && head_)
+            wrk_ok(wrk_kpn_apply(pred_.buf.h(), pred_.c, color, srcH, srcW, origins, out, n, h, w, s), "wrk_kpn_apply");
+        else if (out)
             wrk_ok(wrk_unet_unpack(pred_.buf.h(), 32, in_.buf.h(), 32, out, (uint64_t)n * h * w, s), "wrk_unet_unpack");
     }
 
@@ -175,9 +189,14 @@ public:
                              featCount_, in_.buf.h(), n, h, w, s),
                "wrk_unet_pack");
         runForward(s);
-        wrk_ok(wrk_unet_loss(pred_.buf.h(), 32, target, srcH, srcW, origins, target2, in_.buf.h(), 32, gPred_.buf.h(), n, h,
-                             w, lossMode, lossPartial_.f(), s),
-               "wrk_unet_loss");
+        if (head_)
+            wrk_ok(wrk_kpn_loss(pred_.buf.h(), pred_.c, color, target, srcH, srcW, origins, target2, gPred_.buf.h(), n,
+                                h, w, lossMode, lossPartial_.f(), s),
+                   "wrk_kpn_loss");
+        else
+            wrk_ok(wrk_unet_loss(pred_.buf.h(), 32, target, srcH, srcW, origins, target2, in_.buf.h(), 32,
+                                 gPred_.buf.h(), n, h, w, lossMode, lossPartial_.f(), s),
+                   "wrk_unet_loss");
         runBackward(s);
         ++step_;
         wrk_ok(wrk_adam(P_.f(), G_.f(), M_.f(), V_.f(), nParams_, lr, 0.9f, 0.999f, 1e-8f, step_, s), "wrk_adam");
@@ -210,12 +229,12 @@ private:
             a.buf.need((size_t)n * (h >> lvl) * (w >> lvl) * c * 2);
         };
         alloc(in_, 32, 0); alloc(a0_, 32, 0); alloc(cat0_, 96, 0); alloc(t0_, 32, 0); alloc(d0_, 32, 0);
-        alloc(pred_, 32, 0);
+        alloc(pred_, conv_[14].cout, 0);
         alloc(p1_, 32, 1); alloc(a1_, 64, 1); alloc(cat1_, 160, 1); alloc(t1_, 64, 1); alloc(d1_, 64, 1);
         alloc(p2_, 64, 2); alloc(a2_, 96, 2); alloc(cat2_, 224, 2); alloc(t2_, 96, 2); alloc(d2_, 96, 2);
         alloc(p3_, 96, 3); alloc(a3_, 128, 3); alloc(b3_, 128, 3);
         if (!training) return;
-        alloc(gPred_, 32, 0); alloc(g0a_, 32, 0); alloc(g0b_, 32, 0); alloc(gCat0_, 96, 0);
+        alloc(gPred_, conv_[14].cout, 0); alloc(g0a_, 32, 0); alloc(g0b_, 32, 0); alloc(gCat0_, 96, 0);
         alloc(g1a_, 64, 1); alloc(g1b_, 64, 1); alloc(gCat1_, 160, 1); alloc(gP1_, 32, 1);
         alloc(g2a_, 96, 2); alloc(g2b_, 96, 2); alloc(gCat2_, 224, 2); alloc(gP2_, 64, 2);
         alloc(g3a_, 128, 3); alloc(g3b_, 128, 3); alloc(gP3_, 96, 3);
@@ -357,6 +376,7 @@ private:
                    "wrk_weights_bf16");
     }
 
+    uint32_t head_ = 0;
     std::vector<ConvSpec> conv_;
     uint64_t nParams_ = 0;
     uint32_t step_ = 0;
@@ -394,10 +414,21 @@ extern "C" {
 
 const char* wrk_unet_last_error(void) { return g_unetError.c_str(); }
 
-void* wrk_unet_create(void) {
+void* wrk_unet_create(void) { return wrk_unet_create_head(0); }
+void* wrk_unet_create_head(uint32_t head) {
     UNet* u = nullptr;
-    if (guard([&] { u = new UNet(); }) != WRK_OK) return nullptr;
+    if (guard([&] { u = new UNet(head); }) != WRK_OK) return nullptr;
     return u;
+}
+uint32_t wrk_unet_head(void* net) {
+    uint32_t h = 0;
+    guard([&] { h = N(net)->head(); });
+    return h;
+}
+uint64_t wrk_unet_params_for_head(uint32_t head) {
+    uint64_t n = 0;
+    guard([&] { n = UNet::paramsFor(head); });
+    return n;
 }
 void wrk_unet_destroy(void* net) { delete static_cast<UNet*>(net); }
 uint64_t wrk_unet_num_params(void* net) {

@@ -392,12 +392,21 @@ struct RendererImpl {
         if (!f) throw std::runtime_error("loadDenoiser: cannot open " + path);
         const std::streamsize bytes = f.tellg();
         f.seekg(0);
-        if (!unet) unet = wrk_unet_create();
+        // the file's parameter count names the network's output head
+        int head = -1;
+        for (uint32_t h = 0; h < 2; ++h)
+            if ((uint64_t)bytes == wrk_unet_params_for_head(h) * sizeof(float)) head = (int)h;
+        if (head < 0)
+            throw std::runtime_error("loadDenoiser: " + path + " does not hold the " +
+                                     std::to_string(wrk_unet_params_for_head(0)) + " or " +
+                                     std::to_string(wrk_unet_params_for_head(1)) + " FP32 parameters of a network");
+        if (unet && wrk_unet_head(unet) != (uint32_t)head) {
+            wrk_unet_destroy(unet);
+            unet = nullptr;
+        }
+        if (!unet) unet = wrk_unet_create_head((uint32_t)head);
         if (!unet) throw std::runtime_error(std::string("loadDenoiser: ") + wrk_unet_last_error());
         const uint64_t n = wrk_unet_num_params(unet);
-        if ((uint64_t)bytes != n * sizeof(float))
-            throw std::runtime_error("loadDenoiser: " + path + " does not hold " + std::to_string(n) +
-                                     " FP32 parameters");
         std::vector<float> p(n);
         f.read(reinterpret_cast<char*>(p.data()), bytes);
         if (wrk_unet_set_params(unet, p.data()) != WRK_OK)

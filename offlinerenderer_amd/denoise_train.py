@@ -272,7 +272,7 @@ def evaluate(net: denoiser.UNet, width=1920, height=1080, seed=9001, spp_noisy=4
         "learned_seconds_to_bar": learned_hit["seconds"] if learned_hit else None,
         "noisy_rel_mse": rel_mse(noisy, ref), "noisy_seconds": t_noisy,
         "inference_ms": t_inf * 1e3,
-        "inference_tflops": denoiser.flops_per_pixel() * width * height / t_inf / 1e12,
+        "inference_tflops": denoiser.flops_per_pixel(net.head) * width * height / t_inf / 1e12,
     }
     # equal-quality plain time of the learned 4-spp frame
     pl = sorted(plain, key=lambda q: q["spp"])

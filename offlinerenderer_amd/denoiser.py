@@ -16,15 +16,25 @@ import numpy as np
 
 from . import lib
 
-# 15 convolutions: (cin, cout, level); weights [9][cout][cin] then bias [cout]
+# 15 convolutions: (cin, cout, level); weights [9][cout][cin] then bias [cout].
+# Output heads: 0 = channels 0..2 of a 32-channel output correct the input's
+# log(1 + colour); 1 = the first KPN_TAPS channels of a 64-channel output are
+# a 7x7 filter per pixel over the noisy colour (kernel prediction).
 LAYERS = [(32, 32, 0), (32, 32, 0), (32, 64, 1), (64, 64, 1), (64, 96, 2), (96, 96, 2), (96, 128, 3), (128, 128, 3),
           (224, 96, 2), (96, 96, 2), (160, 64, 1), (64, 64, 1), (96, 32, 0), (32, 32, 0), (32, 32, 0)]
+KPN_RADIUS = 3
+KPN_TAPS = (2 * KPN_RADIUS + 1) ** 2
+HEADS = {"residual": 0, "kernel": 1}
 
 
-def layer_slices():
+def layers(head: int = 0):
+    return LAYERS[:-1] + [(32, 64 if head else 32, 0)]
+
+
+def layer_slices(head: int = 0):
     """(weight slice, bias slice) of each convolution in the parameter vector."""
     out, off = [], 0
-    for cin, cout, _ in LAYERS:
+    for cin, cout, _ in layers(head):
         w = slice(off, off + 9 * cin * cout)
         off += 9 * cin * cout
         b = slice(off, off + cout)
@@ -33,9 +43,9 @@ def layer_slices():
     return out
 
 
-def flops_per_pixel() -> float:
+def flops_per_pixel(head: int = 0) -> float:
     """Forward FLOPs per full-resolution pixel (2 per multiply-add)."""
-    return sum(2.0 * 9 * cin * cout / (4 ** lvl) for cin, cout, lvl in LAYERS)
+    return sum(2.0 * 9 * cin * cout / (4 ** lvl) for cin, cout, lvl in layers(head))
 
 
 def _err():
@@ -47,10 +57,11 @@ def _ptr(t):
 
 
 class UNet:
-    def __init__(self, seed: int = 1):
-        self._h = lib().wrk_unet_create()
+    def __init__(self, seed: int = 1, head: int | str = 0):
+        self.head = HEADS[head] if isinstance(head, str) else int(head)
+        self._h = lib().wrk_unet_create_head(self.head)
         if not self._h:
-            raise RuntimeError(f"wrk_unet_create: {_err()}")
+            raise RuntimeError(f"wrk_unet_create_head: {_err()}")
         self.num_params = int(lib().wrk_unet_num_params(self._h))
         self.init(seed)
 

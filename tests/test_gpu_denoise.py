@@ -424,3 +424,125 @@ def test_renderer_denoises_sizes_not_divisible_by_8(gpu, h, w):
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy()[0][:h, :w], den)
     assert not np.array_equal(den, col)
+
+
+def _kpn_ref(z, color, origins, h, w):
+    """PyTorch FP64 restatement of the kernel-prediction output: z (n, K + 3,
+    h, w) = filter logits and per-channel offsets, color (imgs, H, W, 3);
+    crop k reads image origins[k][0] at (x0, y0) = origins[k][1:],
+    neighbours clamped to the image. Returns u = sum_k softmax(logits)_k
+    log(1 + max(c_k, 0)) + offset, (n, 3, h, w); the output is exp(u) - 1."""
+    R = denoiser.KPN_RADIUS
+    side = 2 * R + 1
+    K = side * side
+    imgs, H, W, _ = color.shape
+    out = []
+    for k, (img, x0, y0) in enumerate(origins):
+        ys = torch.arange(h) + y0
+        xs = torch.arange(w) + x0
+        wk = torch.softmax(z[k, :K], dim=0)  # (K, h, w)
+        acc = torch.zeros((3, h, w), dtype=torch.float64)
+        for dy in range(-R, R + 1):
+            yy = (ys + dy).clamp(0, H - 1)
+            for dx in range(-R, R + 1):
+                xx = (xs + dx).clamp(0, W - 1)
+                c = torch.log1p(color[img][yy][:, xx].clamp(min=0)).permute(2, 0, 1)  # (3, h, w)
+                acc = acc + wk[(dy + R) * side + dx + R] * c
+        out.append(acc + z[k, K:K + 3])
+    return torch.stack(out)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 3])
+def test_kpn_apply_and_loss_match_torch(gpu, mode):
+    rng = np.random.default_rng(31 + mode)
+    imgs, H, W = 2, 20, 28
+    n, h, w, pitch = 3, 8, 16, 64
+    K = denoiser.KPN_TAPS
+    assert int(wr.lib().wrk_kpn_taps()) == K
+    color = (rng.gamma(0.7, 0.6, size=(imgs, H, W, 3)) - 0.02).astype(np.float32)  # a few negatives, clamped
+    target = rng.gamma(0.7, 0.6, size=(imgs, H, W, 3)).astype(np.float32)
+    target2 = rng.gamma(0.7, 0.6, size=(imgs, H, W, 3)).astype(np.float32)
+    z = _rand(rng, (n, h, w, pitch), 1.5)
+    z[..., K:K + 3] = U.bf16_round(z[..., K:K + 3] * 0.1)  # offsets
+    # crops at the image edges, so the 7x7 neighbourhoods reach past them
+    origins = [(0, 0, 0), (1, W - w, H - h), (1, 5, 7)]
+    zt = torch.from_numpy(z[..., :K + 3]).permute(0, 3, 1, 2).double().requires_grad_(True)
+    u = _kpn_ref(zt, torch.from_numpy(color).double(), origins, h, w)
+    y = torch.expm1(u)
+    o = torch.tensor(origins, dtype=torch.int32, device="cuda").view(-1)
+    out = torch.zeros((n, h, w, 3), dtype=torch.float32, device="cuda")
+    zb = _dev_bits(z)
+    cd = torch.from_numpy(color).cuda()
+    rc = wr.lib().wrk_kpn_apply(C.c_void_p(zb.data_ptr()), pitch, C.c_void_p(cd.data_ptr()), H, W,
+                                C.c_void_p(o.data_ptr()), C.c_void_p(out.data_ptr()), n, h, w, None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    want = y.detach().clamp(min=0).permute(0, 2, 3, 1).numpy()
+    np.testing.assert_allclose(got, want, rtol=3e-5, atol=3e-6)
+    # loss and gradient against autograd of the same expression
+    t = torch.from_numpy(target).double()
+    t2 = torch.from_numpy(target2).double()
+    tc = torch.stack([t[img, y0:y0 + h, x0:x0 + w] for img, x0, y0 in origins]).permute(0, 3, 1, 2)
+    t2c = torch.stack([t2[img, y0:y0 + h, x0:x0 + w] for img, x0, y0 in origins]).permute(0, 3, 1, 2)
+    if mode == 0:
+        per = (u - torch.log1p(tc)) ** 2
+    else:
+        den = (t2c if mode == 3 else tc) ** 2 + 0.01
+        per = (y - tc) ** 2 / den
+    per.mean().backward()
+    dz = torch.zeros((n, h, w, pitch), dtype=torch.int16, device="cuda")
+    part = torch.zeros(((n * h * w + 255) // 256,), dtype=torch.float32, device="cuda")
+    td, t2d = torch.from_numpy(target).cuda(), torch.from_numpy(target2).cuda()
+    rc = wr.lib().wrk_kpn_loss(C.c_void_p(zb.data_ptr()), pitch, C.c_void_p(cd.data_ptr()),
+                               C.c_void_p(td.data_ptr()), H, W, C.c_void_p(o.data_ptr()),
+                               C.c_void_p(t2d.data_ptr()), C.c_void_p(dz.data_ptr()), n, h, w, mode,
+                               C.c_void_p(part.data_ptr()), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    loss = float(part.sum()) / (3 * n * h * w)
+    want_loss = float(per.detach().mean())
+    assert abs(loss - want_loss) <= 2e-5 * abs(want_loss) + 1e-9
+    g = U.from_bits(dz.cpu().numpy())
+    want_g = zt.grad.permute(0, 2, 3, 1).numpy()
+    # gradients are stored in bf16: within one bf16 rounding of the FP64 value
+    np.testing.assert_allclose(g[..., :K + 3], want_g, rtol=2 ** -8, atol=1e-3 * np.abs(want_g).max())
+    assert not g[..., K + 3:].any()
+
+
+def test_kpn_network_trains_and_loads_by_size(gpu, tmp_path):
+    rng = np.random.default_rng(41)
+    n, h, w = 2, 32, 32
+    color, albedo, nrm = _images(rng, n, h, w)
+    target = U.bf16_round(color * 0.5 + 0.1)
+    net = denoiser.UNet(seed=5, head="kernel")
+    assert net.num_params == int(wr.lib().wrk_unet_params_for_head(1)) != int(wr.lib().wrk_unet_params_for_head(0))
+    dev = [torch.from_numpy(a).cuda() for a in (color, albedo, nrm, target)]
+    losses = [net.train_step(*dev, lr=2e-3, loss="relmse") for _ in range(40)]
+    assert losses[-1] < 0.7 * losses[0], losses[::8]
+    # the forward equals the filter of the network's own logits
+    out = torch.zeros((n, h, w, 3), dtype=torch.float32, device="cuda")
+    net.forward(*dev[:3], out)
+    torch.cuda.synchronize()
+    assert np.isfinite(out.cpu().numpy()).all()
+    # Renderer.loadDenoiser picks the head from the file's parameter count
+    path = tmp_path / "kpn.f32"
+    net.save(str(path))
+    d = scenes.cornell(24, 16, 4)
+    s = wr.Scene()
+    d.build(s)
+    s.finalize()
+    r = wr.Renderer()
+    r.loadDenoiser(str(path))
+    p = wr.RenderParams(width=24, height=16, spp=4, maxBounces=8, russianRouletteDepth=3.0)
+    p.denoiser.enabled = True
+    img = r.render(s, wr.Camera(**d.camera), p)
+    p.denoiser.enabled = False  # the noisy layers the network takes
+    col, alb, nr, nse = r.renderWithGuides(s, wr.Camera(**d.camera), p, noise=True)
+    net.set_sample_counts([4])
+    nd = torch.from_numpy(nse).cuda().unsqueeze(0)
+    net.set_noise(nd)
+    ref = torch.zeros((1, 16, 24, 3), dtype=torch.float32, device="cuda")
+    net.forward(*[torch.from_numpy(a).cuda().unsqueeze(0) for a in (col, alb, nr)], ref)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(img, ref[0].cpu().numpy())
