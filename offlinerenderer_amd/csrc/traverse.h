@@ -20,33 +20,42 @@ __device__ __forceinline__ float slab_inv(float d) { return 1.0f / (fabsf(d) > 1
 // 0-3 | child 4-7 | 48 quantised planes): code[c] is the child word and
 // key[c] the near distance of the ray (origin o, reciprocal direction ix,
 // iy, iz) into box c over [tmin, tbest], or +inf on a miss or an empty slot.
-// Boxes are dequantised as origin + q * 2^e per axis.
+// A plane origin + q * 2^e sits at ray distance (origin + q * 2^e - o) * inv
+// = q * (2^e * inv) + (origin - o) * inv, one fma per plane after a per-node
+// setup (2^e * inv is exact). Per axis the ray's direction sign names the
+// near and the far plane (lo then hi for inv >= 0), so no per-child min/max
+// of the two slab distances is needed. The build pads every box outward by
+// 2^-20 * (|coordinate| + scene extent) (kBoxPadRel), several times the
+// rounding of either distance form for origins within a few scene extents,
+// and far distances are scaled by kRobust, so a box that may hold the
+// closest hit is never culled.
 __device__ __forceinline__ void node8_children(const uint4* __restrict__ raw, float4 o, float ix, float iy, float iz,
                                                float tmin, float tbest, float key[8], uint32_t code[8]) {
-    auto slab = [&](float lx, float ly, float lz, float hx, float hy, float hz, uint32_t cc) -> float {
-        float t0x = (lx - o.x) * ix, t1x = (hx - o.x) * ix;
-        float t0y = (ly - o.y) * iy, t1y = (hy - o.y) * iy;
-        float t0z = (lz - o.z) * iz, t1z = (hz - o.z) * iz;
-        float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-        float tf = fminf(fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)), tbest) * kRobust;
-        return (cc != kEmptyChild && tn <= tf) ? tn : INFINITY;
-    };
     const uint4 r0 = raw[0], r1 = raw[1], r2 = raw[2], r3 = raw[3], r4 = raw[4], r5 = raw[5];
     const float ox = __uint_as_float(r0.x), oy = __uint_as_float(r0.y), oz = __uint_as_float(r0.z);
     const float sx = __uint_as_float((r0.w & 0xFFu) << 23);
     const float sy = __uint_as_float(((r0.w >> 8) & 0xFFu) << 23);
     const float sz = __uint_as_float(((r0.w >> 16) & 0xFFu) << 23);
+    const float ax = sx * ix, ay = sy * iy, az = sz * iz;
+    const float bx = (ox - o.x) * ix, by = (oy - o.y) * iy, bz = (oz - o.z) * iz;
     code[0] = r1.x; code[1] = r1.y; code[2] = r1.z; code[3] = r1.w;
     code[4] = r2.x; code[5] = r2.y; code[6] = r2.z; code[7] = r2.w;
     // plane words: qlo x (r3.x r3.y) y (r3.z r3.w) z (r4.x r4.y),
     //              qhi x (r4.z r4.w) y (r5.x r5.y) z (r5.z r5.w)
-    const uint32_t qw[12] = {r3.x, r3.y, r3.z, r3.w, r4.x, r4.y, r4.z, r4.w, r5.x, r5.y, r5.z, r5.w};
+    const bool px = ix >= 0.0f, py = iy >= 0.0f, pz = iz >= 0.0f;
+    const uint32_t nx[2] = {px ? r3.x : r4.z, px ? r3.y : r4.w}, fx[2] = {px ? r4.z : r3.x, px ? r4.w : r3.y};
+    const uint32_t ny[2] = {py ? r3.z : r5.x, py ? r3.w : r5.y}, fy[2] = {py ? r5.x : r3.z, py ? r5.y : r3.w};
+    const uint32_t nz[2] = {pz ? r4.x : r5.z, pz ? r4.y : r5.w}, fz[2] = {pz ? r5.z : r4.x, pz ? r5.w : r4.y};
+    const float tfar = tbest;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int wi = c >> 2, sh = 8 * (c & 3);
-        auto q = [&](int word) { return (float)((qw[word + wi] >> sh) & 0xFFu); };
-        key[c] = slab(fmaf(q(0), sx, ox), fmaf(q(2), sy, oy), fmaf(q(4), sz, oz), fmaf(q(6), sx, ox),
-                      fmaf(q(8), sy, oy), fmaf(q(10), sz, oz), code[c]);
+        auto q = [&](uint32_t word) { return (float)((word >> sh) & 0xFFu); };
+        const float tn = fmaxf(fmaxf(fmaf(q(nx[wi]), ax, bx), fmaf(q(ny[wi]), ay, by)),
+                               fmaxf(fmaf(q(nz[wi]), az, bz), tmin));
+        const float tf = fminf(fminf(fmaf(q(fx[wi]), ax, bx), fmaf(q(fy[wi]), ay, by)),
+                               fminf(fmaf(q(fz[wi]), az, bz), tfar)) * kRobust;
+        key[c] = (code[c] != kEmptyChild && tn <= tf) ? tn : INFINITY;
     }
 }
 
