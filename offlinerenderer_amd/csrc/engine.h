@@ -169,6 +169,8 @@ private:
     bool admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase);
     template <typename F> void timed(int category, hipStream_t stream, F&& launch);
     uint64_t& footprintUsed() { return cfg_.sharedFootprint ? *cfg_.sharedFootprint : footprintUsed_; }
+    DevBuf takeBuf(size_t bytes);
+    void giveBuf(DevBuf& b);  // keeps b's memory for later requests (or frees it past the pool's limit)
     // pool ranges are whole granules of 256 entries, so one byte per granule
     // (slotMap_) names the slot that owns an entry
     static uint32_t granuleRound(uint32_t n) { return (n + 255u) & ~255u; }
@@ -204,6 +206,11 @@ private:
     uint64_t footprintUsed_ = 0;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks_;
     std::vector<hipEvent_t> eventPool_;
+    // freed request buffers by size, reused by later requests (at most
+    // kBufPoolBytes held; 512 MB is twenty 1080p accumulation buffers)
+    static constexpr size_t kBufPoolBytes = size_t(512) << 20;
+    std::multimap<size_t, DevBuf> bufPool_;
+    size_t pooledBytes_ = 0;
     size_t eventNext_ = 0;
 };
 
