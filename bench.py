@@ -309,7 +309,8 @@ def measure_render(args, workload_kind, W, H, spp, steps, warmup):
             "traffic_source": traffic_src,
             "measured_gbs": round(measured_gbs, 2) if measured_gbs else None,
             "measured_frac": round(measured_gbs / HBM_PEAK_GBS, 4) if measured_gbs else None,
-            "limiter": "latency of dependent node fetches and lane divergence, not HBM bandwidth: the "
+            "limiter": "node-test instruction issue and the latency of dependent node fetches, with under half "
+                       "of each wave's lanes active, not HBM bandwidth: the "
                        "measured HBM traffic is below the algorithmic bytes (cache hits) and below 8 TB/s",
             "algorithmic_bytes_per_launch": round(bytes_per_ray * rays_c / max(launches_c, 1)),
             "bytes_per_ray": round(bytes_per_ray, 2),
