@@ -652,10 +652,10 @@ __global__ __launch_bounds__(256) void k_reorder_tris(const uint32_t* __restrict
 #define WRK_TRACE_WAVES 8
 #endif
 #ifndef WRK_LEAF_BATCH_NUM
-#define WRK_LEAF_BATCH_NUM 1
+#define WRK_LEAF_BATCH_NUM 5
 #endif
 #ifndef WRK_LEAF_BATCH_DEN
-#define WRK_LEAF_BATCH_DEN 2
+#define WRK_LEAF_BATCH_DEN 8
 #endif
 // What a finished ray does to its path when the traversal also resolves the
 // result (the per-entry decisions of wrk_classify, taken where the ray ends
@@ -793,7 +793,11 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
             // all of the lanes gave material 1269, 1411, 1434, 1420, 1259
             // Mpaths/s; without the batching, a lane tests a leaf as soon as
             // it reaches one, 1140; with 2-triangle leaves 1/2, 5/8, 3/4 gave
-            // material 1486, 1485, 1446 and large 1061, 1038, 979).
+            // material 1486, 1485, 1446 and large 1061, 1038, 979; after the
+            // node test became one fma per plane, 3/8, 1/2, 5/8, 3/4 gave
+            // material 1901, 1953-1964, 1985-1996, 1974 and large 1189,
+            // 1200-1205, 1197-1204, 1159: the cheaper node steps leave the
+            // leaf steps relatively dearer, so the wave waits for more of them).
             const uint32_t nAct = __popcll(__ballot(true));
             const uint32_t nPend = __popcll(__ballot(leaf != kEmptyChild));
             const bool canStep = __ballot(cur != kEmptyChild && !(cur & kLeafBit)) != 0;
