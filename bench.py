@@ -90,6 +90,9 @@ def main():
     ap.add_argument("--serial-max-paths", type=int, default=1 << 27,
                     help="serving: paths in flight of the serial per-request renders")
     ap.add_argument("--express-pool", type=int, default=1 << 23, help="serving: express lane paths in flight")
+    ap.add_argument("--backfill-paths", type=int, default=0,
+                    help="serving: pool entries the requests behind the oldest unfinished one may hold while it has "
+                    "paths in flight (0 = no limit)")
     ap.add_argument("--express-iters", type=int, default=32,
                     help="serving: express lane iterations per server step")
     ap.add_argument("--express-overlap", action="store_true",
@@ -395,7 +398,8 @@ def serving(args):
                                               expressPathsInFlight=args.express_pool,
                                               expressIterations=args.express_iters,
                                               expressOverlap=args.express_overlap,
-                                              fusedMaxPaths=args.fused_max_paths))
+                                              fusedMaxPaths=args.fused_max_paths,
+                                              backfillPaths=args.backfill_paths))
         ids, done_t, images = [None] * len(jobs), np.zeros(len(jobs)), [None] * len(jobs)
         open_ids = {}
         t0 = srv.clock()
@@ -470,6 +474,7 @@ def serving(args):
             "paths_in_flight": pool, "chunk_paths": chunk, "serial_paths_in_flight": serial_pool,
             "fused_max_paths": args.fused_max_paths, "serial_fused_max_paths": args.serial_fused_max_paths,
             "admission": args.policy + (" round-robin" if args.round_robin else " in order"),
+            "backfill_paths": args.backfill_paths or None,
             "express_lane": ({"max_paths": args.express_max_paths, "paths_in_flight": args.express_pool,
                               "iterations_per_step": args.express_iters,
                               "schedule": "overlapped" if args.express_overlap else "time-sliced"}

@@ -157,6 +157,11 @@ struct ServerConfig {
     // requests of at most this many paths render in one fused launch per
     // chunk (as WavefrontConfig::fusedMaxPaths) in the lane they take
     uint64_t fusedMaxPaths = 0;
+    // in-order admission: while the oldest unfinished request has paths in
+    // flight, the requests behind it share at most backfillPaths of the pool
+    // (0 = no limit), so they fill its draining tail without stretching its
+    // iterations; fused requests are exempt
+    uint64_t backfillPaths = 0;
 };
 
 struct RequestStatus {

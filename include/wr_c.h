@@ -98,6 +98,7 @@ typedef struct wr_c_server_config {
     uint32_t expressPathsInFlight;  /* express pool size (0 = ServerConfig's default) */
     uint32_t expressIterations;     /* express iterations per step (0 = ServerConfig's default) */
     uint32_t fusedMaxPaths;         /* ServerConfig::fusedMaxPaths (0 = bounce iterations only) */
+    uint32_t backfillPaths;         /* ServerConfig::backfillPaths (0 = no limit) */
 } wr_c_server_config;
 
 typedef struct wr_c_request_status {

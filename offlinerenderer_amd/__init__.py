@@ -79,7 +79,8 @@ class wrk_paths(C.Structure):
 class wr_c_server_config(C.Structure):
     _fields_ = [("maxPathsInFlight", C.c_uint32), ("maxChunkPaths", C.c_uint32), ("memoryBudgetBytes", C.c_uint64),
                 ("iterationCap", C.c_uint32), ("flags", C.c_uint32), ("expressMaxPaths", C.c_uint32),
-                ("expressPathsInFlight", C.c_uint32), ("expressIterations", C.c_uint32), ("fusedMaxPaths", C.c_uint32)]
+                ("expressPathsInFlight", C.c_uint32), ("expressIterations", C.c_uint32), ("fusedMaxPaths", C.c_uint32),
+                ("backfillPaths", C.c_uint32)]
 
 
 class wr_c_request_status(C.Structure):
@@ -509,7 +510,9 @@ class ServerConfig:
     chunk, memory budget for accumulation/image buffers (0 = unlimited), and
     the express lane for requests of at most expressMaxPaths paths (0 = off);
     requests of at most fusedMaxPaths paths render as one fused launch per
-    chunk in the lane they take (0 = off)."""
+    chunk in the lane they take (0 = off); with backfillPaths the requests
+    behind the oldest unfinished one share at most that many pool entries
+    while it has paths in flight (0 = no limit)."""
     maxPathsInFlight: int = 1 << 26
     maxChunkPaths: int = 1 << 24
     memoryBudgetBytes: int = 0
@@ -522,6 +525,7 @@ class ServerConfig:
     expressIterations: int = 32
     expressOverlap: bool = False
     fusedMaxPaths: int = 0
+    backfillPaths: int = 0
 
 
 class RenderServer:
@@ -535,7 +539,7 @@ class RenderServer:
                                 int(c.iterationCap), (1 if c.timeKernels else 0) | (2 if c.shortestFirst else 0)
                                 | (4 if c.roundRobin else 0) | (8 if c.expressOverlap else 0),
                                 int(c.expressMaxPaths), int(c.expressPathsInFlight),
-                                int(c.expressIterations), int(c.fusedMaxPaths))
+                                int(c.expressIterations), int(c.fusedMaxPaths), int(c.backfillPaths))
         self._h = lib().wr_c_server_create(C.byref(cc))
         if not self._h:
             raise RuntimeError(f"RenderServer: {_err()}")

@@ -381,14 +381,14 @@ void Engine::complete(uint32_t s) {
 // Admits the next chunk of r into a free pool range and slot and generates
 // its camera rays behind the queue entries already placed. Returns false
 // when no slot, no memory or no large enough range is free.
-bool Engine::admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase) {
+bool Engine::admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase, uint64_t maxPaths) {
     if (freeSlots_.empty() || capacity_ == 0) return false;
     if (!r.admitted && cfg_.memoryBudgetBytes && footprintUsed() != 0 &&
         footprintUsed() + r.footprint > cfg_.memoryBudgetBytes)
         return false;  // back-pressure: wait for memory to be released
     const uint64_t want = r.endPath - r.nextPath;
     const uint64_t chunkCap = cfg_.maxChunkPaths ? std::min<uint64_t>(cfg_.maxChunkPaths, capacity_) : capacity_;
-    const uint64_t full = std::min<uint64_t>(want, chunkCap);  // the chunk an empty pool would take
+    const uint64_t full = std::min<uint64_t>(std::min<uint64_t>(want, chunkCap), maxPaths);  // the chunk an empty pool would take
     // smaller chunks wait for more paths to end, unless nothing is in flight
     const uint64_t floor = cfg_.minChunkPaths ? cfg_.minChunkPaths
                                               : std::max<uint64_t>(capacity_ / 32, std::min<uint64_t>(65536, capacity_ / 8));
@@ -556,9 +556,33 @@ void Engine::launch() {
         });
     }
     if (!cfg_.roundRobin) {
+        // backfill budget: requests behind the oldest unfinished one (the
+        // head) share at most backfillPaths of the pool while it has paths
+        // in flight, counted from the live counts of the last step
+        uint64_t headId = 0, behindLive = 0;
+        if (cfg_.backfillPaths) {
+            for (auto& kv : requests_)
+                if (!kv.second->done) {
+                    headId = kv.first;
+                    break;
+                }
+            for (uint32_t sl = 0; sl < WRK_MAX_SLOTS; ++sl)
+                if (slots_[sl].active && !slots_[sl].finished && slots_[sl].request != headId)
+                    behindLive += hostLive_[sl];
+        }
         while (!pending_.empty()) {
             RequestState& r = *requests_.at(pending_.front());
-            if (!admitChunk(r, &added, total)) break;
+            uint64_t cap = ~0ull;
+            const bool fused = cfg_.fusedMaxPaths && r.endPath - r.firstPath <= cfg_.fusedMaxPaths;
+            if (cfg_.backfillPaths && r.id != headId && !fused) {
+                // no chunk smaller than a 64K-path share (or the rest of the request)
+                const uint64_t least = std::min<uint64_t>(65536, r.endPath - r.nextPath);
+                if (behindLive + least > cfg_.backfillPaths) break;
+                cap = cfg_.backfillPaths - behindLive;
+            }
+            const uint64_t before = r.nextPath;
+            if (!admitChunk(r, &added, total, cap)) break;
+            if (r.id != headId && !fused) behindLive += r.nextPath - before;
             if (r.nextPath == r.endPath) pending_.erase(pending_.begin());
         }
     }

@@ -64,6 +64,11 @@ struct EngineConfig {
     // completion in one fused launch (wrk_render_fused) instead of joining
     // the bounce iterations (0 = never)
     uint64_t fusedMaxPaths = 0;
+    // in-order admission: while the oldest unfinished request has paths in
+    // flight, the requests behind it take at most this many paths of the
+    // pool in all (they fill what its draining tail leaves idle without
+    // stretching its iterations); 0 = no limit. Fused requests are exempt.
+    uint64_t backfillPaths = 0;
 };
 
 class Engine {
@@ -166,7 +171,7 @@ private:
     uint32_t largestFree() const;
     void complete(uint32_t slot);
     void retire(uint32_t slot);
-    bool admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase);
+    bool admitChunk(RequestState& r, uint32_t* added, uint32_t queueBase, uint64_t maxPaths = ~0ull);
     template <typename F> void timed(int category, hipStream_t stream, F&& launch);
     uint64_t& footprintUsed() { return cfg_.sharedFootprint ? *cfg_.sharedFootprint : footprintUsed_; }
     DevBuf takeBuf(size_t bytes);

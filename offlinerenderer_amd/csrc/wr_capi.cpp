@@ -220,6 +220,7 @@ void* wr_c_server_create(const wr_c_server_config* cfg) {
         if (cfg->expressPathsInFlight) c.expressPathsInFlight = cfg->expressPathsInFlight;
         if (cfg->expressIterations) c.expressIterations = cfg->expressIterations;
         c.fusedMaxPaths = cfg->fusedMaxPaths;
+        c.backfillPaths = cfg->backfillPaths;
         out = new wr::RenderServer(c);
     });
     return out;

@@ -591,6 +591,7 @@ struct ServerImpl {
         e.roundRobin = c.roundRobin;
         e.sharedFootprint = &footprint;
         e.fusedMaxPaths = c.fusedMaxPaths;
+        e.backfillPaths = c.backfillPaths;
         eng = std::make_unique<detail::Engine>(e);
         eng->ensureCapacity(e.capacity);
         if (c.expressMaxPaths) {

@@ -115,6 +115,32 @@ def test_in_order_admission(world):
         assert _same(srv.fetch(rid), ref[i])
 
 
+def test_backfill_budget(world):
+    """With a backfill budget, the requests behind the oldest unfinished one
+    hold at most that many pool entries while it has paths in flight; every
+    image still equals its stand-alone render."""
+    reqs = _requests(world)
+    ref = _alone(reqs, cap=0)
+    budget = 2048
+    srv = wr.RenderServer(wr.ServerConfig(maxPathsInFlight=1 << 16, maxChunkPaths=8192, backfillPaths=budget))
+    ids = [srv.submit(s, c, p) for s, c, p in reqs]
+    out = {}
+    for _ in range(10000):
+        more = srv.step()
+        for rid in srv.takeCompleted():
+            out[rid] = srv.fetch(rid)
+        if not more:
+            break
+    assert sorted(out) == sorted(ids)
+    for i, rid in enumerate(ids):
+        assert _same(out[rid], ref[i]), f"request {i} differs with a backfill budget"
+    for stagger in (False, True):
+        got, _ = _serve(reqs, wr.ServerConfig(maxPathsInFlight=1 << 16, maxChunkPaths=8192, backfillPaths=budget,
+                                              fusedMaxPaths=4096), stagger=stagger)
+        for i, (a, b) in enumerate(zip(got, ref)):
+            assert _same(a, b), f"request {i} differs with a backfill budget and fused requests (stagger {stagger})"
+
+
 def test_reference_mode_cap_with_staggered_admission(world):
     reqs = _requests(world)[:3]
     ref = _alone(reqs, cap=8)
