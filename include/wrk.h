@@ -277,7 +277,11 @@ uint32_t wrk_partition_noalloc_max_items(void);
  * request (pixel = g % (width*height), sample = g / (width*height)),
  * stored at pool position poolBase + k, tagged with `slot`
  * (trace.cu:53-125). queue[k] = poolBase + k; if category is non-NULL,
- * category[k] = categoryValue. */
+ * category[k] = categoryValue. Writes org, dir, rad (zero), state (and the
+ * guides, zero, when allocated); thr and result are left as they are: a
+ * path whose state still holds isFirstHit has throughput 1 and pdf 1 in
+ * every kernel that reads it, and each path writes its result where it
+ * ends, unflushed ends (iteration cap, out-of-range material id) as zero. */
 int wrk_generate(const wrk_camera* cam, const wrk_render_params* params, const wrk_paths* paths, uint64_t first,
                  uint32_t n, uint32_t poolBase, uint32_t slot, uint32_t* queue, uint8_t* category,
                  uint32_t categoryValue, void* stream);

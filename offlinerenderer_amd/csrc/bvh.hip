@@ -701,20 +701,25 @@ __device__ __forceinline__ void resolve_shadow(const TraceResolve& R, uint32_t p
 // environment term (trace.cu:143-145) and the flush (trace.cu:146-148).
 __device__ __forceinline__ void resolve_closest(const TraceResolve& R, uint2* hits, uint32_t p, uint32_t qi,
                                                 uint32_t bestPrim, float tbest) {
+    // the path's result is written where it ends: a miss flushes it, and an
+    // out-of-range material id ends it unflushed (shade.cu:35-38)
+    f3 res = mk(0.0f, 0.0f, 0.0f);
+    uint32_t c = kCatDone;
     if (bestPrim != 0xFFFFFFFFu) {
         const uint32_t prim = bestPrim & kPrimMask;
         hits[p] = make_uint2(__float_as_uint(tbest), prim);
         const uint32_t tag = bestPrim >> 28;
-        R.cat[qi] = (uint8_t)(tag != 0u ? tag - 1u : shade_category(R.triMat[prim], R.numMaterials, R.materials));
+        c = tag != 0u ? tag - 1u : shade_category(R.triMat[prim], R.numMaterials, R.materials);
     } else {
-        const float4 T = R.thr[p], L = R.rad[p];
-        const uint32_t slot = R.st[p].z >> FLAG_SLOT_SHIFT;
+        const uint32_t z = R.st[p].z;
+        const float4 T = path_throughput(R.thr, p, z), L = R.rad[p];
+        const uint32_t slot = z >> FLAG_SLOT_SHIFT;
         const f3 Lr = mk(L.x + T.x * R.env[0], L.y + T.y * R.env[1], L.z + T.z * R.env[2]);
-        const f3 Lc = clamp_fireflies(Lr, R.prm[slot].fireflyClamp);
+        res = clamp_fireflies(Lr, R.prm[slot].fireflyClamp);
         R.rad[p] = f4(Lr, 0.0f);
-        R.result[p] = f4(Lc, 0.0f);
-        R.cat[qi] = (uint8_t)kCatDone;
     }
+    if (c == kCatDone) R.result[p] = f4(res, 0.0f);
+    R.cat[qi] = (uint8_t)c;
 }
 
 template <bool ANY, bool COUNT, int W, bool RESOLVE>

@@ -74,7 +74,12 @@ static SceneDev scene_of(const wrk_scene_view* v) {
 
 // ------------------------------------------------------------- generate
 // trace.cu:53-125: camera ray of global path g = (pixel, sample) into pool
-// entry i, path state reset.
+// entry i, path state reset. The throughput (1, 1, 1, prevPdf 1) and the
+// result are not stored: a path whose flags still hold FLAG_FIRST_HIT has
+// that throughput by definition (its first shade or miss takes it from the
+// flag), and every path writes its result where it ends, an unflushed end
+// (iteration cap, out-of-range material id) included. Radiance is stored as
+// zero because shaders that never add to it leave it in the pool.
 __device__ __forceinline__ void generate_path(const wrk_camera& cam, const wrk_render_params& prm, const Pool& P,
                                               uint64_t g, uint32_t i, uint32_t slot) {
     const uint64_t npix = (uint64_t)prm.width * prm.height;
@@ -112,10 +117,8 @@ __device__ __forceinline__ void generate_path(const wrk_camera& cam, const wrk_r
     }
     P.org[i] = f4(o, 0.0001f);
     P.dir[i] = f4(d, 1e20f);
-    P.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);  // prevPdf = 1
     P.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.st[i] = make_uint4(seed, 0u, FLAG_FIRST_HIT | FLAG_PREV_DELTA | (slot << FLAG_SLOT_SHIFT), pixel);
-    P.result[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (P.guides) {
         P.guides[2 * (size_t)i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         P.guides[2 * (size_t)i + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -166,8 +169,9 @@ __global__ __launch_bounds__(256) void k_classify(SceneDev S, const wrk_render_p
     } else if (i < nT + nTS) {
         uint2 h = P.hit[p];
         if (h.y == 0xFFFFFFFFu) {
-            float4 T = P.thr[p], L = P.rad[p];
-            const uint32_t slot = P.st[p].z >> FLAG_SLOT_SHIFT;
+            const uint32_t z = P.st[p].z;
+            float4 T = path_throughput(P.thr, p, z), L = P.rad[p];
+            const uint32_t slot = z >> FLAG_SLOT_SHIFT;
             f3 Lr = mk(L.x + T.x * S.env[0], L.y + T.y * S.env[1], L.z + T.z * S.env[2]);
             f3 Lc = clamp_fireflies(Lr, prmTable[slot].fireflyClamp);
             P.rad[p] = f4(Lr, 0.0f);
@@ -175,6 +179,7 @@ __global__ __launch_bounds__(256) void k_classify(SceneDev S, const wrk_render_p
             c = CAT_DONE;
         } else {
             c = shade_category(S.triMat[h.y], S.numMaterials, S.materials);
+            if (c == CAT_DONE) P.result[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // ends unflushed
         }
     } else if (P.st[p].z & FLAG_NEE_DONE) {
         c = WRK_MAT_LAMBERTIAN;
@@ -519,10 +524,13 @@ __device__ __forceinline__ uint32_t shade_path(const SceneDev& S, const wrk_rend
     // light contribution goes through the shadow ray), so their L is read
     // only for a flush and is not written back
     const bool lazyL = seg == WRK_MAT_LAMBERTIAN || seg == WRK_MAT_GLASS || seg == WRK_MAT_GGX_TRANSMISSION;
+    // a path at its first hit has throughput 1 and radiance 0 (the camera
+    // pass stores the radiance as zero and no throughput)
+    const bool fresh = r.firstHit;
     float4 o4 = P.org[p];
     float4 d4 = P.dir[p];
-    float4 T4 = P.thr[p];
-    const float4 L4 = lazyL ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : P.rad[p];
+    float4 T4 = path_throughput(P.thr, p, st.z);
+    const float4 L4 = (lazyL || fresh) ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : P.rad[p];
     r.o = xyz(o4); r.tMin = o4.w; r.d = xyz(d4); r.tMax = d4.w;
     r.T = xyz(T4); r.prevPdf = T4.w; r.L = xyz(L4);
     r.flushed = false; r.done = false; r.toShadow = false;
@@ -564,7 +572,7 @@ __device__ __forceinline__ uint32_t shade_path(const SceneDev& S, const wrk_rend
         else shade_glass(S, prm, r, eh, mat);
     }
 
-    if (lazyL && r.flushed) r.flushL = clamp_fireflies(xyz(P.rad[p]), prm.fireflyClamp);
+    if (lazyL && r.flushed) r.flushL = clamp_fireflies(fresh ? mk(0.0f, 0.0f, 0.0f) : xyz(P.rad[p]), prm.fireflyClamp);
     uint32_t next;
     if (r.done && r.toShadow) {
         next = NEXT_SHADOW;  // flushed once the shadow result is in
@@ -573,6 +581,7 @@ __device__ __forceinline__ uint32_t shade_path(const SceneDev& S, const wrk_rend
         if (r.flushed) P.result[p] = f4(r.flushL, 0.0f);
     } else if (defer && iter >= prm.iterationCap) {
         next = NEXT_DONE;  // renderer.cpp:213 cap: the path is dropped unflushed
+        P.result[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     } else if (r.toShadow) {
         next = defer ? NEXT_SHADOW : NEXT_TRACE_SHADOW;
     } else {
@@ -668,6 +677,7 @@ __global__ __launch_bounds__(256) void k_paths_fused(wrk_camera cam, SceneDev S,
     const wrk_render_params prm = prmTable[slot];
     generate_path(cam, prm, P, first + k, i, slot);
     uint32_t next = NEXT_TRACE;
+    bool fresh = true;  // not shaded yet: throughput 1, radiance 0, none stored
     for (;;) {
         uint32_t cat = CAT_DONE;
         if (next == NEXT_TRACE_SHADOW || next == NEXT_SHADOW) {
@@ -701,15 +711,20 @@ __global__ __launch_bounds__(256) void k_paths_fused(wrk_camera cam, SceneDev S,
                 cat = tag != 0u ? tag - 1u : shade_category(S.triMat[prim], S.numMaterials, S.materials);
             } else {
                 // trace.cu:143-148: environment term, then the flush
-                const float4 T = P.thr[i], L = P.rad[i];
+                const float4 T = fresh ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : P.thr[i];
+                const float4 L = P.rad[i];
                 const f3 Lr = mk(L.x + T.x * S.env[0], L.y + T.y * S.env[1], L.z + T.z * S.env[2]);
                 P.rad[i] = f4(Lr, 0.0f);
                 P.result[i] = f4(clamp_fireflies(Lr, prm.fireflyClamp), 0.0f);
                 break;
             }
         }
-        if (cat == CAT_DONE) break;  // out-of-range material id: ends unflushed (shade.cu:35-38)
+        if (cat == CAT_DONE) {  // out-of-range material id: ends unflushed (shade.cu:35-38)
+            P.result[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            break;
+        }
         next = shade_path(S, prmTable, P, i, (int)cat);
+        fresh = false;
         if (next == NEXT_DONE) break;
     }
 }

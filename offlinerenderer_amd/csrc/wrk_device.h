@@ -345,4 +345,11 @@ constexpr uint32_t FLAG_ITER_MASK = 31u << FLAG_ITER_SHIFT;
 constexpr uint32_t FLAG_SLOT_SHIFT = 24;
 constexpr uint32_t kMaxSlots = 256;
 
+// Throughput (xyz) and previous-bounce pdf (w) of pool entry p with flags z.
+// A path not yet shaded (FLAG_FIRST_HIT) has throughput 1 and pdf 1, which
+// the camera-ray pass leaves unstored; its lanes issue no load.
+__device__ __forceinline__ float4 path_throughput(const float4* thr, uint32_t p, uint32_t z) {
+    return (z & FLAG_FIRST_HIT) ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : thr[p];
+}
+
 }  // namespace wrk

@@ -1,6 +1,8 @@
 """GPU parity of whole paths: per-path radiance of the wavefront renderer
 against the oracle's restatement of libWR's loop at the same PCG seeds.
 Tolerance: 1e-4 relative (FP32); bit-exact agreement is reported."""
+import math
+
 import numpy as np
 import pytest
 
@@ -84,6 +86,59 @@ def test_guides_match_oracle(gpu, oracle, which):
     assert np.array_equal(alb.reshape(-1, 3).view(np.uint32), oa.view(np.uint32))
     assert np.array_equal(nrm.reshape(-1, 3).view(np.uint32), on.view(np.uint32))
     assert np.count_nonzero(alb) > alb.size // 2
+
+
+def _cornell_with_unflushed_ends(width, height, spp):
+    """The Cornell box with its short box on material id 99 (out of range:
+    such paths end without a flush, shade.cu:35-38) and a nonzero
+    environment, seen past the open front by a wider field of view, so
+    camera rays that miss add it at depth 0."""
+    d = scenes.cornell(width, height, spp)
+    d.camera = dict(d.camera, fovY=math.radians(80.0))
+    ops, boxes = [], 0
+    for op in d.ops:
+        if op[0] == "mesh" and len(op[2]) == 36:  # a box: 12 triangles
+            boxes += 1
+            if boxes == 1:
+                op = op[:5] + (99,)
+        elif op[0] == "env":
+            op = ("env", (0.3, 0.2, 0.1))
+        ops.append(op)
+    assert boxes == 2
+    d.ops = ops
+    return d
+
+
+@pytest.mark.parametrize("cap,fused", [(0, 0), (8, 0), (0, 1 << 20), (8, 1 << 20)])
+def test_unflushed_ends_add_zero_with_reused_pool_entries(gpu, oracle, cap, fused):
+    """Paths that end unflushed (the out-of-range material id; in reference
+    mode also the iteration cap, renderer.cpp:213) add zero, and camera rays
+    that miss add the environment at throughput 1. The pool holds fewer paths
+    than the image, so every pool entry is reused across chunks: a path that
+    ended without writing its result would add a previous chunk's value."""
+    W, H, spp = 48, 40, 3
+    d = _cornell_with_unflushed_ends(W, H, spp)
+    ps = G.product_scene(d)
+    os_ = oracle.build_oracle_scene(d, bvh=True)
+    npix, n = W * H, W * H * spp
+    r = wr.Renderer()
+    r.setWavefrontConfig(wr.WavefrontConfig(iterationCap=cap, maxPathsInFlight=npix // 2 + 64, fusedMaxPaths=fused))
+    oprm = oracle.params_in(W, H, d.maxBounces, d.rrDepth, iterationCap=cap)
+    L_g = r.renderPaths(ps, G.camera(d), G.params(d), 0, n)
+    idx = np.arange(n)
+    L_o, _ = oracle.render_paths(os_, d.camera, oprm, idx % npix, idx // npix, use_bvh=True)
+    rel = np.abs(L_g - L_o) / np.maximum(np.abs(L_o), 1e-6)
+    assert rel.max() <= REL_TOL, rel.max()
+    zero = ~np.any(L_o, axis=1)
+    env = np.all(L_o == np.float32([0.3, 0.2, 0.1]), axis=1)
+    assert zero.sum() > n // 20 and env.sum() > n // 20, (zero.sum(), env.sum())
+    img = r.render(ps, G.camera(d), G.params(d))
+    ref = oracle.render_image(os_, d.camera, oprm, spp, use_bvh=True) * np.float32(1.0 / spp)
+    rel = np.abs(img - ref) / np.maximum(np.abs(ref), 1e-6)
+    assert rel.max() <= REL_TOL, rel.max()
+    st = r.getStats()
+    assert st["batches"] >= 3
+    assert (st["fusedPaths"] > 0) == (fused > 0)
 
 
 def test_first_sample_offset_matches_oracle(gpu, oracle):
