@@ -24,6 +24,38 @@ extern "C" {
 int wrk_conv3x3_fwd(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* w, const float* bias,
                     uint16_t* y, uint32_t yPitch, uint32_t cout, uint32_t relu, uint32_t n, uint32_t h, uint32_t wd,
                     void* stream);
+/* The same convolution, computing only output channels [0, c) with c =
+ * coutUsed rounded up to 16 (at most cout); channels c.. of y are not
+ * written. w has cout output channels. For an output layer whose consumer
+ * reads its first few channels. */
+int wrk_conv3x3_fwd_first(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* w, const float* bias,
+                          uint16_t* y, uint32_t yPitch, uint32_t cout, uint32_t coutUsed, uint32_t relu, uint32_t n,
+                          uint32_t h, uint32_t wd, void* stream);
+/* The same convolution over a decoder input [2x upsampled xUp | skip]:
+ * input channels [0, cUp) are the nearest 2x upsampling of xUp ((n, h/2,
+ * wd/2), channel pitch upPitch), read from xUp while the kernel stages its
+ * input, and channels [cUp, cin) are read from x at those channel positions
+ * (x's first cUp channels are not read). Equal to wrk_up2_fwd of xUp into
+ * x's first cUp channels followed by wrk_conv3x3_fwd. h and wd even, cUp a
+ * multiple of 32 in 32..cin. */
+int wrk_conv3x3_fwd_up(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* xUp, uint32_t upPitch,
+                       uint32_t cUp, const uint16_t* w, const float* bias, uint16_t* y, uint32_t yPitch, uint32_t cout,
+                       uint32_t relu, uint32_t n, uint32_t h, uint32_t wd, void* stream);
+/* wrk_conv3x3_fwd, and the 2x2 average pool of its bf16 output into yPool
+ * ((n, h/2, wd/2), channel pitch poolPitch) from the same epilogue: equal
+ * to wrk_conv3x3_fwd followed by wrk_pool2_fwd of y. h and wd even. */
+int wrk_conv3x3_fwd_pool(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* w, const float* bias,
+                         uint16_t* y, uint32_t yPitch, uint32_t cout, uint32_t relu, uint16_t* yPool,
+                         uint32_t poolPitch, uint32_t n, uint32_t h, uint32_t wd, void* stream);
+/* The residual head's output convolution and wrk_unet_unpack in one pass:
+ * out (n*h*wd, 3) FP32 = max(expm1(y_c + base_c), 0) for c < 3, y the bf16
+ * convolution output (no activation) over output channels 0..15 of w
+ * (cout output channels), base the packed input (channel pitch basePitch).
+ * Equal to wrk_conv3x3_fwd_first (coutUsed 3, relu 0) followed by
+ * wrk_unet_unpack; no tensor y is written. cin = 32. */
+int wrk_conv3x3_fwd_unpack(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* w, const float* bias,
+                           uint32_t cout, const uint16_t* base, uint32_t basePitch, float* out, uint32_t n,
+                           uint32_t h, uint32_t wd, void* stream);
 /* dw[tap][co][ci] = sum_p dz[p][co] x[p + off(tap)][ci] (FP32, overwritten);
  * db[co] = sum_p dz[p][co] when db != NULL. */
 uint64_t wrk_conv3x3_wgrad_scratch_bytes(uint32_t cin, uint32_t cout, uint32_t n, uint32_t h, uint32_t w);

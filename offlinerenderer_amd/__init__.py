@@ -137,6 +137,14 @@ _SIGS = {
     # wrk_unet_* / denoiser kernels (include/wrk_unet.h)
     "wrk_conv3x3_fwd": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32,
                                   C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "wrk_conv3x3_fwd_first": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "wrk_conv3x3_fwd_up": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint32,
+                                     C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "wrk_conv3x3_fwd_pool": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P,
+                                       C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "wrk_conv3x3_fwd_unpack": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32,
+                                         C.c_uint32, C.c_uint32, _P]),
     "wrk_conv3x3_wgrad_scratch_bytes": (C.c_uint64, [C.c_uint32] * 5),
     "wrk_conv3x3_wgrad": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                     C.c_uint32, _P, _P, _P, C.c_uint64, _P]),

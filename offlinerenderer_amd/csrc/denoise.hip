@@ -18,6 +18,8 @@
 // 4*(l >> 4) + i, column l & 15.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "wrk_common.h"
 
 namespace wrk {
@@ -48,11 +50,129 @@ __device__ __forceinline__ bf16 f2bf(float f) {
 constexpr int kTH = 8, kTW = 16;            // output pixels per block (4 waves x 2 rows x 16)
 constexpr int kIH = kTH + 2, kIW = kTW + 2;  // input tile with the 1-pixel halo
 
+// The MFMAs of one 32-channel input chunk over a block's 8x16 output tile:
+// wave w takes output rows 2w, 2w + 1; sIn holds the input tile with its
+// halo ([kIH * kIW][32]), sW the chunk's weights ([9][CT * 16][32]).
+template <int CT>
+__device__ __forceinline__ void conv_tile_mma(const bf16* sIn, const bf16* sW, int lane, int wave,
+                                              f32x4 (&acc)[2][CT]) {
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+        const int dy = tap / 3, dx = tap % 3;
+        bf16x8 a[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c)
+            a[c] = *(const bf16x8*)(sW + ((tap * CT + c) * 16 + (lane & 15)) * 32 + (lane >> 4) * 8);
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int py = wave * 2 + r + dy, px = (lane & 15) + dx;
+            const bf16x8 b = *(const bf16x8*)(sIn + (py * kIW + px) * 32 + (lane >> 4) * 8);
+#pragma unroll
+            for (int c = 0; c < CT; ++c) acc[r][c] = mfma16(a[c], b, acc[r][c]);
+        }
+    }
+}
+
+// Bias, ReLU and the bf16 store of a block's 8x16 output tile at (x0, y0)
+// of image base `img`, output channels co0 .. co0 + 16 * CT - 1 (those
+// below cout); bv holds each lane's biases (conv_tile_bias).
+template <int CT>
+__device__ __forceinline__ void conv_tile_bias(const float* __restrict__ bias, uint32_t cout, int co0, int lane,
+                                               f32x4 (&bv)[CT]) {
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+        const int co = co0 + c * 16 + (lane >> 4) * 4;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) bv[c][i] = (bias && co < (int)cout) ? bias[co + i] : 0.0f;
+    }
+}
+
+// With rgb given, the tile's channels 0..2 are unpacked instead of stored:
+// rgb[p][c] = max(expm1(y + base[p][c]), 0), as wrk_unet_unpack computes it.
+template <int CT>
+__device__ __forceinline__ void conv_tile_store(const f32x4 (&acc)[2][CT], const f32x4 (&bv)[CT],
+                                                bf16* __restrict__ y, uint32_t yPitch, uint32_t cout, int co0,
+                                                uint32_t relu, size_t img, int x0, int y0, uint32_t H, uint32_t W,
+                                                int lane, int wave, bf16* __restrict__ yPool, uint32_t poolPitch,
+                                                const bf16* __restrict__ base = nullptr, uint32_t basePitch = 0,
+                                                float* __restrict__ rgb = nullptr) {
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+        const int co = co0 + c * 16 + (lane >> 4) * 4;
+        union { bf16 h[4]; uint2 u; } pk[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float v = acc[r][c][i] + bv[c][i];
+                if (relu) v = fmaxf(v, 0.0f);
+                pk[r].h[i] = f2bf(v);
+            }
+        const int ox = x0 + (lane & 15);
+        if (rgb) {
+            if (co == 0 && ox < (int)W) {
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const int oy = y0 + wave * 2 + r;
+                    if (oy >= (int)H) continue;
+                    const size_t px = img + (size_t)oy * W + ox;
+#pragma unroll
+                    for (int c3 = 0; c3 < 3; ++c3) {
+                        const float q = (float)pk[r].h[c3] + (float)base[px * basePitch + c3];
+                        rgb[px * 3 + c3] = fmaxf(expm1f(q), 0.0f);
+                    }
+                }
+            }
+        } else if (co < (int)cout && ox < (int)W) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int oy = y0 + wave * 2 + r;
+                if (oy < (int)H) *(uint2*)(y + (img + (size_t)oy * W + ox) * yPitch + co) = pk[r].u;
+            }
+        }
+        if (yPool) {
+            // 2x2 average of the stored bf16 values in wrk_pool2_fwd's order
+            // (row-major over the window): rows 2w, 2w + 1 are this wave's,
+            // columns x, x + 1 are lanes l, l ^ 1
+            uint2 nb[2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                nb[r].x = (uint32_t)__shfl_xor((int)pk[r].u.x, 1);
+                nb[r].y = (uint32_t)__shfl_xor((int)pk[r].u.y, 1);
+            }
+            const int oy = y0 + wave * 2;
+            if ((lane & 1) == 0 && co < (int)cout && ox < (int)W && oy < (int)H) {
+                union { bf16 h[4]; uint2 u; } q[2], o;
+                q[0].u = nb[0];
+                q[1].u = nb[1];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    float s = 0.0f;
+                    s += (float)pk[0].h[i];
+                    s += (float)q[0].h[i];
+                    s += (float)pk[1].h[i];
+                    s += (float)q[1].h[i];
+                    o.h[i] = f2bf(s * 0.25f);
+                }
+                *(uint2*)(yPool + (img / 4 + (size_t)(oy >> 1) * (W >> 1) + (ox >> 1)) * poolPitch + co) = o.u;
+            }
+        }
+    }
+}
+
+// wCout is the weights' output-channel count (their tap stride); cout the
+// channels computed (co0 + 16 * CT <= cout rounded up to 16). Input
+// channels below cUp are the 2x nearest upsampling of xUp ((H/2, W/2),
+// pitch upPitch), read at (y/2, x/2) while staging, so a decoder's
+// [upsampled | skip] input needs no upsampled copy; the others come from x.
 template <int CT>  // output-channel tiles of 16 per block
-__global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, uint32_t xPitch, uint32_t cin,
-                                                  const bf16* __restrict__ w, const float* __restrict__ bias,
-                                                  bf16* __restrict__ y, uint32_t yPitch, uint32_t cout,
-                                                  uint32_t relu, uint32_t H, uint32_t W, uint32_t tilesX) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_conv_fwd(const bf16* __restrict__ x, uint32_t xPitch, uint32_t cin,
+                                                  const bf16* __restrict__ xUp, uint32_t upPitch, uint32_t cUp,
+                                                  const bf16* __restrict__ w, uint32_t wCout,
+                                                  const float* __restrict__ bias, bf16* __restrict__ y,
+                                                  uint32_t yPitch, uint32_t cout, uint32_t relu, uint32_t H,
+                                                  uint32_t W, uint32_t tilesX, bf16* __restrict__ yPool,
+                                                  uint32_t poolPitch) {
     __shared__ __attribute__((aligned(16))) bf16 sIn[kIH * kIW * 32];
     __shared__ __attribute__((aligned(16))) bf16 sW[9 * CT * 16 * 32];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -72,6 +192,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, ui
     constexpr int kInPer = (kInParts + 255) / 256, kWPer = (kWParts + 255) / 256;
     uint4 rIn[kInPer], rW[kWPer];
     auto fetch = [&](uint32_t ci0) {
+        const bool up = ci0 < cUp;  // the same for the whole block
 #pragma unroll
         for (int k = 0; k < kInPer; ++k) {
             const int e = t + 256 * k;
@@ -79,8 +200,13 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, ui
             if (e < kInParts) {
                 const int pix = e >> 2, part = e & 3;
                 const int gy = y0 + pix / kIW - 1, gx = x0 + pix % kIW - 1;
-                if (gy >= 0 && gy < (int)H && gx >= 0 && gx < (int)W)
-                    v = *(const uint4*)(x + (img + (size_t)gy * W + gx) * xPitch + ci0 + part * 8);
+                if (gy >= 0 && gy < (int)H && gx >= 0 && gx < (int)W) {
+                    if (up)
+                        v = *(const uint4*)(xUp + (img / 4 + (size_t)(gy >> 1) * (W >> 1) + (gx >> 1)) * upPitch +
+                                            ci0 + part * 8);
+                    else
+                        v = *(const uint4*)(x + (img + (size_t)gy * W + gx) * xPitch + ci0 + part * 8);
+                }
             }
             rIn[k] = v;
         }
@@ -92,7 +218,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, ui
                 const int row = e >> 2, part = e & 3;
                 const int tap = row / (CT * 16), col = row % (CT * 16);
                 if (co0 + col < (int)cout)
-                    v = *(const uint4*)(w + ((size_t)tap * cout + co0 + col) * cin + ci0 + part * 8);
+                    v = *(const uint4*)(w + ((size_t)tap * wCout + co0 + col) * cin + ci0 + part * 8);
             }
             rW[k] = v;
         }
@@ -128,24 +254,82 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ x, ui
         }
         __syncthreads();
     }
+    f32x4 bv[CT];
+    conv_tile_bias<CT>(bias, cout, co0, lane, bv);
+    conv_tile_store<CT>(acc, bv, y, yPitch, cout, co0, relu, img, x0, y0, H, W, lane, wave, yPool, poolPitch);
+}
+
+// The same convolution for 32 input channels (one chunk), over many tiles
+// per block: the block stages its weight slice once, then walks the pixel
+// tiles tile = blockIdx.x, blockIdx.x + gridDim.x, ... of all images, and
+// loads the next tile's input into registers while the MFMAs and the store
+// of the current one run. With one chunk, the kernel above has nothing to
+// overlap its loads with and fetches the weights again for every tile.
+template <int CT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_conv_fwd_c32(const bf16* __restrict__ x, uint32_t xPitch,
+                                                      const bf16* __restrict__ w, uint32_t wCout,
+                                                      const float* __restrict__ bias, bf16* __restrict__ y,
+                                                      uint32_t yPitch, uint32_t cout, uint32_t relu, uint32_t H,
+                                                      uint32_t W, uint32_t tilesX, uint32_t tilesPerImage,
+                                                      uint32_t nTiles, bf16* __restrict__ yPool,
+                                                      uint32_t poolPitch, const bf16* __restrict__ base,
+                                                      uint32_t basePitch, float* __restrict__ rgb) {
+    __shared__ __attribute__((aligned(16))) bf16 sIn[kIH * kIW * 32];
+    __shared__ __attribute__((aligned(16))) bf16 sW[9 * CT * 16 * 32];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int co0 = (int)blockIdx.y * CT * 16;
+    constexpr int kWParts = 9 * CT * 16 * 4;
+    for (int e = t; e < kWParts; e += 256) {
+        const int row = e >> 2, part = e & 3;
+        const int tap = row / (CT * 16), col = row % (CT * 16);
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (co0 + col < (int)cout) v = *(const uint4*)(w + ((size_t)tap * wCout + co0 + col) * 32 + part * 8);
+        *(uint4*)(sW + row * 32 + part * 8) = v;
+    }
+    // the biases are loaded once: a load in the tile loop would also wait
+    // for the next tile's prefetch (loads complete in order)
+    f32x4 bv[CT];
+    conv_tile_bias<CT>(bias, cout, co0, lane, bv);
+    constexpr int kInParts = kIH * kIW * 4, kInPer = (kInParts + 255) / 256;
+    uint4 rIn[kInPer];
+    auto fetch = [&](uint32_t tile) {
+        const uint32_t tin = tile % tilesPerImage;
+        const size_t img = (size_t)(tile / tilesPerImage) * H * W;
+        const int x0 = (int)(tin % tilesX) * kTW, y0 = (int)(tin / tilesX) * kTH;
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-        const int oy = y0 + wave * 2 + r, ox = x0 + (lane & 15);
-        if (oy >= (int)H || ox >= (int)W) continue;
-        bf16* out = y + (img + (size_t)oy * W + ox) * yPitch;
-#pragma unroll
-        for (int c = 0; c < CT; ++c) {
-            const int co = co0 + c * 16 + (lane >> 4) * 4;
-            if (co >= (int)cout) continue;
-            union { bf16 h[4]; uint2 u; } pk;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                float v = acc[r][c][i] + (bias ? bias[co + i] : 0.0f);
-                if (relu) v = fmaxf(v, 0.0f);
-                pk.h[i] = f2bf(v);
+        for (int k = 0; k < kInPer; ++k) {
+            const int e = t + 256 * k;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (e < kInParts) {
+                const int pix = e >> 2, part = e & 3;
+                const int gy = y0 + pix / kIW - 1, gx = x0 + pix % kIW - 1;
+                if (gy >= 0 && gy < (int)H && gx >= 0 && gx < (int)W)
+                    v = *(const uint4*)(x + (img + (size_t)gy * W + gx) * xPitch + part * 8);
             }
-            *(uint2*)(out + co) = pk.u;
+            rIn[k] = v;
         }
+    };
+    uint32_t tile = blockIdx.x;
+    if (tile < nTiles) fetch(tile);
+    for (; tile < nTiles; tile += gridDim.x) {
+#pragma unroll
+        for (int k = 0; k < kInPer; ++k) {
+            const int e = t + 256 * k;
+            if (e < kInParts) *(uint4*)(sIn + (e >> 2) * 32 + (e & 3) * 8) = rIn[k];
+        }
+        __syncthreads();  // the tile (and, the first time, the weights) are in LDS
+        if (tile + gridDim.x < nTiles) fetch(tile + gridDim.x);
+        f32x4 acc[2][CT];
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < CT; ++c) acc[r][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        conv_tile_mma<CT>(sIn, sW, lane, wave, acc);
+        const uint32_t tin = tile % tilesPerImage;
+        conv_tile_store<CT>(acc, bv, y, yPitch, cout, co0, relu, (size_t)(tile / tilesPerImage) * H * W,
+                            (int)(tin % tilesX) * kTW, (int)(tin / tilesX) * kTH, H, W, lane, wave, yPool,
+                            poolPitch, base, basePitch, rgb);
+        __syncthreads();  // every wave is done with sIn before the next tile overwrites it
     }
 }
 
@@ -669,25 +853,125 @@ static int conv_check(uint32_t cin, uint32_t cout, uint32_t xPitch, uint32_t yPi
     return WRK_OK;
 }
 
+// Resident blocks of a persistent launch: blocks per CU the kernel's
+// registers and LDS allow, times the CUs, shared over gridY channel tiles.
+static uint32_t resident_blocks(const void* kernel, uint32_t gridY) {
+    int dev = 0, cus = 0, perCu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, 256, 0) != hipSuccess || cus <= 0 || perCu <= 0)
+        return 1024u / gridY + 1u;
+    const uint32_t total = (uint32_t)cus * (uint32_t)perCu;
+    return total > gridY ? total / gridY : 1u;
+}
+
+template <int CT>
+static void conv_fwd_launch(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* xUp, uint32_t upPitch,
+                            uint32_t cUp, const uint16_t* w, uint32_t wCout, const float* bias, uint16_t* y,
+                            uint32_t yPitch, uint32_t cout, uint32_t relu, uint16_t* yPool, uint32_t poolPitch,
+                            uint32_t n, uint32_t h, uint32_t wd, hipStream_t stream, const uint16_t* base = nullptr,
+                            uint32_t basePitch = 0, float* rgb = nullptr) {
+    const uint32_t tilesX = ceil_div(wd, kTW), tilesY = ceil_div(h, kTH);
+    const uint32_t coTiles = ceil_div(cout, 16 * CT);
+    if (cin == 32 && cUp == 0) {
+        const uint32_t tilesPerImage = tilesX * tilesY, nTiles = tilesPerImage * n;
+        const uint32_t blocks = std::min(nTiles, resident_blocks((const void*)&k_conv_fwd_c32<CT>, coTiles));
+        hipLaunchKernelGGL(k_conv_fwd_c32<CT>, dim3(blocks, coTiles), dim3(256), 0, stream, (const bf16*)x, xPitch,
+                           (const bf16*)w, wCout, bias, (bf16*)y, yPitch, cout, relu, h, wd, tilesX, tilesPerImage,
+                           nTiles, (bf16*)yPool, poolPitch, (const bf16*)base, basePitch, rgb);
+    } else {
+        hipLaunchKernelGGL(k_conv_fwd<CT>, dim3(tilesX * tilesY, coTiles, n), dim3(256), 0, stream, (const bf16*)x,
+                           xPitch, cin, (const bf16*)xUp, upPitch, cUp, (const bf16*)w, wCout, bias, (bf16*)y, yPitch,
+                           cout, relu, h, wd, tilesX, (bf16*)yPool, poolPitch);
+    }
+}
+
+// Output channels [0, cout) of a convolution whose weights have wCout;
+// input channels below cUp upsampled from xUp; the 2x2 average pool of
+// the output into yPool when it is given.
+static int conv_fwd(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* xUp, uint32_t upPitch,
+                    uint32_t cUp, const uint16_t* w, uint32_t wCout, const float* bias, uint16_t* y, uint32_t yPitch,
+                    uint32_t cout, uint32_t relu, uint16_t* yPool, uint32_t poolPitch, uint32_t n, uint32_t h,
+                    uint32_t wd, void* stream) {
+    if (n == 0 || h == 0 || wd == 0) return WRK_OK;
+    WRK_REQUIRE(n <= 65535, "wrk_conv3x3_fwd: batch too large");
+    WRK_REQUIRE((uint64_t)n * ceil_div(h, kTH) * ceil_div(wd, kTW) <= 0xFFFFFFFFull,
+                "wrk_conv3x3_fwd: too many pixel tiles");
+    const hipStream_t s = as_stream(stream);
+    if (cout % 64 == 0)
+        conv_fwd_launch<4>(x, xPitch, cin, xUp, upPitch, cUp, w, wCout, bias, y, yPitch, cout, relu, yPool, poolPitch, n,
+                           h, wd, s);
+    else if (cout % 32 == 0)
+        conv_fwd_launch<2>(x, xPitch, cin, xUp, upPitch, cUp, w, wCout, bias, y, yPitch, cout, relu, yPool, poolPitch, n,
+                           h, wd, s);
+    else
+        conv_fwd_launch<1>(x, xPitch, cin, xUp, upPitch, cUp, w, wCout, bias, y, yPitch, cout, relu, yPool, poolPitch, n,
+                           h, wd, s);
+    WRK_CHECK_LAUNCH();
+    return WRK_OK;
+}
+
 extern "C" int wrk_conv3x3_fwd(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* w,
                                const float* bias, uint16_t* y, uint32_t yPitch, uint32_t cout, uint32_t relu,
                                uint32_t n, uint32_t h, uint32_t wd, void* stream) {
     WRK_REQUIRE(x && w && y, "wrk_conv3x3_fwd: null argument");
     if (int rc = conv_check(cin, cout, xPitch, yPitch)) return rc;
+    return conv_fwd(x, xPitch, cin, nullptr, 0, 0, w, cout, bias, y, yPitch, cout, relu, nullptr, 0, n, h, wd,
+                    stream);
+}
+
+extern "C" int wrk_conv3x3_fwd_first(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* w,
+                                     const float* bias, uint16_t* y, uint32_t yPitch, uint32_t cout,
+                                     uint32_t coutUsed, uint32_t relu, uint32_t n, uint32_t h, uint32_t wd,
+                                     void* stream) {
+    WRK_REQUIRE(x && w && y, "wrk_conv3x3_fwd_first: null argument");
+    if (int rc = conv_check(cin, cout, xPitch, yPitch)) return rc;
+    WRK_REQUIRE(coutUsed > 0 && coutUsed <= cout, "wrk_conv3x3_fwd_first: coutUsed must be in 1..cout");
+    const uint32_t computed = std::min(cout, (coutUsed + 15u) / 16u * 16u);
+    return conv_fwd(x, xPitch, cin, nullptr, 0, 0, w, cout, bias, y, yPitch, computed, relu, nullptr, 0, n, h, wd,
+                    stream);
+}
+
+extern "C" int wrk_conv3x3_fwd_up(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* xUp,
+                                  uint32_t upPitch, uint32_t cUp, const uint16_t* w, const float* bias, uint16_t* y,
+                                  uint32_t yPitch, uint32_t cout, uint32_t relu, uint32_t n, uint32_t h, uint32_t wd,
+                                  void* stream) {
+    WRK_REQUIRE(x && xUp && w && y, "wrk_conv3x3_fwd_up: null argument");
+    if (int rc = conv_check(cin, cout, xPitch, yPitch)) return rc;
+    WRK_REQUIRE(cUp > 0 && cUp <= cin && cUp % 32 == 0, "wrk_conv3x3_fwd_up: cUp must be a multiple of 32 in 32..cin");
+    WRK_REQUIRE(upPitch % 8 == 0 && upPitch >= cUp, "wrk_conv3x3_fwd_up: bad upsampled-input pitch");
+    WRK_REQUIRE(h % 2 == 0 && wd % 2 == 0, "wrk_conv3x3_fwd_up: height and width must be even");
+    return conv_fwd(x, xPitch, cin, xUp, upPitch, cUp, w, cout, bias, y, yPitch, cout, relu, nullptr, 0, n, h, wd,
+                    stream);
+}
+
+extern "C" int wrk_conv3x3_fwd_unpack(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* w,
+                                      const float* bias, uint32_t cout, const uint16_t* base, uint32_t basePitch,
+                                      float* out, uint32_t n, uint32_t h, uint32_t wd, void* stream) {
+    WRK_REQUIRE(x && w && base && out, "wrk_conv3x3_fwd_unpack: null argument");
+    WRK_REQUIRE(cin == 32, "wrk_conv3x3_fwd_unpack: 32 input channels");
+    if (int rc = conv_check(cin, cout, xPitch, cout)) return rc;
+    WRK_REQUIRE(basePitch >= 3, "wrk_conv3x3_fwd_unpack: base pitch below 3");
     if (n == 0 || h == 0 || wd == 0) return WRK_OK;
-    const uint32_t tilesX = ceil_div(wd, kTW), tilesY = ceil_div(h, kTH);
-    WRK_REQUIRE(n <= 65535, "wrk_conv3x3_fwd: batch too large");
-    if (cout % 64 == 0) {
-        hipLaunchKernelGGL(k_conv_fwd<4>, dim3(tilesX * tilesY, cout / 64, n), dim3(256), 0, as_stream(stream),
-                           (const bf16*)x, xPitch, cin, (const bf16*)w, bias, (bf16*)y, yPitch, cout, relu, h, wd,
-                           tilesX);
-    } else {
-        hipLaunchKernelGGL(k_conv_fwd<2>, dim3(tilesX * tilesY, cout / 32, n), dim3(256), 0, as_stream(stream),
-                           (const bf16*)x, xPitch, cin, (const bf16*)w, bias, (bf16*)y, yPitch, cout, relu, h, wd,
-                           tilesX);
-    }
+    WRK_REQUIRE(n <= 65535, "wrk_conv3x3_fwd_unpack: batch too large");
+    WRK_REQUIRE((uint64_t)n * ceil_div(h, kTH) * ceil_div(wd, kTW) <= 0xFFFFFFFFull,
+                "wrk_conv3x3_fwd_unpack: too many pixel tiles");
+    conv_fwd_launch<1>(x, xPitch, cin, nullptr, 0, 0, w, cout, bias, nullptr, 16, 16, 0u, nullptr, 0, n, h, wd,
+                       as_stream(stream), base, basePitch, out);
     WRK_CHECK_LAUNCH();
     return WRK_OK;
+}
+
+extern "C" int wrk_conv3x3_fwd_pool(const uint16_t* x, uint32_t xPitch, uint32_t cin, const uint16_t* w,
+                                    const float* bias, uint16_t* y, uint32_t yPitch, uint32_t cout, uint32_t relu,
+                                    uint16_t* yPool, uint32_t poolPitch, uint32_t n, uint32_t h, uint32_t wd,
+                                    void* stream) {
+    WRK_REQUIRE(x && w && y && yPool, "wrk_conv3x3_fwd_pool: null argument");
+    if (int rc = conv_check(cin, cout, xPitch, yPitch)) return rc;
+    WRK_REQUIRE(poolPitch % 8 == 0 && poolPitch >= cout, "wrk_conv3x3_fwd_pool: bad pooled-output pitch");
+    WRK_REQUIRE(h % 2 == 0 && wd % 2 == 0, "wrk_conv3x3_fwd_pool: height and width must be even");
+    return conv_fwd(x, xPitch, cin, nullptr, 0, 0, w, cout, bias, y, yPitch, cout, relu, yPool, poolPitch, n, h, wd,
+                    stream);
 }
 
 static void wgrad_plan(uint32_t cin, uint32_t cout, uint32_t n, uint32_t h, uint32_t w, uint32_t* splits,

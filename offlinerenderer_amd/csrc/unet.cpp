@@ -173,11 +173,9 @@ public:
         wrk_ok(wrk_unet_pack(color, albedo, normal, noise_, srcH, srcW, origins, featCount_ ? feat_.f() : nullptr,
                              featCount_, in_.buf.h(), n, h, w, s),
                "wrk_unet_pack");
-        runForward(s);
+        runForward(s, false, out);
         if (out && head_)
             wrk_ok(wrk_kpn_apply(pred_.buf.h(), pred_.c, color, srcH, srcW, origins, out, n, h, w, s), "wrk_kpn_apply");
-        else if (out)
-            wrk_ok(wrk_unet_unpack(pred_.buf.h(), 32, in_.buf.h(), 32, out, (uint64_t)n * h * w, s), "wrk_unet_unpack");
     }
 
     float trainStep(const float* color, const float* albedo, const float* normal, const float* target,
@@ -188,7 +186,7 @@ public:
         wrk_ok(wrk_unet_pack(color, albedo, normal, noise_, srcH, srcW, origins, featCount_ ? feat_.f() : nullptr,
                              featCount_, in_.buf.h(), n, h, w, s),
                "wrk_unet_pack");
-        runForward(s);
+        runForward(s, true, nullptr);
         if (head_)
             wrk_ok(wrk_kpn_loss(pred_.buf.h(), pred_.c, color, target, srcH, srcW, origins, target2, gPred_.buf.h(), n,
                                 h, w, lossMode, lossPartial_.f(), s),
@@ -253,6 +251,14 @@ private:
 
     void conv(int i, const T& x, const T& y, hipStream_t s) {
         const ConvSpec& c = conv_[i];
+        if (i == 14 && head_ == 0) {
+            // the residual head reads channels 0..2 of its output: compute
+            // the first 16 (the others stay as they are)
+            wrk_ok(wrk_conv3x3_fwd_first(x.p, x.pitch, c.cin, wb_[i]->h(), P_.f() + c.bOff, y.p, y.pitch, c.cout, 3u,
+                                         c.relu ? 1u : 0u, n_, H(c.lvl), W(c.lvl), s),
+                   "wrk_conv3x3_fwd_first");
+            return;
+        }
         wrk_ok(wrk_conv3x3_fwd(x.p, x.pitch, c.cin, wb_[i]->h(), P_.f() + c.bOff, y.p, y.pitch, c.cout,
                                c.relu ? 1u : 0u, n_, H(c.lvl), W(c.lvl), s),
                "wrk_conv3x3_fwd");
@@ -271,40 +277,62 @@ private:
     void reluBack(const T& g, const T& y, hipStream_t s) {
         wrk_ok(wrk_relu_bwd(g.p, g.pitch, y.p, y.pitch, g.c, npix(g.lvl), s), "wrk_relu_bwd");
     }
-    void pool(const T& x, const T& y, hipStream_t s) {
-        wrk_ok(wrk_pool2_fwd(x.p, x.pitch, x.c, y.p, y.pitch, n_, H(x.lvl), W(x.lvl), s), "wrk_pool2_fwd");
+    // encoder convolution i writing its skip y and, from the same epilogue,
+    // the 2x2 average pool of y (the next level's input)
+    void convPool(int i, const T& x, const T& y, const T& pooled, hipStream_t s) {
+        const ConvSpec& c = conv_[i];
+        wrk_ok(wrk_conv3x3_fwd_pool(x.p, x.pitch, c.cin, wb_[i]->h(), P_.f() + c.bOff, y.p, y.pitch, c.cout,
+                                    c.relu ? 1u : 0u, pooled.p, pooled.pitch, n_, H(c.lvl), W(c.lvl), s),
+               "wrk_conv3x3_fwd_pool");
     }
     void up(const T& x, const T& y, hipStream_t s) {
         wrk_ok(wrk_up2_fwd(x.p, x.pitch, x.c, y.p, y.pitch, n_, H(y.lvl), W(y.lvl), s), "wrk_up2_fwd");
     }
+    // decoder convolution i over [2x upsampled low | skip] (cat holds the
+    // skip after low.c channels): training materialises the upsampled half,
+    // which the weight gradient reads; inference reads low directly
+    void upConv(int i, const T& low, const T& cat, const T& y, bool training, hipStream_t s) {
+        const ConvSpec& c = conv_[i];
+        if (training) {
+            up(low, T{cat.p, cat.pitch, low.c, cat.lvl}, s);
+            conv(i, cat, y, s);
+            return;
+        }
+        wrk_ok(wrk_conv3x3_fwd_up(cat.p, cat.pitch, c.cin, low.p, low.pitch, low.c, wb_[i]->h(), P_.f() + c.bOff, y.p,
+                                  y.pitch, c.cout, c.relu ? 1u : 0u, n_, H(c.lvl), W(c.lvl), s),
+               "wrk_conv3x3_fwd_up");
+    }
 
-    void runForward(hipStream_t s) {
+    // rgb (inference, residual head): the output convolution writes the
+    // unpacked image there instead of pred_
+    void runForward(hipStream_t s, bool training, float* rgb = nullptr) {
         // encoder; each level's second convolution writes its skip slice
         conv(0, view(in_), view(a0_), s);
         const T e0 = view(cat0_, 64, 32);
-        conv(1, view(a0_), e0, s);
-        pool(e0, view(p1_), s);
+        convPool(1, view(a0_), e0, view(p1_), s);
         conv(2, view(p1_), view(a1_), s);
         const T e1 = view(cat1_, 96, 64);
-        conv(3, view(a1_), e1, s);
-        pool(e1, view(p2_), s);
+        convPool(3, view(a1_), e1, view(p2_), s);
         conv(4, view(p2_), view(a2_), s);
         const T e2 = view(cat2_, 128, 96);
-        conv(5, view(a2_), e2, s);
-        pool(e2, view(p3_), s);
+        convPool(5, view(a2_), e2, view(p3_), s);
         conv(6, view(p3_), view(a3_), s);
         conv(7, view(a3_), view(b3_), s);
         // decoder: [upsampled | skip]
-        up(view(b3_), view(cat2_, 0, 128), s);
-        conv(8, view(cat2_), view(t2_), s);
+        upConv(8, view(b3_), view(cat2_), view(t2_), training, s);
         conv(9, view(t2_), view(d2_), s);
-        up(view(d2_), view(cat1_, 0, 96), s);
-        conv(10, view(cat1_), view(t1_), s);
+        upConv(10, view(d2_), view(cat1_), view(t1_), training, s);
         conv(11, view(t1_), view(d1_), s);
-        up(view(d1_), view(cat0_, 0, 64), s);
-        conv(12, view(cat0_), view(t0_), s);
+        upConv(12, view(d1_), view(cat0_), view(t0_), training, s);
         conv(13, view(t0_), view(d0_), s);
-        conv(14, view(d0_), view(pred_), s);
+        if (rgb && head_ == 0) {
+            const ConvSpec& c = conv_[14];
+            wrk_ok(wrk_conv3x3_fwd_unpack(d0_.buf.h(), d0_.c, c.cin, wb_[14]->h(), P_.f() + c.bOff, c.cout, in_.buf.h(),
+                                          32, rgb, n_, H(0), W(0), s),
+                   "wrk_conv3x3_fwd_unpack");
+        } else {
+            conv(14, view(d0_), view(pred_), s);
+        }
     }
 
     void runBackward(hipStream_t s) {

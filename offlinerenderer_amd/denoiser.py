@@ -44,8 +44,16 @@ def layer_slices(head: int = 0):
 
 
 def flops_per_pixel(head: int = 0) -> float:
-    """Forward FLOPs per full-resolution pixel (2 per multiply-add)."""
-    return sum(2.0 * 9 * cin * cout / (4 ** lvl) for cin, cout, lvl in layers(head))
+    """Forward FLOPs the network executes per full-resolution pixel (2 per
+    multiply-add). The residual head's output convolution computes its first
+    16 output channels (wrk_conv3x3_fwd_first), of which 3 are read."""
+    ls = layers(head)
+    total = 0.0
+    for k, (cin, cout, lvl) in enumerate(ls):
+        if head == 0 and k == len(ls) - 1:
+            cout = 16
+        total += 2.0 * 9 * cin * cout / (4 ** lvl)
+    return total
 
 
 def _err():

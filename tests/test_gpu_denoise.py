@@ -35,6 +35,8 @@ def _rand(rng, shape, scale=1.0):
     (1, 12, 20, 96, 64, 0, 128, 96),     # partial tiles, pitched input and output slices
     (1, 24, 16, 224, 96, 1, 224, 96),
     (3, 8, 8, 64, 128, 1, 64, 128),
+    (2, 264, 528, 32, 32, 1, 64, 96),    # 32 input channels: several tiles per block, channel slices
+    (1, 60, 100, 32, 64, 0, 32, 64),     # 32 input channels, 64 output channels, partial tiles
 ])
 def test_conv_fwd_matches_torch(gpu, n, h, w, cin, cout, relu, xpitch, ypitch):
     rng = np.random.default_rng(cin * 7 + cout)
@@ -57,6 +59,111 @@ def test_conv_fwd_matches_torch(gpu, n, h, w, cin, cout, relu, xpitch, ypitch):
     assert np.all(err <= np.abs(ref) * 2.0 ** -8 + 1e-4 * np.abs(ref).max() + 1e-6), err.max()
     if ypitch > cout:
         assert np.all(_host_bits(yd)[..., cout:] == 0)
+
+
+@pytest.mark.parametrize("cin,cout,used", [(32, 32, 3), (96, 32, 20), (64, 64, 16)])
+def test_conv_fwd_first_channels(gpu, cin, cout, used):
+    """wrk_conv3x3_fwd_first computes output channels [0, ceil16(used)) of a
+    convolution with cout output channels, as wrk_conv3x3_fwd does, and
+    leaves the rest of y untouched."""
+    n, h, w = 2, 40, 72
+    rng = np.random.default_rng(cin + cout + used)
+    x = _rand(rng, (n, h, w, cin))
+    wt = _rand(rng, (9, cout, cin), 1.0 / np.sqrt(9 * cin))
+    b = rng.normal(size=cout).astype(np.float32) * 0.1
+    xd, wd, bd = _dev_bits(x), _dev_bits(wt), torch.from_numpy(b).cuda()
+    full = torch.zeros((n, h, w, cout), dtype=torch.int16, device="cuda")
+    G.check(wr.lib().wrk_conv3x3_fwd(G.vp(xd), cin, cin, G.vp(wd), G.vp(bd), G.vp(full), cout, cout, 1, n, h, w,
+                                     G.stream()))
+    part = torch.full((n, h, w, cout), 0x7F7F, dtype=torch.int16, device="cuda")
+    G.check(wr.lib().wrk_conv3x3_fwd_first(G.vp(xd), cin, cin, G.vp(wd), G.vp(bd), G.vp(part), cout, cout, used, 1,
+                                           n, h, w, G.stream()))
+    torch.cuda.synchronize()
+    c = min(cout, (used + 15) // 16 * 16)
+    f, p_ = full.cpu().numpy(), part.cpu().numpy()
+    assert np.array_equal(p_[..., :c], f[..., :c])
+    assert np.all(p_[..., c:] == 0x7F7F)
+
+
+@pytest.mark.parametrize("n,h,w,cup,cskip,cout,uppitch", [(2, 36, 52, 64, 32, 32, 64), (1, 24, 40, 96, 64, 64, 128),
+                                                           (1, 20, 36, 128, 96, 96, 128)])
+def test_conv_fwd_up_equals_upsample_then_conv(gpu, n, h, w, cup, cskip, cout, uppitch):
+    """wrk_conv3x3_fwd_up reads the upsampled channels of a decoder input
+    from the half-resolution tensor: bit for bit wrk_up2_fwd into the
+    concatenation followed by wrk_conv3x3_fwd (partial tiles, pitches)."""
+    cin = cup + cskip
+    rng = np.random.default_rng(cin + cout)
+    low = _rand(rng, (n, h // 2, w // 2, uppitch))
+    cat = _rand(rng, (n, h, w, cin))
+    wt = _rand(rng, (9, cout, cin), 1.0 / np.sqrt(9 * cin))
+    b = rng.normal(size=cout).astype(np.float32) * 0.1
+    lowd, wd, bd = _dev_bits(low), _dev_bits(wt), torch.from_numpy(b).cuda()
+    ref_in = _dev_bits(cat)
+    G.check(wr.lib().wrk_up2_fwd(G.vp(lowd), uppitch, cup, G.vp(ref_in), cin, n, h, w, G.stream()))
+    ref = torch.zeros((n, h, w, cout), dtype=torch.int16, device="cuda")
+    G.check(wr.lib().wrk_conv3x3_fwd(G.vp(ref_in), cin, cin, G.vp(wd), G.vp(bd), G.vp(ref), cout, cout, 1, n, h, w,
+                                     G.stream()))
+    # the fused form never reads the first cup channels of its input
+    junk = cat.copy()
+    junk[..., :cup] = np.float32(1e4)
+    got_in = _dev_bits(junk)
+    got = torch.zeros((n, h, w, cout), dtype=torch.int16, device="cuda")
+    G.check(wr.lib().wrk_conv3x3_fwd_up(G.vp(got_in), cin, cin, G.vp(lowd), uppitch, cup, G.vp(wd), G.vp(bd),
+                                        G.vp(got), cout, cout, 1, n, h, w, G.stream()))
+    torch.cuda.synchronize()
+    assert np.array_equal(got.cpu().numpy(), ref.cpu().numpy())
+    assert np.count_nonzero(ref.cpu().numpy()) > ref.numel() // 4
+
+
+@pytest.mark.parametrize("n,h,w,cin,cout,ypitch,ppitch", [(2, 36, 52, 32, 32, 96, 32), (1, 24, 40, 64, 64, 160, 64),
+                                                          (1, 20, 36, 96, 96, 224, 96), (1, 18, 34, 32, 64, 64, 64)])
+def test_conv_fwd_pool_equals_conv_then_pool(gpu, n, h, w, cin, cout, ypitch, ppitch):
+    """wrk_conv3x3_fwd_pool writes the convolution and its 2x2 average pool
+    from one epilogue: bit for bit wrk_conv3x3_fwd then wrk_pool2_fwd (both
+    kernels: 32 and more input channels; partial tiles; pitched slices)."""
+    rng = np.random.default_rng(cin + cout + h)
+    x = _rand(rng, (n, h, w, cin))
+    wt = _rand(rng, (9, cout, cin), 1.0 / np.sqrt(9 * cin))
+    b = rng.normal(size=cout).astype(np.float32) * 0.1
+    xd, wd, bd = _dev_bits(x), _dev_bits(wt), torch.from_numpy(b).cuda()
+    y_ref = torch.zeros((n, h, w, ypitch), dtype=torch.int16, device="cuda")
+    p_ref = torch.zeros((n, h // 2, w // 2, ppitch), dtype=torch.int16, device="cuda")
+    G.check(wr.lib().wrk_conv3x3_fwd(G.vp(xd), cin, cin, G.vp(wd), G.vp(bd), G.vp(y_ref), ypitch, cout, 1, n, h, w,
+                                     G.stream()))
+    G.check(wr.lib().wrk_pool2_fwd(G.vp(y_ref), ypitch, cout, G.vp(p_ref), ppitch, n, h, w, G.stream()))
+    y = torch.zeros_like(y_ref)
+    pooled = torch.zeros_like(p_ref)
+    G.check(wr.lib().wrk_conv3x3_fwd_pool(G.vp(xd), cin, cin, G.vp(wd), G.vp(bd), G.vp(y), ypitch, cout, 1,
+                                          G.vp(pooled), ppitch, n, h, w, G.stream()))
+    torch.cuda.synchronize()
+    assert np.array_equal(y.cpu().numpy(), y_ref.cpu().numpy())
+    assert np.array_equal(pooled.cpu().numpy(), p_ref.cpu().numpy())
+    assert np.count_nonzero(p_ref.cpu().numpy()) > p_ref.numel() // 8
+
+
+@pytest.mark.parametrize("n,h,w", [(2, 36, 52), (1, 1080 // 8, 1920 // 8)])
+def test_conv_fwd_unpack_equals_conv_then_unpack(gpu, n, h, w):
+    """wrk_conv3x3_fwd_unpack: the residual head's output convolution and
+    wrk_unet_unpack from one epilogue, bit for bit the two steps."""
+    cin, cout = 32, 32
+    rng = np.random.default_rng(h * w)
+    x = _rand(rng, (n, h, w, cin))
+    wt = _rand(rng, (9, cout, cin), 0.3 / np.sqrt(9 * cin))
+    b = rng.normal(size=cout).astype(np.float32) * 0.1
+    base = _rand(rng, (n, h, w, 32), 0.5)
+    xd, wd, bd, based = _dev_bits(x), _dev_bits(wt), torch.from_numpy(b).cuda(), _dev_bits(base)
+    pred = torch.zeros((n, h, w, cout), dtype=torch.int16, device="cuda")
+    G.check(wr.lib().wrk_conv3x3_fwd_first(G.vp(xd), cin, cin, G.vp(wd), G.vp(bd), G.vp(pred), cout, cout, 3, 0, n, h,
+                                           w, G.stream()))
+    ref = torch.zeros((n * h * w, 3), dtype=torch.float32, device="cuda")
+    G.check(wr.lib().wrk_unet_unpack(G.vp(pred), cout, G.vp(based), 32, G.vp(ref), n * h * w, G.stream()))
+    got = torch.full((n * h * w, 3), -1.0, dtype=torch.float32, device="cuda")
+    G.check(wr.lib().wrk_conv3x3_fwd_unpack(G.vp(xd), cin, cin, G.vp(wd), G.vp(bd), cout, G.vp(based), 32, G.vp(got),
+                                            n, h, w, G.stream()))
+    torch.cuda.synchronize()
+    r, g = ref.cpu().numpy(), got.cpu().numpy()
+    assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
+    assert np.count_nonzero(r) > r.size // 4
 
 
 @pytest.mark.parametrize("n,h,w,cin,cout", [(2, 16, 32, 32, 32), (1, 12, 40, 96, 64), (1, 8, 16, 224, 96)])
