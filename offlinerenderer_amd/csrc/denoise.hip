@@ -19,6 +19,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "wrk_common.h"
 
@@ -855,13 +858,28 @@ static int conv_check(uint32_t cin, uint32_t cout, uint32_t xPitch, uint32_t yPi
 
 // Resident blocks of a persistent launch: blocks per CU the kernel's
 // registers and LDS allow, times the CUs, shared over gridY channel tiles.
+// Queried once per device and kernel.
 static uint32_t resident_blocks(const void* kernel, uint32_t gridY) {
-    int dev = 0, cus = 0, perCu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, 256, 0) != hipSuccess || cus <= 0 || perCu <= 0)
-        return 1024u / gridY + 1u;
-    const uint32_t total = (uint32_t)cus * (uint32_t)perCu;
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, uint32_t> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1024u / gridY + 1u;
+    uint32_t total = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find({dev, kernel});
+        if (it != cache.end()) {
+            total = it->second;
+        } else {
+            int cus = 0, perCu = 0;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, 256, 0) != hipSuccess || cus <= 0 ||
+                perCu <= 0)
+                return 1024u / gridY + 1u;
+            total = (uint32_t)cus * (uint32_t)perCu;
+            cache[{dev, kernel}] = total;
+        }
+    }
     return total > gridY ? total / gridY : 1u;
 }
 
