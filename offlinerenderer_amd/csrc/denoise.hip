@@ -168,14 +168,14 @@ __device__ __forceinline__ void conv_tile_store(const f32x4 (&acc)[2][CT], const
 // channels below cUp are the 2x nearest upsampling of xUp ((H/2, W/2),
 // pitch upPitch), read at (y/2, x/2) while staging, so a decoder's
 // [upsampled | skip] input needs no upsampled copy; the others come from x.
+// At least 3 waves per SIMD: without the bound the 64-channel form takes
+// 174 registers and runs 2 (its layers then take ~17 us longer each).
 template <int CT>  // output-channel tiles of 16 per block
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_conv_fwd(const bf16* __restrict__ x, uint32_t xPitch, uint32_t cin,
-                                                  const bf16* __restrict__ xUp, uint32_t upPitch, uint32_t cUp,
-                                                  const bf16* __restrict__ w, uint32_t wCout,
-                                                  const float* __restrict__ bias, bf16* __restrict__ y,
-                                                  uint32_t yPitch, uint32_t cout, uint32_t relu, uint32_t H,
-                                                  uint32_t W, uint32_t tilesX, bf16* __restrict__ yPool,
-                                                  uint32_t poolPitch) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
+void k_conv_fwd(const bf16* __restrict__ x, uint32_t xPitch, uint32_t cin, const bf16* __restrict__ xUp,
+                uint32_t upPitch, uint32_t cUp, const bf16* __restrict__ w, uint32_t wCout,
+                const float* __restrict__ bias, bf16* __restrict__ y, uint32_t yPitch, uint32_t cout, uint32_t relu,
+                uint32_t H, uint32_t W, uint32_t tilesX, bf16* __restrict__ yPool, uint32_t poolPitch) {
     __shared__ __attribute__((aligned(16))) bf16 sIn[kIH * kIW * 32];
     __shared__ __attribute__((aligned(16))) bf16 sW[9 * CT * 16 * 32];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -268,15 +268,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 // loads the next tile's input into registers while the MFMAs and the store
 // of the current one run. With one chunk, the kernel above has nothing to
 // overlap its loads with and fetches the weights again for every tile.
+// At least 4 waves per SIMD: at 3 it ran no faster than that kernel.
 template <int CT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_conv_fwd_c32(const bf16* __restrict__ x, uint32_t xPitch,
-                                                      const bf16* __restrict__ w, uint32_t wCout,
-                                                      const float* __restrict__ bias, bf16* __restrict__ y,
-                                                      uint32_t yPitch, uint32_t cout, uint32_t relu, uint32_t H,
-                                                      uint32_t W, uint32_t tilesX, uint32_t tilesPerImage,
-                                                      uint32_t nTiles, bf16* __restrict__ yPool,
-                                                      uint32_t poolPitch, const bf16* __restrict__ base,
-                                                      uint32_t basePitch, float* __restrict__ rgb) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void k_conv_fwd_c32(const bf16* __restrict__ x, uint32_t xPitch, const bf16* __restrict__ w, uint32_t wCout,
+                    const float* __restrict__ bias, bf16* __restrict__ y, uint32_t yPitch, uint32_t cout, uint32_t relu,
+                    uint32_t H, uint32_t W, uint32_t tilesX, uint32_t tilesPerImage, uint32_t nTiles,
+                    bf16* __restrict__ yPool, uint32_t poolPitch, const bf16* __restrict__ base, uint32_t basePitch,
+                    float* __restrict__ rgb) {
     __shared__ __attribute__((aligned(16))) bf16 sIn[kIH * kIW * 32];
     __shared__ __attribute__((aligned(16))) bf16 sW[9 * CT * 16 * 32];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
