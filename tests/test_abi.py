@@ -83,3 +83,29 @@ def test_null_handles_are_errors_not_crashes():
     L.wr_c_scene_destroy(None)
     L.wr_c_renderer_destroy(None)
     L.wr_c_server_destroy(None)
+
+
+def test_fused_convolution_arguments_are_checked():
+    """The fused convolution entry points reject bad shapes with an error code
+    and a message before any device work (so this runs without a GPU)."""
+    import ctypes as C
+
+    L = wr.lib()
+    p = C.c_void_p(64)  # never dereferenced: every call below fails its checks first
+
+    def err(rc, words):
+        assert rc != 0
+        msg = L.wrk_last_error()
+        assert all(w in msg for w in words), msg
+
+    err(L.wrk_conv3x3_fwd_first(p, 32, 32, p, None, p, 32, 32, 0, 0, 1, 8, 8, None), [b"coutUsed"])
+    err(L.wrk_conv3x3_fwd_first(p, 32, 32, p, None, p, 32, 32, 33, 0, 1, 8, 8, None), [b"coutUsed"])
+    err(L.wrk_conv3x3_fwd_up(p, 96, 96, None, 64, 64, p, None, p, 32, 32, 1, 1, 8, 8, None), [b"null"])
+    err(L.wrk_conv3x3_fwd_up(p, 96, 96, p, 64, 48, p, None, p, 32, 32, 1, 1, 8, 8, None), [b"cUp"])
+    err(L.wrk_conv3x3_fwd_up(p, 96, 96, p, 64, 128, p, None, p, 32, 32, 1, 1, 8, 8, None), [b"cUp"])
+    err(L.wrk_conv3x3_fwd_up(p, 96, 96, p, 32, 64, p, None, p, 32, 32, 1, 1, 8, 8, None), [b"pitch"])
+    err(L.wrk_conv3x3_fwd_up(p, 96, 96, p, 64, 64, p, None, p, 32, 32, 1, 1, 9, 8, None), [b"even"])
+    err(L.wrk_conv3x3_fwd_pool(p, 32, 32, p, None, p, 32, 32, 1, p, 16, 1, 8, 8, None), [b"pitch"])
+    err(L.wrk_conv3x3_fwd_pool(p, 32, 32, p, None, p, 32, 32, 1, p, 32, 1, 8, 7, None), [b"even"])
+    err(L.wrk_conv3x3_fwd_unpack(p, 64, 64, p, None, 32, p, 32, p, 1, 8, 8, None), [b"32 input"])
+    err(L.wrk_conv3x3_fwd_unpack(p, 32, 32, p, None, 32, p, 2, p, 1, 8, 8, None), [b"base pitch"])
