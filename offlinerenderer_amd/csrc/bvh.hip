@@ -802,7 +802,10 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
             // node test became one fma per plane, 3/8, 1/2, 5/8, 3/4 gave
             // material 1901, 1953-1964, 1985-1996, 1974 and large 1189,
             // 1200-1205, 1197-1204, 1159: the cheaper node steps leave the
-            // leaf steps relatively dearer, so the wave waits for more of them).
+            // leaf steps relatively dearer, so the wave waits for more of them;
+            // with both triangle records of a leaf loaded together, 1/2, 5/8,
+            // 3/4 gave material 1989-1995, 2013-2020, 1992-2000 and large
+            // 1199-1224, 1204-1207, 1156-1166).
             const uint32_t nAct = __popcll(__ballot(true));
             const uint32_t nPend = __popcll(__ballot(leaf != kEmptyChild));
             const bool canStep = __ballot(cur != kEmptyChild && !(cur & kLeafBit)) != 0;

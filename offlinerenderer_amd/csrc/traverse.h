@@ -124,17 +124,35 @@ __device__ __forceinline__ bool trace_thread(const Node8* __restrict__ nodes, co
             if (cur & kLeafBit) {
                 const uint32_t first = cur & 0x07FFFFFFu;
                 const uint32_t count = ((cur >> 27) & 0xFu) + 1u;
-                for (uint32_t k = 0; k < count; ++k) {
-                    const float4* tp = tris + (size_t)(first + k) * 3;
-                    const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+                // the first two triangles' records are loaded together (one
+                // memory round trip for a two-triangle leaf), then tested in
+                // leaf order
+                const float4* tp = tris + (size_t)first * 3;
+                const float4 a0 = tp[0], a1 = tp[1], a2 = tp[2];
+                float4 b0 = a0, b1 = a1, b2 = a2;
+                if (count > 1) { b0 = tp[3]; b1 = tp[4]; b2 = tp[5]; }
+                bool found = false;
+                auto test = [&](float4 v0, float4 v1, float4 v2) {
                     float th;
                     if (intersect_tri(rp, v0, v1, v2, tmin, tbest, &th)) {
-                        if (ANY) return true;
                         const uint32_t prim = __float_as_uint(v0.w);
-                        if (th < tbest || prim < (bestPrim & kPrimMask)) {
+                        if (ANY) {
+                            found = true;
+                        } else if (th < tbest || prim < (bestPrim & kPrimMask)) {
                             tbest = th;
                             bestPrim = prim | ((__float_as_uint(v1.w) & 7u) << 28);
                         }
+                    }
+                };
+                test(a0, a1, a2);
+                if (ANY && found) return true;
+                if (count > 1) test(b0, b1, b2);
+                if (ANY && found) return true;
+                if constexpr (kLeafMax > 2) {
+                    for (uint32_t k = 2; k < count; ++k) {
+                        const float4* tk = tris + (size_t)(first + k) * 3;
+                        test(tk[0], tk[1], tk[2]);
+                        if (ANY && found) return true;
                     }
                 }
                 cur = kEmptyChild;
