@@ -96,8 +96,10 @@ constexpr float kBoxPadRel = 9.5367431640625e-07f;  // 2^-20 outward box padding
 #ifndef WRK_STACK_LDS
 #define WRK_STACK_LDS 8
 #endif
+// With lanes refilled in batches of 16, caps of 12, 16 and 24 rays per lane
+// gave material 2059-2063, 2067-2074 and 2073-2078 Mpaths/s, large within noise.
 #ifndef WRK_RAYS_PER_LANE
-#define WRK_RAYS_PER_LANE 12
+#define WRK_RAYS_PER_LANE 24
 #endif
 constexpr uint32_t kStackLds = WRK_STACK_LDS;        // traversal stack entries per lane in LDS (power of 2)
 constexpr uint32_t kStackSpill = 192 - WRK_STACK_LDS; // further entries per lane in scratch
