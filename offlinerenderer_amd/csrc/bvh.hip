@@ -651,10 +651,11 @@ __global__ __launch_bounds__(256) void k_reorder_tris(const uint32_t* __restrict
 #ifndef WRK_TRACE_WAVES
 #define WRK_TRACE_WAVES 8
 #endif
-#ifndef WRK_REFILL_MIN
-#define WRK_REFILL_MIN 16
-#endif
-static_assert(WRK_REFILL_MIN >= 1 && WRK_REFILL_MIN <= 64, "a refill threshold of 1..64 lanes");
+// Lanes a wave refills at once (see the refill in k_trace): 16 for scenes
+// whose nodes and triangle records fit the 256 MB MALL, 32 for larger ones,
+// whose node fetches go to HBM and make each refill's wait longer.
+constexpr uint32_t kRefillMinResident = 16, kRefillMinLarge = 32;
+constexpr uint64_t kMallBytes = 256ull << 20;
 #ifndef WRK_LEAF_BATCH_NUM
 #define WRK_LEAF_BATCH_NUM 5
 #endif
@@ -734,7 +735,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
                                                const uint32_t* __restrict__ queue, uint32_t n,
                                                const uint32_t* __restrict__ countDev, uint2* __restrict__ hits,
                                                uint8_t* __restrict__ occl, unsigned long long* __restrict__ fetches,
-                                               uint32_t raysPerLane, TraceResolve R) {
+                                               uint32_t raysPerLane, uint32_t refillMin, TraceResolve R) {
     __shared__ uint32_t sNode[kStackLds][256];
     __shared__ float sT[kStackLds][256];
     __shared__ uint32_t sQi[(ANY && !RESOLVE) ? 1 : 256];  // queue position of each lane's ray
@@ -762,7 +763,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
 
     for (;;) {
         // refill lanes without a ray from the wave's chunk
-        // once WRK_REFILL_MIN lanes wait for one (or all do): the wave then
+        // once refillMin lanes wait for one (or all do): the wave then
         // waits for the new rays' loads (queue entry, then ray) once for
         // several lanes instead of at nearly every step (thresholds 1, 4,
         // 16, 32 gave material 2010-2014, 2043-2046, 2045-2047, 2032-2039
@@ -771,7 +772,7 @@ void k_trace(const void* __restrict__ nodesIn, const float4* __restrict__ tris,
         // of time, read across lanes at refill, measured 2% slower)
         uint64_t need = __ballot(!active);
         const uint32_t nNeed = (uint32_t)__popcll(need);
-        if (nNeed >= WRK_REFILL_MIN && cursor < chunkEnd) {
+        if (nNeed >= refillMin && cursor < chunkEnd) {
             // this lane's rank among the lanes that need a ray
             const uint32_t qi = cursor + __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
@@ -1204,6 +1205,8 @@ static int launch_trace(const wrk_bvh* bvh, const void* org4, const void* dir4, 
     const uint32_t rpl = rays_per_lane(n);
     WRK_REQUIRE((uint64_t)n + 256ull * rpl <= 0xFFFFFFFFull, "wrk_trace: too many rays for one launch");
     dim3 grid(ceil_div(n, 256u * rpl)), block(256);
+    const uint64_t sceneBytes = (uint64_t)bvh->numNodes * bvh->nodeBytes + (uint64_t)bvh->numTris * 48u;
+    const uint32_t refill = sceneBytes > kMallBytes ? kRefillMinLarge : kRefillMinResident;
     hipStream_t s = as_stream(stream);
     const void* nd = (const void*)bvh->nodes;
     const float4* tr = (const float4*)bvh->tris;
@@ -1213,18 +1216,18 @@ static int launch_trace(const wrk_bvh* bvh, const void* org4, const void* dir4, 
     if (resolve) {
         if (fetches)
             hipLaunchKernelGGL((k_trace<ANY, true, 8, true>), grid, block, 0, s, nd, tr, o4, d4, queue, n, countDev,
-                               h2, occl, fetches, rpl, *resolve);
+                               h2, occl, fetches, rpl, refill, *resolve);
         else
             hipLaunchKernelGGL((k_trace<ANY, false, 8, true>), grid, block, 0, s, nd, tr, o4, d4, queue, n, countDev,
-                               h2, occl, nullptr, rpl, *resolve);
+                               h2, occl, nullptr, rpl, refill, *resolve);
     } else {
         const TraceResolve none{};
         if (fetches)
             hipLaunchKernelGGL((k_trace<ANY, true, 8, false>), grid, block, 0, s, nd, tr, o4, d4, queue, n, countDev,
-                               h2, occl, fetches, rpl, none);
+                               h2, occl, fetches, rpl, refill, none);
         else
             hipLaunchKernelGGL((k_trace<ANY, false, 8, false>), grid, block, 0, s, nd, tr, o4, d4, queue, n, countDev,
-                               h2, occl, nullptr, rpl, none);
+                               h2, occl, nullptr, rpl, refill, none);
     }
     WRK_CHECK_LAUNCH();
     return WRK_OK;
