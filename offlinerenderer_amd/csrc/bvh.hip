@@ -1180,7 +1180,8 @@ extern "C" int wrk_bvh_build(const float* vertices, const uint32_t* indices, uin
 // busy than one ray per lane (fewer than half of the lanes are active per
 // VALU instruction in the closest-hit kernel). At 39M rays per launch 8 rays
 // per lane gave +12% over one; 12 gave another 3.5% of closest-hit time,
-// 16 no more, and at 30M rays the cap is within 1% of 8.
+// 16 no more, and at 30M rays the cap is within 1% of 8. Once lanes were
+// refilled in batches, a cap of 24 gave another 1% (kRaysPerLane).
 static uint32_t rays_per_lane(uint32_t n) {
     static int cus = 0;
     if (cus == 0) {
